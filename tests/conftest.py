@@ -1,0 +1,22 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs a real MI355X (HIP device)")
+
+
+@pytest.fixture(scope="session")
+def oracle_mod():
+    from oracle import pyoracle
+    try:
+        pyoracle.lib()
+    except Exception as e:  # pragma: no cover - build step missing
+        pytest.skip("oracle not built: %s" % e)
+    return pyoracle

@@ -1,0 +1,281 @@
+"""Pins for the CPU oracle (oracle/compact_oracle.c) before anything trusts it.
+
+Each check is chosen so that a plausible slip (a dropped boundary row, a
+wrong sign in a folded symmetric weight, a transposed band, a missing +v or
+metric) fails at least one of them:
+
+* weights and LU factors/solves: bit-for-bit against the reference's own
+  stencils.f90 / pentadiagonal.f90 compiled into oracle/_ref;
+* periodic operators: closed-form transfer functions (Fourier symbols) of
+  the published interior weights, exact per mode;
+* bounded and symmetric operators: dense numpy solves of the assembled
+  band, built here independently of the oracle's LU and RHS forms;
+* the whole chain: the reference-held regression value of
+  examples/unit_test.py (tests/cases/testUnit.py:3).
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from meshgen import mesh
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+OPS = ["D1", "D2", "D8", "SF", "GF"]
+
+
+def _golden():
+    with open(os.path.join(GOLDEN, "reference_scalars.json")) as f:
+        return json.load(f)
+
+
+# ---------------------------------------------------------------------------
+# Bit-for-bit against the reference's own Fortran (oracle/_ref)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("op", range(5))
+def test_weights_bitexact_vs_reference_build(oracle_mod, op):
+    po = oracle_mod
+    if not po.ref_available():
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    a, b = po.weights(op), po.ref_weights(op)
+    for key in ("nol", "nor", "ncl", "ncr", "null_option", "implicit_op"):
+        assert a[key] == b[key], (OPS[op], key)
+    for key in ("ali", "ari", "alb1", "alb2", "arb1", "arb2"):
+        assert np.array_equal(a[key], b[key]), (OPS[op], key)
+
+
+def _raw_band(po, op, n, periodic):
+    """Unfactored band (n, nal) from the weight tables, numpy-side."""
+    w = po.weights(op)
+    ncl = w["ncl"]
+    nal = ncl + 4 if periodic else ncl
+    al = np.zeros((n, nal))
+    al[:, :ncl] = w["ali"][:ncl]
+    if not periodic:
+        al[:4, :ncl] = w["alb1"][1, :, :ncl]
+        al[n - 4:, :ncl] = w["alb2"][1, :, :ncl]
+    return al
+
+
+@pytest.mark.parametrize("op", [0, 1, 2, 3])
+@pytest.mark.parametrize("n", [8, 33, 100])
+@pytest.mark.parametrize("periodic", [True, False])
+def test_lu_vs_reference_build(oracle_mod, op, n, periodic):
+    po = oracle_mod
+    if not po.ref_available():
+        pytest.skip("oracle/_ref not built (reference tree absent)")
+    import ctypes
+    al_ora, _ = po.assemble(op, n, periodic)
+    raw = np.asfortranarray(_raw_band(po, op, n, periodic))
+    r = po.ref_lib()
+    ptr = raw.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    (r.ref_ppentLUD1 if periodic else r.ref_bpentLUD1)(ptr, n)
+    assert np.array_equal(raw, al_ora), "LU factors differ from the reference build"
+    # the per-line 3-D sweep, x direction over an (n,3,2) block
+    rng = np.random.default_rng(7 + n)
+    rhs = np.asfortranarray(rng.standard_normal((n, 3, 2)))
+    ref = rhs.copy(order="F")
+    (r.ref_ppentLUS3 if periodic else r.ref_bpentLUS3)(
+        1, ptr, ref.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n, n, 3, 2)
+    c = np.asfortranarray(al_ora)
+    for j in range(3):
+        for k in range(2):
+            line = np.ascontiguousarray(rhs[:, j, k])
+            po.lib().ora_line_solve(int(periodic), c.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                    line.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), n)
+            assert np.array_equal(line, ref[:, j, k])
+
+
+# ---------------------------------------------------------------------------
+# Closed-form transfer functions (periodic)
+# ---------------------------------------------------------------------------
+def _symbols(theta):
+    """Published interior symbols, independent of the oracle's code."""
+    c, s = np.cos, np.sin
+    d1 = 2 * (6.375 * s(theta) + 1.515 * s(2 * theta) + .015 * s(3 * theta)) / (9 + 9 * c(theta) + .9 * c(2 * theta))
+    d2 = (-28670 + 19170 * c(theta) + 9342 * c(2 * theta) + 158 * c(3 * theta)) / (16182 + 12024 * c(theta) + 774 * c(2 * theta))
+    d8 = (4200 - 6720 * c(theta) + 3360 * c(2 * theta) - 960 * c(3 * theta) + 120 * c(4 * theta)) / (29 + 28 * c(theta) + 3 * c(2 * theta))
+    sf = (.99965 + 1.33304 * c(theta) + .33348 * c(2 * theta) + 8e-5 * c(3 * theta) - 1e-5 * c(4 * theta)) / (1 + 1.33248 * c(theta) + .33376 * c(2 * theta))
+    a, b, cc, d, e = 3565 / 10368, 3091 / 12960, 1997 / 25920, 149 / 12960, 107 / 103680
+    gf = a + 2 * (b * c(theta) + cc * c(2 * theta) + d * c(3 * theta) + e * c(4 * theta))
+    return d1, d2, d8, sf, gf
+
+
+@pytest.mark.parametrize("n", [16, 32, 100])
+def test_periodic_closed_form(oracle_mod, n):
+    po = oracle_mod
+    L = 2 * np.pi
+    h = L / n
+    (X, Y, Z), dd = mesh([(0.0, L * (n - 1) / n, n)] * 3)
+    assert np.allclose(dd, h, rtol=1e-14)
+    P = po.Oracle((n, n, n), dd, (True, True, True))
+    rng = np.random.default_rng(n)
+    ks = rng.integers(0, n // 2 + 1, size=6)
+    ks[0] = n // 2  # always include Nyquist
+    amps = rng.standard_normal(6)
+    phis = rng.uniform(0, 2 * np.pi, 6)
+    for d, C in enumerate((X, Y, Z)):
+        f = sum(A * np.cos(k * C + p) for A, k, p in zip(amps, ks, phis))
+        th = ks * h
+        s1, s2, s8, ssf, sgf = _symbols(th)
+        ex = {
+            po.D1: sum(-A * s * np.sin(k * C + p) / h for A, s, k, p in zip(amps, s1, ks, phis)),
+            po.D2: sum(A * s * np.cos(k * C + p) / h ** 2 for A, s, k, p in zip(amps, s2, ks, phis)),
+            po.D8: sum(A * s * np.cos(k * C + p) for A, s, k, p in zip(amps, s8, ks, phis)),
+            po.SF: sum(A * s * np.cos(k * C + p) for A, s, k, p in zip(amps, ssf, ks, phis)),
+            po.GF: sum(A * s * np.cos(k * C + p) for A, s, k, p in zip(amps, sgf, ks, phis)),
+        }
+        for op, want in ex.items():
+            got = P.apply(op, d, f)
+            scale = max(np.abs(want).max(), 1e-300)
+            assert np.abs(got - want).max() / scale < 5e-12, (OPS[op], d, n)
+
+
+def test_filters_kill_nyquist(oracle_mod):
+    po = oracle_mod
+    n = 32
+    (X, _, _), dd = mesh([(0.0, 2 * np.pi * (n - 1) / n, n), (0, 1, 1), (0, 1, 1)])
+    P = po.Oracle((n, 1, 1), dd, (True, False, False))
+    f = np.cos(n / 2 * X)
+    for op in (po.SF, po.GF):
+        assert np.abs(P.apply(op, 0, f)).max() < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# Bounded / symmetric: dense solves of the assembled band
+# ---------------------------------------------------------------------------
+def _dense(po, op, n, periodic, v1, v2):
+    w = po.weights(op)
+    ncl, ncr = w["ncl"], w["ncr"]
+    hl, hr = ncl // 2, ncr // 2
+    A = np.zeros((n, n))
+    B = np.zeros((n, n))
+    for i in range(n):
+        al, ar = w["ali"][:ncl], w["ari"][:ncr]
+        if not periodic and i < 4:
+            al, ar = w["alb1"][v1 + 1, i, :ncl], w["arb1"][v1 + 1, i, :ncr]
+        elif not periodic and i >= n - 4:
+            j = i - (n - 4)
+            al, ar = w["alb2"][v2 + 1, j, :ncl], w["arb2"][v2 + 1, j, :ncr]
+        for c in range(ncl):
+            col = i + c - hl
+            if periodic:
+                A[i, col % n] += al[c]
+            elif 0 <= col < n:
+                A[i, col] += al[c]
+            else:
+                assert al[c] == 0.0
+        for c in range(ncr):
+            col = i + c - hr
+            if periodic:
+                B[i, col % n] += ar[c]
+            elif 0 <= col < n:
+                B[i, col] += ar[c]
+            else:
+                assert ar[c] == 0.0
+    return A, B
+
+
+@pytest.mark.parametrize("op", range(5))
+@pytest.mark.parametrize("n", [9, 24, 61])
+@pytest.mark.parametrize("bc", ["periodic", "none", "symm"])
+def test_dense_solve(oracle_mod, op, n, bc):
+    po = oracle_mod
+    periodic = bc == "periodic"
+    var = 1 if bc == "symm" else 0
+    A, B = _dense(po, op, n, periodic, var, var)
+    h = 0.37
+    rng = np.random.default_rng(n * 10 + op)
+    v = rng.standard_normal((n, 2, 3)) + 3.0
+    want = np.einsum("ij,jab->iab", np.linalg.solve(A, B), v)
+    if op in (po.SF, po.GF):
+        want = want + v
+    if op == po.D1:
+        want = want / h
+    if op == po.D2:
+        want = want / h ** 2
+    sym = None
+    if bc == "symm":
+        sym = [[True, True], [False, False], [False, False]]
+    P = po.Oracle((n, 2, 3), (h, 1.0, 1.0), (periodic, False, False), symmetric=sym)
+    got = P.apply(op, 0, v)
+    assert np.abs(got - want).max() / np.abs(want).max() < 1e-11
+    # the same line operator along y and z (transposed layouts)
+    vy = np.asfortranarray(np.transpose(v, (1, 0, 2)))
+    Py = P.__class__((2, n, 3), (1.0, h, 1.0), (False, periodic, False),
+                     symmetric=None if sym is None else [[False, False], [True, True], [False, False]])
+    goty = Py.apply(op, 1, vy)
+    assert np.abs(np.transpose(goty, (1, 0, 2)) - got).max() <= 1e-13 * np.abs(got).max()
+    vz = np.asfortranarray(np.transpose(v, (1, 2, 0)))
+    Pz = P.__class__((2, 3, n), (1.0, 1.0, h), (False, False, periodic),
+                     symmetric=None if sym is None else [[False, False], [False, False], [True, True]])
+    gotz = Pz.apply(op, 2, vz)
+    assert np.abs(np.transpose(gotz, (2, 0, 1)) - got).max() <= 1e-13 * np.abs(got).max()
+
+
+def test_null_direction(oracle_mod):
+    po = oracle_mod
+    v = np.random.default_rng(0).standard_normal((10, 3, 1))
+    P = po.Oracle((10, 3, 1), (0.1, 0.2, 0.3), (True, False, False))
+    assert np.all(P.apply(po.D1, 1, v) == 0.0)      # n<4: zero (compact.f90:95-97)
+    assert np.array_equal(P.apply(po.SF, 1, v), v)  # filters copy
+    assert np.array_equal(P.apply(po.GF, 2, v), v)
+
+
+# ---------------------------------------------------------------------------
+# The reference-held regression value of examples/unit_test.py
+# ---------------------------------------------------------------------------
+def unit_test_value(po):
+    """examples/unit_test.py restated on the oracle (same expressions, order)."""
+    N = 32
+    L = np.pi * 2.0 * (N - 1) / N
+    (X, Y, Z), dd = mesh([(0.0, L, N)] * 3)
+    P = po.Oracle((N, N, N), dd, (True, True, True))
+    E = 0.0
+    for d, C in enumerate((X, Y, Z)):
+        E += np.sum(np.abs(P.apply(po.D1, d, np.sin(C)) - np.cos(C)))
+    (X2, Y2, Z2), dd2 = mesh([(0.0, 1.0, N)] * 3)
+    B = po.Oracle((N, N, N), dd2, (False, False, False))
+    npts = N ** 3
+
+    def mean(a):
+        return 1.0 / float(npts) * np.sum(a)
+
+    for d, C in enumerate((X2, Y2, Z2)):
+        E += mean(np.abs(B.apply(po.D1, d, C * 0.0)))
+    for d, C in enumerate((X2, Y2, Z2)):
+        E += mean(np.abs(B.apply(po.D1, d, C * 0.0 + 1.0)))
+    for p in [1.0, 2.0, 3.0, 4.0]:
+        for d, C in enumerate((X2, Y2, Z2)):
+            E += mean(np.abs(B.apply(po.D1, d, C ** p) - p * C ** (p - 1)))
+    for bnd, per in ((B.sfilter, P.sfilter), (B.gfilter, P.gfilter)):
+        E += mean(np.abs(bnd(X2 * 0.0)))
+        E += mean(np.abs(bnd(X2 * 0.0 + 1.0) - (X2 * 0.0 + 1.0)))
+        E += mean(np.abs(bnd(X2) - X2))
+        E += mean(np.abs(per(np.sin(X)) - np.sin(X)))
+        E += mean(np.abs(per(X * 0.0 + 1.0) - 1.0))
+        E += mean(np.abs(per(X * 0.0)))
+    return E
+
+
+def test_unit_test_golden(oracle_mod):
+    want = _golden()["unit-test-3D"]["value"]
+    got = unit_test_value(oracle_mod)
+    assert abs(got - want) / abs(want) < 1e-9, (got, want)
+
+
+def test_planted_mistake_is_caught(oracle_mod, monkeypatch):
+    """A dropped metric (a plausible slip) must break the golden check."""
+    po = oracle_mod
+    orig = po.Oracle.apply
+
+    def no_metric(self, op, d, v):
+        out = orig(self, op, d, v)
+        return out * self.dd[d] if op == po.D1 else out
+
+    monkeypatch.setattr(po.Oracle, "apply", no_metric)
+    got = unit_test_value(po)
+    want = _golden()["unit-test-3D"]["value"]
+    assert abs(got - want) / abs(want) > 1e-3
