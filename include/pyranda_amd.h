@@ -1,0 +1,122 @@
+/*
+ * pyranda_amd C-ABI: MI355X-native compact finite-difference core.
+ *
+ * Drop-in boundary for pyranda's f2py module `parcop` (pyranda/parcop/parcop.f90,
+ * reached only through pyranda/pyrandaMPI.py:151-155,664-740 and
+ * pyranda/pyrandaMesh.py:85-208). Every entry point below names the parcop
+ * routine it replaces.
+ *
+ * Conventions
+ *  - Fields are DEVICE pointers to (nx,ny,nz) arrays in the reference's
+ *    Fortran order: x fastest, then y, then z (parcop's `val(nx,ny,nz)`).
+ *  - Outputs are caller-allocated (parcop returns fresh intent(out) arrays;
+ *    the Python shim allocates them). Inputs and outputs must not alias.
+ *  - Work is enqueued on the context's stream (pyr_set_stream); calls do not
+ *    synchronise the host.
+ *  - Errors return a nonzero status; pyr_last_error() gives the message
+ *    (replaces the reference's print + STOP, compact_d1.f90:65-72).
+ *  - `_f64` entry points compute like the reference in FP64. `_r32` entry
+ *    points read/write FP32 fields, sweep in FP32 and refine once with an
+ *    FP64 residual against the same banded operator.
+ */
+#ifndef PYRANDA_AMD_H
+#define PYRANDA_AMD_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct pyr_ctx pyr_ctx;
+
+/* boundary codes per side, as the strings pyrandaMPI passes to parcop.setup
+ * (pyrandaMPI.py:101-131): 'PERI', 'NONE', 'SYMM' */
+enum { PYR_BC_PERI = 0, PYR_BC_NONE = 1, PYR_BC_SYMM = 2 };
+
+/* operator ids for the generic entry point */
+enum { PYR_OP_D1 = 0, PYR_OP_D2 = 1, PYR_OP_D8 = 2, PYR_OP_SFILTER = 3, PYR_OP_GFILTER = 4 };
+
+/* how a directional result lands in `out` (fuses the pointwise combination of
+ * lap/div/ring into the operator's store) */
+enum { PYR_EPI_STORE = 0, PYR_EPI_ADD = 1, PYR_EPI_ABS_SCALED = 2, PYR_EPI_MAXABS_SCALED = 3 };
+
+/* precision of the generic entry point */
+enum { PYR_F64 = 0, PYR_R32 = 1 };
+
+/* --- setup / teardown -------------------------------------------------- */
+
+/* Replaces parcop.setup (parcop.f90:23-61) for one Cartesian block on one GPU.
+ * x1..zn are node bounds as pyranda gives them; bc6 = {x1,xn,y1,yn,z1,zn}. */
+int pyr_setup(pyr_ctx** ctx, int nx, int ny, int nz, double x1, double xn, double y1,
+              double yn, double z1, double zn, const int* bc6);
+/* Same tables built on the host only (no device allocation): for CPU tests of
+ * the host-side plan and pyr_host_emulate. */
+int pyr_setup_host(pyr_ctx** ctx, int nx, int ny, int nz, double x1, double xn, double y1,
+                   double yn, double z1, double zn, const int* bc6);
+int pyr_destroy(pyr_ctx* ctx);
+/* HIP stream (hipStream_t as void*), 0 = null stream */
+int pyr_set_stream(pyr_ctx* ctx, void* stream);
+const char* pyr_last_error(void);
+
+/* --- mesh (parcop.f90 xgrid/ygrid/zgrid/dxgrid/mesh_getgridlen; mesh.f90:173-212)
+ * Host outputs: coords of one axis (length n of that axis), the patch spacing
+ * d[3] (metric of ddx), the mesh lengths d1..d3 (ring scale) and GridLen. */
+int pyr_axis_coords(pyr_ctx* ctx, int axis, double* coords_host);
+int pyr_mesh_scalars(pyr_ctx* ctx, double* dpatch3, double* dmesh3, double* gridlen);
+
+/* --- FP64 operators (device pointers) ---------------------------------- */
+int pyr_ddx_f64(pyr_ctx* ctx, const double* val, double* dval);      /* parcop.f90:225 ddx */
+int pyr_ddy_f64(pyr_ctx* ctx, const double* val, double* dval);      /* parcop.f90:235 ddy */
+int pyr_ddz_f64(pyr_ctx* ctx, const double* val, double* dval);      /* parcop.f90:245 ddz */
+int pyr_dd8x_f64(pyr_ctx* ctx, const double* val, double* dval);     /* parcop.f90:283 dd8x */
+int pyr_dd8y_f64(pyr_ctx* ctx, const double* val, double* dval);     /* parcop.f90:290 dd8y */
+int pyr_dd8z_f64(pyr_ctx* ctx, const double* val, double* dval);     /* parcop.f90:297 dd8z */
+int pyr_d2x_f64(pyr_ctx* ctx, const double* val, double* dval);      /* compact_operators.f90:130 d2x */
+int pyr_d2y_f64(pyr_ctx* ctx, const double* val, double* dval);      /* compact_operators.f90:156 d2y */
+int pyr_d2z_f64(pyr_ctx* ctx, const double* val, double* dval);      /* compact_operators.f90:182 d2z */
+int pyr_plaplacian_f64(pyr_ctx* ctx, const double* val, double* dval); /* parcop.f90:303 plaplacian */
+int pyr_pring_f64(pyr_ctx* ctx, const double* val, double* dval);    /* parcop.f90:313 pRing (L=2) */
+int pyr_pringv_f64(pyr_ctx* ctx, const double* vx, const double* vy, const double* vz,
+                   double* dval);                                    /* parcop.f90:324 pRingV (L=1) */
+int pyr_sfilter_f64(pyr_ctx* ctx, const double* val, double* dval);  /* parcop.f90:337 sFilter (fbar) */
+int pyr_gfilter_f64(pyr_ctx* ctx, const double* val, double* dval);  /* parcop.f90:348 gFilter (gbar) */
+int pyr_gfilterdir_f64(pyr_ctx* ctx, const double* val, double* dval, int dir); /* parcop.f90:359 */
+int pyr_grads_f64(pyr_ctx* ctx, const double* val, double* v1, double* v2,
+                  double* v3);                                       /* parcop.f90:371 gradS */
+int pyr_divergence_f64(pyr_ctx* ctx, const double* fx, const double* fy, const double* fz,
+                       double* val);                                 /* parcop.f90:202 divergence */
+
+/* --- FP32 fields, FP32 sweeps + one FP64-residual refinement ------------ */
+int pyr_ddx_r32(pyr_ctx* ctx, const float* val, float* dval);
+int pyr_ddy_r32(pyr_ctx* ctx, const float* val, float* dval);
+int pyr_ddz_r32(pyr_ctx* ctx, const float* val, float* dval);
+int pyr_plaplacian_r32(pyr_ctx* ctx, const float* val, float* dval);
+int pyr_pring_r32(pyr_ctx* ctx, const float* val, float* dval);
+int pyr_sfilter_r32(pyr_ctx* ctx, const float* val, float* dval);
+int pyr_gfilter_r32(pyr_ctx* ctx, const float* val, float* dval);
+int pyr_gfilterdir_r32(pyr_ctx* ctx, const float* val, float* dval, int dir);
+int pyr_grads_r32(pyr_ctx* ctx, const float* val, float* v1, float* v2, float* v3);
+int pyr_divergence_r32(pyr_ctx* ctx, const float* fx, const float* fy, const float* fz,
+                       float* val);
+
+/* --- generic directional operator (compact_operators.f90:13-497) --------
+ * op: PYR_OP_*, dir: 0..2, prec: PYR_F64 (double*) or PYR_R32 (float*),
+ * epi: PYR_EPI_*, epi_scale multiplies |y| for the ABS epilogues. */
+int pyr_apply(pyr_ctx* ctx, int op, int dir, int prec, const void* in, void* out, int epi,
+              double epi_scale);
+
+/* --- host-side checks (no GPU needed) ----------------------------------- */
+/* Weight tables as the library restates them (stencils.f90), for comparison
+ * with the oracle: ali[5], ari[9], alb1/alb2[3*4*5], arb1/arb2[3*4*9]. */
+int pyr_host_weights(int op, double* ali, double* ari, double* alb1, double* alb2, double* arb1,
+                     double* arb2, int* dims6);
+/* Runs the GPU kernels' per-thread arithmetic (chunked line solve) on the host
+ * over every line of one direction: a CPU emulation of the device code path
+ * used only by tests to check the partitioned algorithm without a GPU. */
+int pyr_host_emulate(pyr_ctx* ctx, int op, int dir, int prec, const void* in, void* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

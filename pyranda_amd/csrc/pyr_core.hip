@@ -1,0 +1,779 @@
+// MI355X (gfx950) kernels and C-ABI of the compact finite-difference core.
+//
+// One launch applies one compact operator along one direction to a whole
+// (nx,ny,nz) field. Each thread owns one chunk of one line (pyr_plan.hpp):
+//   y/z lines: lanes of a wave are adjacent lines, i.e. adjacent x, so every
+//     row load/store of a wave is one contiguous 256-512 B segment;
+//   x lines:   a workgroup stages LX contiguous lines in LDS with coalesced
+//     copies, threads then read their chunk window from LDS.
+// The reduced system couples the chunks of a line through LDS within the
+// workgroup, so every field element is read once and written once from HBM.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <type_traits>
+#include <vector>
+
+#include "../../include/pyranda_amd.h"
+#include "pyr_line.hpp"
+
+using namespace pyr;
+
+// ---------------------------------------------------------------------------
+// error reporting
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+static int fail(const std::string& m) {
+  g_err = m;
+  return 1;
+}
+#define HIPCHECK(x)                                                              \
+  do {                                                                           \
+    hipError_t e_ = (x);                                                         \
+    if (e_ != hipSuccess) return fail(std::string(#x) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+template <int MODE> struct SolveT { using type = double; };
+template <> struct SolveT<MODE_R32> { using type = float; };
+
+constexpr int WG_MAX = 512;
+
+// Solve one chunk given its RHS; the LDS exchange of edge values goes through
+// Y (4P x ly entries of the solve type), column `col` belongs to this line.
+template <bool IMPL, int MODE, typename CS>
+__device__ __forceinline__ void solve_chunk(const LineArgs& a, int p, int s, int m, CS* Y, int ly,
+                                            int col, double* rhs, double* y) {
+  if (!IMPL) {
+#pragma unroll
+    for (int k = 0; k < CHUNK; ++k) y[k] = rhs[k];
+    return;
+  }
+  auto yr = [&](int c) { return Y[c * ly + col]; };
+  if (MODE == MODE_F64) {
+    double x[CHUNK];
+#pragma unroll
+    for (int k = 0; k < CHUNK; ++k) x[k] = rhs[k];
+    local_solve<double>(a.lu, s, m, x);
+    double e4[4];
+    edges(x, m, e4);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) Y[(4 * p + q) * ly + col] = (CS)e4[q];
+    __syncthreads();
+    double z[4];
+    neighbours<double>(a.gsel, p, a.P, yr, z);
+    spike_correct(a.rc, s, m, z, x);
+#pragma unroll
+    for (int k = 0; k < CHUNK; ++k) y[k] = x[k];
+  } else {
+    float x[CHUNK];
+#pragma unroll
+    for (int k = 0; k < CHUNK; ++k) x[k] = (float)rhs[k];
+    local_solve<float>(a.lu32, s, m, x);
+    float e4[4];
+    edges(x, m, e4);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) Y[(4 * p + q) * ly + col] = (CS)e4[q];
+    __syncthreads();
+    float z[4];
+    neighbours<float>(a.gsel32, p, a.P, yr, z);
+    spike_correct(a.rc32, s, m, z, x);
+    // one refinement step: FP64 residual against the same banded operator
+    double res[CHUNK];
+    residual64(a.band, s, m, rhs, x, z, res);
+    float d[CHUNK];
+#pragma unroll
+    for (int k = 0; k < CHUNK; ++k) d[k] = (float)res[k];
+    local_solve<float>(a.lu32, s, m, d);
+    float f4[4];
+    edges(d, m, f4);
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < 4; ++q) Y[(4 * p + q) * ly + col] = (CS)f4[q];
+    __syncthreads();
+    float zd[4];
+    neighbours<float>(a.gsel32, p, a.P, yr, zd);
+    spike_correct(a.rc32, s, m, zd, d);
+#pragma unroll
+    for (int k = 0; k < CHUNK; ++k) y[k] = (double)x[k] + (double)d[k];
+  }
+}
+
+// y/z lines. block = (LY lanes, P chunks); LY = 64 makes the chunk index
+// wave-uniform so the tables are read with scalar loads.
+template <typename TS, int KIND, bool IMPL, int MODE, int LY>
+__global__ void __launch_bounds__(WG_MAX) k_line_yz(LineArgs a, const TS* __restrict__ in,
+                                                    TS* __restrict__ out) {
+  constexpr int R = Halo<KIND>::R;
+  using CS = typename SolveT<MODE>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  CS* Y = reinterpret_cast<CS*>(smem);
+  const int lane = threadIdx.x;
+  int p = threadIdx.y;
+  if (LY == 64) p = __builtin_amdgcn_readfirstlane(p);
+  const long long L = (long long)blockIdx.x * LY + lane;
+  const bool valid = L < a.nlines;
+  const long long Lc = valid ? L : (long long)a.nlines - 1;
+  const long long base = (Lc % a.A) + a.B * (Lc / a.A);
+  const int s = a.cstart[p];
+  const int m = a.cstart[p + 1] - s;
+  const TS* vin = in + base;
+
+  TS win[CHUNK + 2 * R];
+#pragma unroll
+  for (int w = 0; w < CHUNK + 2 * R; ++w) {
+    int row = s - R + w;
+    TS v = (TS)0;
+    if (w < m + 2 * R) {
+      if (a.periodic) {
+        row = row < 0 ? row + a.N : (row >= a.N ? row - a.N : row);
+        v = vin[a.stride * row];
+      } else if (row >= 0 && row < a.N) {
+        v = vin[a.stride * row];
+      }
+    }
+    win[w] = v;
+  }
+  double rhs[CHUNK];
+  rhs_interior<KIND>(a, win, rhs, m);
+  rhs_boundary(a, s, m, rhs, [&](int i) { return vin[a.stride * i]; });
+  double y[CHUNK];
+  solve_chunk<IMPL, MODE, CS>(a, p, s, m, Y, LY, lane, rhs, y);
+  if (valid) {
+    TS* vout = out + base;
+#pragma unroll
+    for (int k = 0; k < CHUNK; ++k) {
+      if (k < m) {
+        double val = y[k] * a.scale;
+        if (a.addv) val += (double)win[k + R];
+        TS* dst = vout + a.stride * (s + k);
+        const TS old = (a.epi == EPI_ADD || a.epi == EPI_MAXABS) ? *dst : (TS)0;
+        *dst = epilogue<TS>(a.epi, a.epi_c, val, old);
+      }
+    }
+  }
+}
+
+// x lines: LX contiguous lines per workgroup staged in LDS (row pitch N+1).
+template <typename TS, int KIND, bool IMPL, int MODE>
+__global__ void __launch_bounds__(WG_MAX) k_line_x(LineArgs a, const TS* __restrict__ in,
+                                                   TS* __restrict__ out) {
+  constexpr int R = Halo<KIND>::R;
+  using CS = typename SolveT<MODE>::type;
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = a.N, pitch = a.N + 1, LX = a.lx;
+  TS* tile = reinterpret_cast<TS*>(smem);
+  const size_t tile_bytes = ((size_t)LX * pitch * sizeof(TS) + 15) & ~(size_t)15;
+  CS* Y = reinterpret_cast<CS*>(smem + tile_bytes);
+  const int tid = threadIdx.x;
+  const int l = tid % LX;
+  const int p = tid / LX;
+  const long long L0 = (long long)blockIdx.x * LX;
+  const int nval = (int)min((long long)LX, (long long)a.nlines - L0);
+  const TS* src = in + L0 * N;
+  for (int e = tid; e < nval * N; e += blockDim.x) {
+    const int ll = e / N, i = e - ll * N;
+    tile[ll * pitch + i] = src[e];
+  }
+  __syncthreads();
+  const int lr = l < nval ? l : 0;
+  const TS* row0 = tile + lr * pitch;
+  const int s = a.cstart[p];
+  const int m = a.cstart[p + 1] - s;
+  TS win[CHUNK + 2 * R];
+#pragma unroll
+  for (int w = 0; w < CHUNK + 2 * R; ++w) {
+    int row = s - R + w;
+    TS v = (TS)0;
+    if (w < m + 2 * R) {
+      if (a.periodic) {
+        row = row < 0 ? row + N : (row >= N ? row - N : row);
+        v = row0[row];
+      } else if (row >= 0 && row < N) {
+        v = row0[row];
+      }
+    }
+    win[w] = v;
+  }
+  double rhs[CHUNK];
+  rhs_interior<KIND>(a, win, rhs, m);
+  rhs_boundary(a, s, m, rhs, [&](int i) { return row0[i]; });
+  double y[CHUNK];
+  solve_chunk<IMPL, MODE, CS>(a, p, s, m, Y, LX, l, rhs, y);
+  __syncthreads();  // every window has been read from the tile
+  if (l < nval) {
+    TS* rowo = tile + l * pitch;
+#pragma unroll
+    for (int k = 0; k < CHUNK; ++k) {
+      if (k < m) {
+        double val = y[k] * a.scale;
+        if (a.addv) val += (double)win[k + R];
+        rowo[s + k] = (TS)val;
+      }
+    }
+  }
+  __syncthreads();
+  TS* dst = out + L0 * N;
+  const bool need_old = (a.epi == EPI_ADD || a.epi == EPI_MAXABS);
+  for (int e = tid; e < nval * N; e += blockDim.x) {
+    const int ll = e / N, i = e - ll * N;
+    const double val = (double)tile[ll * pitch + i];
+    const TS old = need_old ? dst[e] : (TS)0;
+    dst[e] = (a.epi == EPI_STORE) ? tile[ll * pitch + i] : epilogue<TS>(a.epi, a.epi_c, val, old);
+  }
+}
+
+// null operator of a direction with fewer than 4 points (zero or copy)
+template <typename TS>
+__global__ void k_null(long long n, int copy, int epi, double c, const TS* __restrict__ in,
+                       TS* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double val = copy ? (double)in[i] : 0.0;
+  const TS old = (epi == EPI_ADD || epi == EPI_MAXABS) ? out[i] : (TS)0;
+  out[i] = epilogue<TS>(epi, c, val, old);
+}
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct DevPlan {
+  void* mem = nullptr;  // one allocation holding every table
+  const int* cstart = nullptr;
+  const double *lu = nullptr, *rc = nullptr, *gsel = nullptr, *band = nullptr, *bcoef = nullptr;
+  const float *lu32 = nullptr, *rc32 = nullptr, *gsel32 = nullptr;
+  const short* bidx = nullptr;
+  const int* bnt = nullptr;
+};
+
+struct pyr_ctx {
+  int n[3] = {1, 1, 1};
+  int bc[6] = {PYR_BC_NONE, PYR_BC_NONE, PYR_BC_NONE, PYR_BC_NONE, PYR_BC_NONE, PYR_BC_NONE};
+  double x1[3] = {0, 0, 0};
+  double dpatch[3] = {1, 1, 1};  // patch dx (metric of derivatives)
+  double dmesh[3] = {1, 1, 1};   // mesh d1..d3 (ring length scales)
+  double gridlen = 1.0;
+  bool device = false;
+  hipStream_t stream = nullptr;
+  OpPlan plan[NOPS][3];
+  DevPlan dplan[NOPS][3];
+  void* scratch[2] = {nullptr, nullptr};  // one field-sized temp per precision
+  size_t npts() const { return (size_t)n[0] * n[1] * n[2]; }
+};
+
+static int upload_plan(const OpPlan& pl, DevPlan& d) {
+  if (pl.null_op) return 0;
+  const size_t n = pl.N, P = pl.P;
+  std::vector<std::pair<const void*, size_t>> parts;
+  auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  size_t off = 0;
+  size_t o_cs = off; off = align(off + (P + 1) * sizeof(int));
+  size_t o_lu = off; off = align(off + n * 5 * sizeof(double));
+  size_t o_rc = off; off = align(off + n * 4 * sizeof(double));
+  size_t o_gs = off; off = align(off + P * 16 * P * sizeof(double));
+  size_t o_bd = off; off = align(off + n * 5 * sizeof(double));
+  size_t o_bc = off; off = align(off + 2 * NB * MAXT * sizeof(double));
+  size_t o_bi = off; off = align(off + 2 * NB * MAXT * 2 * sizeof(short));
+  size_t o_bn = off; off = align(off + 2 * NB * sizeof(int));
+  size_t o_lu32 = off; off = align(off + n * 5 * sizeof(float));
+  size_t o_rc32 = off; off = align(off + n * 4 * sizeof(float));
+  size_t o_gs32 = off; off = align(off + P * 16 * P * sizeof(float));
+  std::vector<unsigned char> host(off, 0);
+  std::memcpy(&host[o_cs], pl.cstart.data(), (P + 1) * sizeof(int));
+  std::memcpy(&host[o_lu], pl.lu.data(), n * 5 * sizeof(double));
+  std::memcpy(&host[o_rc], pl.rc.data(), n * 4 * sizeof(double));
+  std::memcpy(&host[o_gs], pl.gsel.data(), P * 16 * P * sizeof(double));
+  std::memcpy(&host[o_bd], pl.band.data(), n * 5 * sizeof(double));
+  std::memcpy(&host[o_bc], pl.bcoef.data(), 2 * NB * MAXT * sizeof(double));
+  std::memcpy(&host[o_bi], pl.bidx.data(), 2 * NB * MAXT * 2 * sizeof(short));
+  std::memcpy(&host[o_bn], pl.bnt.data(), 2 * NB * sizeof(int));
+  float* lu32 = reinterpret_cast<float*>(&host[o_lu32]);
+  float* rc32 = reinterpret_cast<float*>(&host[o_rc32]);
+  float* gs32 = reinterpret_cast<float*>(&host[o_gs32]);
+  for (size_t i = 0; i < n * 5; ++i) lu32[i] = (float)pl.lu[i];
+  for (size_t i = 0; i < n * 4; ++i) rc32[i] = (float)pl.rc[i];
+  for (size_t i = 0; i < P * 16 * P; ++i) gs32[i] = (float)pl.gsel[i];
+  HIPCHECK(hipMalloc(&d.mem, off));
+  HIPCHECK(hipMemcpy(d.mem, host.data(), off, hipMemcpyHostToDevice));
+  unsigned char* b = static_cast<unsigned char*>(d.mem);
+  d.cstart = reinterpret_cast<const int*>(b + o_cs);
+  d.lu = reinterpret_cast<const double*>(b + o_lu);
+  d.rc = reinterpret_cast<const double*>(b + o_rc);
+  d.gsel = reinterpret_cast<const double*>(b + o_gs);
+  d.band = reinterpret_cast<const double*>(b + o_bd);
+  d.bcoef = reinterpret_cast<const double*>(b + o_bc);
+  d.bidx = reinterpret_cast<const short*>(b + o_bi);
+  d.bnt = reinterpret_cast<const int*>(b + o_bn);
+  d.lu32 = reinterpret_cast<const float*>(b + o_lu32);
+  d.rc32 = reinterpret_cast<const float*>(b + o_rc32);
+  d.gsel32 = reinterpret_cast<const float*>(b + o_gs32);
+  return 0;
+}
+
+// host pointers of the same tables (host emulation)
+static void host_tables(const OpPlan& pl, DevPlan& d, std::vector<float>& lu32,
+                        std::vector<float>& rc32, std::vector<float>& gs32) {
+  lu32.assign(pl.lu.begin(), pl.lu.end());
+  rc32.assign(pl.rc.begin(), pl.rc.end());
+  gs32.assign(pl.gsel.begin(), pl.gsel.end());
+  d.cstart = pl.cstart.data();
+  d.lu = pl.lu.data();
+  d.rc = pl.rc.data();
+  d.gsel = pl.gsel.data();
+  d.band = pl.band.data();
+  d.bcoef = pl.bcoef.data();
+  d.bidx = pl.bidx.data();
+  d.bnt = pl.bnt.data();
+  d.lu32 = lu32.data();
+  d.rc32 = rc32.data();
+  d.gsel32 = gs32.data();
+}
+
+static LineArgs make_args(const pyr_ctx* c, const OpPlan& pl, const DevPlan& d, int dir, int epi,
+                          double ec) {
+  LineArgs a;
+  std::memset(&a, 0, sizeof(a));
+  const long long nx = c->n[0], ny = c->n[1], nz = c->n[2];
+  a.N = pl.N;
+  a.P = pl.P;
+  a.periodic = pl.periodic;
+  if (dir == 0) {
+    a.nlines = (int)(ny * nz);
+    a.A = 1; a.B = nx; a.stride = 1;
+  } else if (dir == 1) {
+    a.nlines = (int)(nx * nz);
+    a.A = nx; a.B = nx * ny; a.stride = nx;
+  } else {
+    a.nlines = (int)(nx * ny);
+    a.A = nx * ny; a.B = 0; a.stride = nx * ny;
+  }
+  a.addv = pl.addv;
+  a.epi = epi;
+  a.scale = pl.scale;
+  a.epi_c = ec;
+  for (int i = 0; i < 9; ++i) a.w[i] = pl.w[i];
+  a.cstart = d.cstart; a.lu = d.lu; a.rc = d.rc; a.gsel = d.gsel; a.band = d.band;
+  a.lu32 = d.lu32; a.rc32 = d.rc32; a.gsel32 = d.gsel32;
+  a.bcoef = d.bcoef; a.bidx = d.bidx; a.bnt = d.bnt;
+  return a;
+}
+
+// ---------------------------------------------------------------------------
+// launch
+// ---------------------------------------------------------------------------
+template <typename TS, int KIND, bool IMPL, int MODE>
+static int launch_typed(pyr_ctx* c, int dir, LineArgs a, const TS* in, TS* out) {
+  using CS = typename SolveT<MODE>::type;
+  const int P = a.P;
+  if (dir == 0) {
+    int lx = WG_MAX / P;
+    if (lx > 64) lx = 64;
+    // keep the staged tile within 48 KiB so several workgroups share a CU
+    while (lx > 4 && (size_t)lx * (a.N + 1) * sizeof(TS) > 48 * 1024) lx /= 2;
+    a.lx = lx;
+    const size_t tile = ((size_t)lx * (a.N + 1) * sizeof(TS) + 15) & ~(size_t)15;
+    const size_t shm = tile + (size_t)4 * P * lx * sizeof(CS);
+    const dim3 grid((a.nlines + lx - 1) / lx), block(lx * P);
+    hipLaunchKernelGGL((k_line_x<TS, KIND, IMPL, MODE>), grid, block, shm, c->stream, a, in, out);
+  } else if (P <= WG_MAX / 64) {
+    const size_t shm = (size_t)4 * P * 64 * sizeof(CS);
+    const dim3 grid((a.nlines + 63) / 64), block(64, P);
+    hipLaunchKernelGGL((k_line_yz<TS, KIND, IMPL, MODE, 64>), grid, block, shm, c->stream, a, in, out);
+  } else {
+    const size_t shm = (size_t)4 * P * 32 * sizeof(CS);
+    const dim3 grid((a.nlines + 31) / 32), block(32, P);
+    hipLaunchKernelGGL((k_line_yz<TS, KIND, IMPL, MODE, 32>), grid, block, shm, c->stream, a, in, out);
+  }
+  HIPCHECK(hipGetLastError());
+  return 0;
+}
+
+template <typename TS, int MODE>
+static int launch_op(pyr_ctx* c, int op, int dir, const TS* in, TS* out, int epi, double ec) {
+  const OpPlan& pl = c->plan[op][dir];
+  if (pl.null_op) {
+    const long long n = (long long)c->npts();
+    hipLaunchKernelGGL((k_null<TS>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n,
+                       pl.null_option == 1 ? 1 : 0, epi, ec, in, out);
+    HIPCHECK(hipGetLastError());
+    return 0;
+  }
+  const LineArgs a = make_args(c, pl, c->dplan[op][dir], dir, epi, ec);
+  switch (pl.kind) {
+    case K_D1: return launch_typed<TS, K_D1, true, MODE>(c, dir, a, in, out);
+    case K_R3: return launch_typed<TS, K_R3, true, MODE>(c, dir, a, in, out);
+    default:
+      if (pl.implicit) return launch_typed<TS, K_R4, true, MODE>(c, dir, a, in, out);
+      return launch_typed<TS, K_R4, false, MODE>(c, dir, a, in, out);
+  }
+}
+
+static int check_ctx(pyr_ctx* c) {
+  if (!c) return fail("null context");
+  if (!c->device) return fail("context was set up host-only (pyr_setup_host)");
+  return 0;
+}
+
+static int apply_any(pyr_ctx* c, int op, int dir, int prec, const void* in, void* out, int epi,
+                     double ec) {
+  if (check_ctx(c)) return 1;
+  if (op < 0 || op >= NOPS || dir < 0 || dir > 2) return fail("bad operator or direction");
+  if (!in || !out) return fail("null field pointer");
+  if (in == out) return fail("input and output must not alias");
+  if (prec == PYR_F64)
+    return launch_op<double, MODE_F64>(c, op, dir, (const double*)in, (double*)out, epi, ec);
+  if (prec == PYR_R32)
+    return launch_op<float, MODE_R32>(c, op, dir, (const float*)in, (float*)out, epi, ec);
+  return fail("bad precision");
+}
+
+static int scratch(pyr_ctx* c, int prec, void** p) {
+  const size_t bytes = c->npts() * (prec == PYR_F64 ? sizeof(double) : sizeof(float));
+  if (!c->scratch[prec]) HIPCHECK(hipMalloc(&c->scratch[prec], bytes));
+  *p = c->scratch[prec];
+  return 0;
+}
+
+// filter x -> y -> z (operators.f90:847-875)
+static int filter3(pyr_ctx* c, int op, int prec, const void* in, void* out) {
+  void* tmp;
+  if (check_ctx(c) || scratch(c, prec, &tmp)) return 1;
+  if (apply_any(c, op, 0, prec, in, out, EPI_STORE, 0.0)) return 1;
+  if (apply_any(c, op, 1, prec, out, tmp, EPI_STORE, 0.0)) return 1;
+  return apply_any(c, op, 2, prec, tmp, out, EPI_STORE, 0.0);
+}
+
+// ring L=2 (operators.f90:615-643): max_d |d8_d f| * d_d^2, fused into the stores
+static int ring(pyr_ctx* c, int prec, const void* in, void* out) {
+  for (int d = 0; d < 3; ++d) {
+    const double s = c->dmesh[d] * c->dmesh[d];
+    if (apply_any(c, OP_D8, d, prec, in, out, d == 0 ? EPI_ABS : EPI_MAXABS, s)) return 1;
+  }
+  return 0;
+}
+
+static int sum3(pyr_ctx* c, int op, int prec, const void* f0, const void* f1, const void* f2,
+                void* out) {
+  const void* f[3] = {f0, f1, f2};
+  for (int d = 0; d < 3; ++d)
+    if (apply_any(c, op, d, prec, f[d], out, d == 0 ? EPI_STORE : EPI_ADD, 0.0)) return 1;
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// setup
+// ---------------------------------------------------------------------------
+static int setup_common(pyr_ctx** out, int nx, int ny, int nz, double x1, double xn, double y1,
+                        double yn, double z1, double zn, const int* bc6, bool device) {
+  if (!out) return fail("null output handle");
+  *out = nullptr;
+  if (nx < 1 || ny < 1 || nz < 1) return fail("grid sizes must be >= 1");
+  if (!bc6) return fail("null bc array");
+  std::unique_ptr<pyr_ctx> c(new pyr_ctx);
+  c->n[0] = nx; c->n[1] = ny; c->n[2] = nz;
+  for (int i = 0; i < 6; ++i) {
+    if (bc6[i] < PYR_BC_PERI || bc6[i] > PYR_BC_SYMM) return fail("bad boundary code");
+    c->bc[i] = bc6[i];
+  }
+  const double lo[3] = {x1, y1, z1}, hi[3] = {xn, yn, zn};
+  for (int d = 0; d < 3; ++d) {
+    // faces from nodes (parcop.f90:46-56), patch spacing (patch.f90:83-85)
+    const double du = (hi[d] - lo[d]) / (double)(c->n[d] - 1 > 1 ? c->n[d] - 1 : 1);
+    const double f1 = lo[d] - du / 2.0, fn = hi[d] + du / 2.0;
+    c->x1[d] = f1;
+    c->dpatch[d] = (fn - f1) / (double)c->n[d];
+  }
+  // mesh length scales (mesh.f90:190-211)
+  c->dmesh[0] = nx == 1 ? std::fmax(c->dpatch[1], c->dpatch[2]) : c->dpatch[0];
+  c->dmesh[1] = ny == 1 ? std::fmax(c->dpatch[0], c->dpatch[2]) : c->dpatch[1];
+  c->dmesh[2] = nz == 1 ? std::fmax(c->dpatch[0], c->dpatch[1]) : c->dpatch[2];
+  c->gridlen = std::fmin(c->dmesh[0], std::fmin(c->dmesh[1], c->dmesh[2]));
+  try {
+    for (int d = 0; d < 3; ++d) {
+      const int periodic = c->bc[2 * d] == PYR_BC_PERI;
+      const int v1 = c->bc[2 * d] == PYR_BC_SYMM ? 1 : 0;
+      const int v2 = c->bc[2 * d + 1] == PYR_BC_SYMM ? 1 : 0;
+      for (int op = 0; op < NOPS; ++op)
+        c->plan[op][d] = build_plan(op, c->n[d], periodic, v1, v2, c->dpatch[d]);
+    }
+  } catch (const std::exception& e) {
+    return fail(std::string("setup: ") + e.what());
+  }
+  if (device) {
+    for (int op = 0; op < NOPS; ++op)
+      for (int d = 0; d < 3; ++d)
+        if (upload_plan(c->plan[op][d], c->dplan[op][d])) return 1;
+    c->device = true;
+  }
+  *out = c.release();
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// host emulation of the kernels (tests only)
+// ---------------------------------------------------------------------------
+template <typename TS, int KIND, bool IMPL, int MODE>
+static void emulate_line(const LineArgs& a, const TS* vin, long long stride, TS* vout) {
+  constexpr int R = Halo<KIND>::R;
+  using CS = typename SolveT<MODE>::type;
+  const int P = a.P, N = a.N;
+  struct St {
+    TS win[CHUNK + 2 * 4];
+    double rhs[CHUNK];
+    CS x[CHUNK], d[CHUNK], z[4], zd[4];
+    int s, m;
+  };
+  std::vector<St> st(P);
+  std::vector<CS> Y(4 * P);
+  auto rd = [&](int i) { return vin[stride * i]; };
+  for (int p = 0; p < P; ++p) {
+    St& t = st[p];
+    t.s = a.cstart[p];
+    t.m = a.cstart[p + 1] - t.s;
+    for (int w = 0; w < CHUNK + 2 * R; ++w) {
+      int row = t.s - R + w;
+      TS v = (TS)0;
+      if (w < t.m + 2 * R) {
+        if (a.periodic) {
+          row = row < 0 ? row + N : (row >= N ? row - N : row);
+          v = rd(row);
+        } else if (row >= 0 && row < N) {
+          v = rd(row);
+        }
+      }
+      t.win[w] = v;
+    }
+    rhs_interior<KIND>(a, t.win, t.rhs, t.m);
+    rhs_boundary(a, t.s, t.m, t.rhs, rd);
+    if (IMPL) {
+      for (int k = 0; k < CHUNK; ++k) t.x[k] = (CS)t.rhs[k];
+      local_solve<CS>(MODE == MODE_F64 ? (const CS*)a.lu : (const CS*)a.lu32, t.s, t.m, t.x);
+      CS e4[4];
+      edges(t.x, t.m, e4);
+      for (int q = 0; q < 4; ++q) Y[4 * p + q] = e4[q];
+    }
+  }
+  if (IMPL) {
+    const CS* G = MODE == MODE_F64 ? (const CS*)a.gsel : (const CS*)a.gsel32;
+    const CS* RC = MODE == MODE_F64 ? (const CS*)a.rc : (const CS*)a.rc32;
+    const CS* LU = MODE == MODE_F64 ? (const CS*)a.lu : (const CS*)a.lu32;
+    for (int p = 0; p < P; ++p) {
+      St& t = st[p];
+      neighbours<CS>(G, p, P, [&](int cc) { return Y[cc]; }, t.z);
+      spike_correct(RC, t.s, t.m, t.z, t.x);
+    }
+    if (MODE == MODE_R32) {
+      std::vector<CS> Yd(4 * P);
+      for (int p = 0; p < P; ++p) {
+        St& t = st[p];
+        double res[CHUNK];
+        residual64(a.band, t.s, t.m, t.rhs, (const float*)t.x, (const float*)t.z, res);
+        for (int k = 0; k < CHUNK; ++k) t.d[k] = (CS)res[k];
+        local_solve<CS>(LU, t.s, t.m, t.d);
+        CS f4[4];
+        edges(t.d, t.m, f4);
+        for (int q = 0; q < 4; ++q) Yd[4 * p + q] = f4[q];
+      }
+      for (int p = 0; p < P; ++p) {
+        St& t = st[p];
+        neighbours<CS>(G, p, P, [&](int cc) { return Yd[cc]; }, t.zd);
+        spike_correct(RC, t.s, t.m, t.zd, t.d);
+      }
+    }
+  }
+  for (int p = 0; p < P; ++p) {
+    St& t = st[p];
+    for (int k = 0; k < t.m; ++k) {
+      double y;
+      if (!IMPL) y = t.rhs[k];
+      else if (MODE == MODE_F64) y = (double)t.x[k];
+      else y = (double)t.x[k] + (double)t.d[k];
+      double val = y * a.scale;
+      if (a.addv) val += (double)t.win[k + R];
+      vout[stride * (t.s + k)] = (TS)val;
+    }
+  }
+}
+
+template <typename TS, int MODE>
+static int emulate(pyr_ctx* c, int op, int dir, const TS* in, TS* out) {
+  const OpPlan& pl = c->plan[op][dir];
+  const size_t n = c->npts();
+  if (pl.null_op) {
+    for (size_t i = 0; i < n; ++i) out[i] = pl.null_option == 1 ? in[i] : (TS)0;
+    return 0;
+  }
+  DevPlan d;
+  std::vector<float> lu32, rc32, gs32;
+  host_tables(pl, d, lu32, rc32, gs32);
+  const LineArgs a = make_args(c, pl, d, dir, EPI_STORE, 0.0);
+  for (long long L = 0; L < a.nlines; ++L) {
+    const long long base = (dir == 0) ? L * pl.N : (L % a.A) + a.B * (L / a.A);
+    switch (pl.kind) {
+      case K_D1: emulate_line<TS, K_D1, true, MODE>(a, in + base, a.stride, out + base); break;
+      case K_R3: emulate_line<TS, K_R3, true, MODE>(a, in + base, a.stride, out + base); break;
+      default:
+        if (pl.implicit) emulate_line<TS, K_R4, true, MODE>(a, in + base, a.stride, out + base);
+        else emulate_line<TS, K_R4, false, MODE>(a, in + base, a.stride, out + base);
+    }
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+const char* pyr_last_error(void) { return g_err.c_str(); }
+
+int pyr_setup(pyr_ctx** ctx, int nx, int ny, int nz, double x1, double xn, double y1, double yn,
+              double z1, double zn, const int* bc6) {
+  return setup_common(ctx, nx, ny, nz, x1, xn, y1, yn, z1, zn, bc6, true);
+}
+
+int pyr_setup_host(pyr_ctx** ctx, int nx, int ny, int nz, double x1, double xn, double y1,
+                   double yn, double z1, double zn, const int* bc6) {
+  return setup_common(ctx, nx, ny, nz, x1, xn, y1, yn, z1, zn, bc6, false);
+}
+
+int pyr_destroy(pyr_ctx* c) {
+  if (!c) return 0;
+  if (c->device) {
+    for (int op = 0; op < NOPS; ++op)
+      for (int d = 0; d < 3; ++d)
+        if (c->dplan[op][d].mem) (void)hipFree(c->dplan[op][d].mem);
+    for (int i = 0; i < 2; ++i)
+      if (c->scratch[i]) (void)hipFree(c->scratch[i]);
+  }
+  delete c;
+  return 0;
+}
+
+int pyr_set_stream(pyr_ctx* c, void* stream) {
+  if (!c) return fail("null context");
+  c->stream = (hipStream_t)stream;
+  return 0;
+}
+
+int pyr_axis_coords(pyr_ctx* c, int axis, double* coords) {
+  if (!c || !coords || axis < 0 || axis > 2) return fail("bad arguments to pyr_axis_coords");
+  // xgrid = x1f + (2i-1)*half*dx (mesh.f90:173-177)
+  for (int i = 1; i <= c->n[axis]; ++i)
+    coords[i - 1] = c->x1[axis] + (double)(2 * i - 1) * 0.5 * c->dpatch[axis];
+  return 0;
+}
+
+int pyr_mesh_scalars(pyr_ctx* c, double* dpatch3, double* dmesh3, double* gridlen) {
+  if (!c) return fail("null context");
+  for (int d = 0; d < 3; ++d) {
+    if (dpatch3) dpatch3[d] = c->dpatch[d];
+    if (dmesh3) dmesh3[d] = c->dmesh[d];
+  }
+  if (gridlen) *gridlen = c->gridlen;
+  return 0;
+}
+
+int pyr_apply(pyr_ctx* c, int op, int dir, int prec, const void* in, void* out, int epi,
+              double epi_scale) {
+  if (epi < PYR_EPI_STORE || epi > PYR_EPI_MAXABS_SCALED) return fail("bad epilogue");
+  return apply_any(c, op, dir, prec, in, out, epi, epi_scale);
+}
+
+#define PYR_DIR_OP(name, op, dir)                                                     \
+  int name##_f64(pyr_ctx* c, const double* v, double* o) {                           \
+    return apply_any(c, op, dir, PYR_F64, v, o, EPI_STORE, 0.0);                     \
+  }
+PYR_DIR_OP(pyr_ddx, OP_D1, 0)
+PYR_DIR_OP(pyr_ddy, OP_D1, 1)
+PYR_DIR_OP(pyr_ddz, OP_D1, 2)
+PYR_DIR_OP(pyr_dd8x, OP_D8, 0)
+PYR_DIR_OP(pyr_dd8y, OP_D8, 1)
+PYR_DIR_OP(pyr_dd8z, OP_D8, 2)
+PYR_DIR_OP(pyr_d2x, OP_D2, 0)
+PYR_DIR_OP(pyr_d2y, OP_D2, 1)
+PYR_DIR_OP(pyr_d2z, OP_D2, 2)
+#undef PYR_DIR_OP
+
+int pyr_ddx_r32(pyr_ctx* c, const float* v, float* o) { return apply_any(c, OP_D1, 0, PYR_R32, v, o, EPI_STORE, 0.0); }
+int pyr_ddy_r32(pyr_ctx* c, const float* v, float* o) { return apply_any(c, OP_D1, 1, PYR_R32, v, o, EPI_STORE, 0.0); }
+int pyr_ddz_r32(pyr_ctx* c, const float* v, float* o) { return apply_any(c, OP_D1, 2, PYR_R32, v, o, EPI_STORE, 0.0); }
+
+int pyr_plaplacian_f64(pyr_ctx* c, const double* v, double* o) { return sum3(c, OP_D2, PYR_F64, v, v, v, o); }
+int pyr_plaplacian_r32(pyr_ctx* c, const float* v, float* o) { return sum3(c, OP_D2, PYR_R32, v, v, v, o); }
+int pyr_pring_f64(pyr_ctx* c, const double* v, double* o) { return ring(c, PYR_F64, v, o); }
+int pyr_pring_r32(pyr_ctx* c, const float* v, float* o) { return ring(c, PYR_R32, v, o); }
+
+int pyr_pringv_f64(pyr_ctx* c, const double* vx, const double* vy, const double* vz, double* o) {
+  // ringV L=1 (operators.f90:645-699): max over components and directions of |d8| * d
+  const double* f[3] = {vx, vy, vz};
+  bool first = true;
+  for (int d = 0; d < 3; ++d)
+    for (int comp = 0; comp < 3; ++comp) {
+      if (apply_any(c, OP_D8, d, PYR_F64, f[comp], o, first ? EPI_ABS : EPI_MAXABS, c ? c->dmesh[d] : 0.0))
+        return 1;
+      first = false;
+    }
+  return 0;
+}
+
+int pyr_sfilter_f64(pyr_ctx* c, const double* v, double* o) { return filter3(c, OP_SF, PYR_F64, v, o); }
+int pyr_sfilter_r32(pyr_ctx* c, const float* v, float* o) { return filter3(c, OP_SF, PYR_R32, v, o); }
+int pyr_gfilter_f64(pyr_ctx* c, const double* v, double* o) { return filter3(c, OP_GF, PYR_F64, v, o); }
+int pyr_gfilter_r32(pyr_ctx* c, const float* v, float* o) { return filter3(c, OP_GF, PYR_R32, v, o); }
+int pyr_gfilterdir_f64(pyr_ctx* c, const double* v, double* o, int dir) {
+  if (dir < 1 || dir > 3) return fail("gfilterdir direction must be 1, 2 or 3");
+  return apply_any(c, OP_GF, dir - 1, PYR_F64, v, o, EPI_STORE, 0.0);
+}
+int pyr_gfilterdir_r32(pyr_ctx* c, const float* v, float* o, int dir) {
+  if (dir < 1 || dir > 3) return fail("gfilterdir direction must be 1, 2 or 3");
+  return apply_any(c, OP_GF, dir - 1, PYR_R32, v, o, EPI_STORE, 0.0);
+}
+int pyr_grads_f64(pyr_ctx* c, const double* v, double* a, double* b, double* e) {
+  if (apply_any(c, OP_D1, 0, PYR_F64, v, a, EPI_STORE, 0.0)) return 1;
+  if (apply_any(c, OP_D1, 1, PYR_F64, v, b, EPI_STORE, 0.0)) return 1;
+  return apply_any(c, OP_D1, 2, PYR_F64, v, e, EPI_STORE, 0.0);
+}
+int pyr_grads_r32(pyr_ctx* c, const float* v, float* a, float* b, float* e) {
+  if (apply_any(c, OP_D1, 0, PYR_R32, v, a, EPI_STORE, 0.0)) return 1;
+  if (apply_any(c, OP_D1, 1, PYR_R32, v, b, EPI_STORE, 0.0)) return 1;
+  return apply_any(c, OP_D1, 2, PYR_R32, v, e, EPI_STORE, 0.0);
+}
+int pyr_divergence_f64(pyr_ctx* c, const double* fx, const double* fy, const double* fz, double* o) {
+  return sum3(c, OP_D1, PYR_F64, fx, fy, fz, o);
+}
+int pyr_divergence_r32(pyr_ctx* c, const float* fx, const float* fy, const float* fz, float* o) {
+  return sum3(c, OP_D1, PYR_R32, fx, fy, fz, o);
+}
+
+int pyr_host_weights(int op, double* ali, double* ari, double* alb1, double* alb2, double* arb1,
+                     double* arb2, int* dims6) {
+  if (op < 0 || op >= NOPS) return fail("bad operator");
+  const Scheme s = make_scheme(op);
+  std::memcpy(ali, s.ali, sizeof s.ali);
+  std::memcpy(ari, s.ari, sizeof s.ari);
+  std::memcpy(alb1, s.alb1, sizeof s.alb1);
+  std::memcpy(alb2, s.alb2, sizeof s.alb2);
+  std::memcpy(arb1, s.arb1, sizeof s.arb1);
+  std::memcpy(arb2, s.arb2, sizeof s.arb2);
+  const int dims[6] = {s.nol, s.nor, s.ncl, s.ncr, s.null_option, s.implicit};
+  std::memcpy(dims6, dims, sizeof dims);
+  return 0;
+}
+
+int pyr_host_emulate(pyr_ctx* c, int op, int dir, int prec, const void* in, void* out) {
+  if (!c) return fail("null context");
+  if (op < 0 || op >= NOPS || dir < 0 || dir > 2) return fail("bad operator or direction");
+  if (prec == PYR_F64) return emulate<double, MODE_F64>(c, op, dir, (const double*)in, (double*)out);
+  if (prec == PYR_R32) return emulate<float, MODE_R32>(c, op, dir, (const float*)in, (float*)out);
+  return fail("bad precision");
+}
+
+}  // extern "C"

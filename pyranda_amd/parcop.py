@@ -1,0 +1,227 @@
+"""Device-resident replacement of pyranda's f2py module `parcop`.
+
+pyranda reaches parcop only through pyrandaMPI (pyranda/pyrandaMPI.py:151-155
+setup, :664-740 der/gfil/sfil). This shim keeps those method names and their
+argument meaning, but fields are torch tensors resident on the GPU in the
+reference's Fortran layout: shape (nx,ny,nz), x fastest (a permuted view of a
+contiguous (nz,ny,nx) buffer). Every call enqueues HIP kernels from
+libpyranda_amd.so on torch's current stream and returns a freshly allocated
+output, like f2py's intent(out).
+
+Precision: fields of dtype float64 go through the FP64 kernels; float32 fields
+through the FP32-sweep + FP64-residual refinement kernels.
+"""
+import ctypes
+
+import torch
+
+from . import _lib
+
+
+def empty_field(shape, dtype, device):
+    """(nx,ny,nz) tensor with Fortran strides (x fastest)."""
+    nx, ny, nz = shape
+    return torch.empty((nz, ny, nx), dtype=dtype, device=device).permute(2, 1, 0)
+
+
+def is_fortran(t):
+    nx, ny, nz = t.shape
+    return t.stride() == (1, nx, nx * ny) or t.numel() <= 1 or _contig_equiv(t)
+
+
+def _contig_equiv(t):
+    nx, ny, nz = t.shape
+    want = (1, nx, nx * ny)
+    return all(s == w or n == 1 for s, w, n in zip(t.stride(), want, t.shape))
+
+
+def _filter3(op):
+    def f(ctx, v):
+        a = ctx.apply(op, 0, v)
+        b = ctx.apply(op, 1, a)
+        return ctx.apply(op, 2, b)
+    return f
+
+
+def _ring(ctx, v):
+    out = None
+    for d in range(3):
+        out = ctx.apply(_lib.OP_D8, d, v, out=out,
+                        epi=_lib.EPI_ABS_SCALED if d == 0 else _lib.EPI_MAXABS_SCALED,
+                        epi_scale=ctx.dmesh[d] ** 2)
+    return out
+
+
+def _lap(ctx, v):
+    out = None
+    for d in range(3):
+        out = ctx.apply(_lib.OP_D2, d, v, out=out, epi=_lib.EPI_STORE if d == 0 else _lib.EPI_ADD)
+    return out
+
+
+def _d1(d):
+    return lambda ctx, v: ctx.apply(_lib.OP_D1, d, v)
+
+
+# the launches each composite makes, as separately timed apply() calls
+_COMPOSITE = {
+    "sfilter": _filter3(_lib.OP_SFILTER), "gfilter": _filter3(_lib.OP_GFILTER),
+    "pring": _ring, "plaplacian": _lap, "ddx": _d1(0), "ddy": _d1(1), "ddz": _d1(2),
+}
+
+
+class ParcopContext:
+    """One Cartesian block: parcop.setup(...) + the operator suite."""
+
+    def __init__(self, nn, x1, xn, bcs, device="cuda"):
+        self.nn = tuple(int(n) for n in nn)
+        self.device = torch.device(device)
+        L = _lib.lib()
+        h = ctypes.c_void_p()
+        bc6 = (ctypes.c_int * 6)(*[_lib.BC[b] for b in bcs])
+        _lib.check(L.pyr_setup(ctypes.byref(h), *self.nn, x1[0], xn[0], x1[1], xn[1],
+                               x1[2], xn[2], bc6))
+        self._h = h
+        self._L = L
+        dp = (ctypes.c_double * 3)()
+        dm = (ctypes.c_double * 3)()
+        gl = ctypes.c_double()
+        _lib.check(L.pyr_mesh_scalars(h, dp, dm, ctypes.byref(gl)))
+        self.dpatch = tuple(dp)
+        self.dmesh = tuple(dm)
+        self.gridlen = gl.value
+        # optional per-launch timing: {(op, dir, dtype): [n_launches, [(start, end), ...]]}
+        self.timing = None
+        self.coords1d = []
+        for ax in range(3):
+            buf = (ctypes.c_double * self.nn[ax])()
+            _lib.check(L.pyr_axis_coords(h, ax, buf))
+            self.coords1d.append(list(buf))
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            self._L.pyr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- helpers ------------------------------------------------------------
+    def _prep(self, v, dtype=None):
+        if not torch.is_tensor(v):
+            v = torch.full(self.nn, float(v), dtype=dtype or torch.float64, device=self.device)
+        if v.device != self.device:
+            raise ValueError("field on %s, expected %s" % (v.device, self.device))
+        if tuple(v.shape) != self.nn:
+            v = v.expand(self.nn)
+        if dtype is not None and v.dtype != dtype:
+            v = v.to(dtype)
+        if v.dtype not in (torch.float64, torch.float32):
+            v = v.to(torch.float64)
+        if not _contig_equiv(v):
+            v = v.permute(2, 1, 0).contiguous().permute(2, 1, 0)
+        return v
+
+    def _stream(self):
+        self._L.pyr_set_stream(self._h, ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream))
+
+    def _out(self, like):
+        return empty_field(self.nn, like.dtype, self.device)
+
+    @staticmethod
+    def _ptr(t):
+        return ctypes.c_void_p(t.data_ptr())
+
+    @staticmethod
+    def _suffix(t):
+        return "f64" if t.dtype == torch.float64 else "r32"
+
+    def apply(self, op, direction, v, out=None, epi=_lib.EPI_STORE, epi_scale=0.0):
+        """One directional operator (compact_operators.f90:13-497)."""
+        v = self._prep(v)
+        if out is None:
+            out = self._out(v)
+        prec = _lib.F64 if v.dtype == torch.float64 else _lib.R32
+        self._stream()
+        if self.timing is not None:
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            ev0.record()
+        _lib.check(self._L.pyr_apply(self._h, op, direction, prec, self._ptr(v), self._ptr(out), epi,
+                                     float(epi_scale)))
+        if self.timing is not None:
+            ev1.record()
+            self.timing.setdefault((op, direction, str(v.dtype)), []).append((ev0, ev1))
+        return out
+
+    def timing_summary(self):
+        """{(op, dir, dtype): (launches, mean ms)} of the recorded launches."""
+        torch.cuda.synchronize(self.device)
+        out = {}
+        for key, evs in (self.timing or {}).items():
+            ms = [a.elapsed_time(b) for a, b in evs]
+            out[key] = (len(ms), sum(ms) / len(ms))
+        return out
+
+    def _unary(self, name, v):
+        if self.timing is not None and name in _COMPOSITE:
+            return _COMPOSITE[name](self, v)
+        v = self._prep(v)
+        out = self._out(v)
+        self._stream()
+        _lib.check(getattr(self._L, "pyr_%s_%s" % (name, self._suffix(v)))(self._h, self._ptr(v), self._ptr(out)))
+        return out
+
+    # -- parcop names (parcop.f90:202-379) ----------------------------------
+    def ddx(self, v): return self._unary("ddx", v)
+    def ddy(self, v): return self._unary("ddy", v)
+    def ddz(self, v): return self._unary("ddz", v)
+    def dd8x(self, v): return self.apply(_lib.OP_D8, 0, v)
+    def dd8y(self, v): return self.apply(_lib.OP_D8, 1, v)
+    def dd8z(self, v): return self.apply(_lib.OP_D8, 2, v)
+    def plaplacian(self, v): return self._unary("plaplacian", v)
+    def pring(self, v): return self._unary("pring", v)
+    def sfilter(self, v): return self._unary("sfilter", v)
+    def gfilter(self, v): return self._unary("gfilter", v)
+
+    def gfilterdir(self, v, direction):
+        v = self._prep(v)
+        out = self._out(v)
+        self._stream()
+        _lib.check(getattr(self._L, "pyr_gfilterdir_%s" % self._suffix(v))(
+            self._h, self._ptr(v), self._ptr(out), int(direction)))
+        return out
+
+    def pringv(self, vx, vy, vz):
+        vx = self._prep(vx)
+        vy = self._prep(vy, vx.dtype)
+        vz = self._prep(vz, vx.dtype)
+        out = self._out(vx)
+        for d in range(3):
+            for comp, f in enumerate((vx, vy, vz)):
+                first = d == 0 and comp == 0
+                self.apply(_lib.OP_D8, d, f, out=out,
+                           epi=_lib.EPI_ABS_SCALED if first else _lib.EPI_MAXABS_SCALED,
+                           epi_scale=self.dmesh[d])
+        return out
+
+    def grads(self, v):
+        v = self._prep(v)
+        outs = [self._out(v) for _ in range(3)]
+        self._stream()
+        _lib.check(getattr(self._L, "pyr_grads_%s" % self._suffix(v))(
+            self._h, self._ptr(v), *[self._ptr(o) for o in outs]))
+        return outs
+
+    def divergence(self, fx, fy, fz):
+        fx = self._prep(fx)
+        fy = self._prep(fy, fx.dtype)
+        fz = self._prep(fz, fx.dtype)
+        out = self._out(fx)
+        self._stream()
+        _lib.check(getattr(self._L, "pyr_divergence_%s" % self._suffix(fx))(
+            self._h, self._ptr(fx), self._ptr(fy), self._ptr(fz), self._ptr(out)))
+        return out

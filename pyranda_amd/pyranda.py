@@ -1,0 +1,445 @@
+"""pyrandaSim: pyranda's simulation object over GPU-resident fields.
+
+Same public API as pyranda/pyranda.py:32-938 (EOM, setIC, rk4, eval, var,
+addPackage, ...), with every field a torch tensor in HBM and every compact
+operator a HIP kernel from libpyranda_amd.so. Nothing crosses to the host
+inside a time step except the scalar reductions the EOM asks for (dt.courant,
+sum(...)), exactly where the reference does an MPI allreduce.
+
+Precision: `precision="fp64"` (default) keeps every field in FP64 like the
+reference. `precision="fp32"` stores fields in FP32 and runs the operators as
+FP32 sweeps plus one FP64-residual refinement; the RK4 state accumulation
+(PHI and the conserved variables) stays in FP64 shadows.
+"""
+import math
+import random
+import sys
+import time as _time
+
+import torch
+
+from .devnumpy import numpy, scipy, set_default  # noqa: F401  (EOM/IC strings use these names)
+from .pyrandaEq import pyrandaEq
+from .pyrandaMesh import pyrandaMesh, defaultMeshOptions
+from .pyrandaMPI import pyrandaMPI
+from .pyrandaUtils import fortran3d, findVar, splitLines  # noqa: F401
+from .pyrandaVar import pyrandaVar
+
+# 5-stage, 2N-storage fourth-order Runge-Kutta coefficients (Carpenter &
+# Kennedy 1994), as pyranda.py:760-779 uses them.
+RK_A = [0.0, -6234157559845. / 12983515589748., -6194124222391. / 4410992767914.,
+        -31623096876824. / 15682348800105., -12251185447671. / 11596622555746.]
+RK_B = [494393426753. / 4806282396855., 4047970641027. / 5463924506627.,
+        9795748752853. / 13190207949281., 4009051133189. / 8539092990294.,
+        1348533437543. / 7166442652324.]
+RK_ETA = [494393426753. / 4806282396855., 4702696611523. / 9636871101405.,
+          3614488396635. / 5249666457482., 9766892798963. / 10823461281321., 1.0]
+
+
+class pyrandaSim:
+
+    def __init__(self, name, meshOptions, silent=False, precision="fp64", device="cuda"):
+        self.name = name
+        self.silent = silent
+        self.mesh = pyrandaMesh()
+        if isinstance(meshOptions, str):
+            self.mesh.makeMeshStr(meshOptions)
+            meshOptions = self.mesh.options
+        opts = defaultMeshOptions()
+        opts.update(meshOptions)
+        meshOptions = opts
+        self.meshOptions = opts
+        self.mesh.options = opts
+        if "coordsys" in opts:
+            self.mesh.coordsys = opts["coordsys"]
+        else:
+            raise ValueError("No suitable mesh type specified.")
+        if precision not in ("fp64", "fp32"):
+            raise ValueError("precision must be 'fp64' or 'fp32'")
+        self.precision = precision
+        self.dtype = torch.float64 if precision == "fp64" else torch.float32
+        if not torch.cuda.is_available():
+            raise RuntimeError("pyranda_amd needs a HIP device (no GPU visible)")
+        nx, ny, nz = opts["nn"]
+        self.dx = (opts["xn"][0] - opts["x1"][0]) / max(nx - 1, 1)
+        self.dy = (opts["xn"][1] - opts["x1"][1]) / max(ny - 1, 1)
+        self.dz = (opts["xn"][2] - opts["x1"][2]) / max(nz - 1, 1)
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.npts = nx * ny * nz
+        self.equations = []
+        self.conserved = []
+        self.variables = {}
+        self.initial_conditions = []
+        self.nPDE = self.nALG = self.nELP = 0
+        self.mesh.dims = opts["dim"]
+        self.PyMPI = pyrandaMPI(self.mesh, comm=opts.get("comm"), dtype=self.dtype, device=device)
+        self.mesh.PyMPI = self.PyMPI
+        self.mesh.makeMesh()
+        set_default(self.PyMPI.device, self.dtype)
+        self.zero = self.PyMPI.emptyScalar()
+        self.packages = {}
+        self.packagesRestart = []
+        self.userDefined = {}
+        self.get_sMap()
+        self.time = 0.0
+        self.cycle = 0
+        self.deltat = 0.0
+        self._shadow = {}  # FP64 RK state of conserved variables (fp32 mode)
+
+    # -- bookkeeping -------------------------------------------------------------
+    def iprint(self, sprnt, wpm=0):
+        if self.PyMPI.master and not self.silent:
+            print(sprnt)
+            sys.stdout.flush()
+
+    def addPackage(self, package):
+        self.packages[package.name] = package
+        self.packagesRestart.append(package.__module__)
+        package.get_sMap()
+        self.sMap.update(package.sMap)
+
+    def addUserDefinedFunction(self, name, equation):
+        if name in self.sMap:
+            self.iprint("Error: Cant add user-defined equation '%s', it already exists" % name)
+        self.userDefined[name] = equation
+        self.sMap.update({"%s(" % name: "self.userDefined['%s'](self," % name})
+
+    def allocate(self):
+        for v in self.variables:
+            self.variables[v].__allocate__(self.PyMPI)
+
+    def var(self, name):
+        if name in self.variables:
+            return self.variables[name]
+        raise ValueError("Error: variable name: %s not found in database" % name)
+
+    def x(self):
+        return self.variables["meshx"]
+
+    def y(self):
+        return self.variables["meshy"]
+
+    def z(self):
+        return self.variables["meshz"]
+
+    def eval(self, expression):
+        return eval(fortran3d(expression, self.sMap))
+
+    def parse(self, expression, parseDict=None):
+        if parseDict:
+            expression = self.updateStringFromDictionary(expression, parseDict, "parse")
+        exec(fortran3d(expression, self.sMap))
+
+    def addVar(self, name, kind=None, rank="scalar"):
+        if name not in self.variables:
+            self.variables[name] = pyrandaVar(name, kind, rank)
+
+    def addEqu(self, equation):
+        peq = pyrandaEq(equation, self.sMap, self.PyMPI)
+        self.equations.append(peq)
+        if peq.kind == "PDE":
+            self.nPDE += 1
+            self.conserved.append(peq.LHS[0])
+            if len(peq.LHS) > 1:
+                raise ValueError("only single return values for PDEs allowed")
+        elif peq.kind == "ALG":
+            self.nALG += 1
+        else:
+            raise ValueError("No suitable equation type specified for %s" % peq.eqstr)
+
+    def updateStringFromDictionary(self, string, dictionary, handle):
+        for item in dictionary:
+            if item in string:
+                string = string.replace(item, str(dictionary[item]))
+            else:
+                self.iprint("Notice: Given %s dictionary value: '%s' does not exists in the %s string."
+                            % (handle, item, handle))
+        return string
+
+    # -- EOM / IC (pyranda.py:231-369) ------------------------------------------
+    def EOM(self, eom, eomDict=False):
+        if eomDict:
+            eom = self.updateStringFromDictionary(eom, eomDict, "EOM")
+        self.eom = eom
+        eom_lines = [el for el in eom.split("\n") if el.strip()]
+        eom_lines = [el for el in eom_lines if el.strip()[0] != "#"]  # comments
+        names = []
+        for eq in eom_lines:
+            names += findVar(eq, "scalar")
+        for evar in sorted(set(names)):
+            self.addVar(evar, kind="conserved")
+        names = []
+        for eq in eom_lines:
+            names += findVar(eq, "vector")
+        for evar in sorted(set(names)):
+            self.addVar(evar, rank="vector", kind="conserved")
+        for variable in self.variables:
+            self.iprint("Adding variables: %s" % variable)
+        self.iprint("Adding equations of motion: ")
+        for n, eq in enumerate(eom_lines, 1):
+            self.addEqu(eq)
+            self.iprint("(%s)   %s" % (n, eq))
+        self.allocate()
+        self.variables["meshx"] = self.mesh.coords[0]
+        self.variables["meshy"] = self.mesh.coords[1]
+        self.variables["meshz"] = self.mesh.coords[2]
+
+    def checkForNan(self, names=None):
+        svars = ""
+        for ivar in (names or list(self.variables)):
+            if ivar not in self.variables:
+                self.iprint("%s is not a variable" % ivar)
+                continue
+            d = self.variables[ivar].data
+            bad = torch.isnan(d).any().item() if torch.is_tensor(d) else (d is not None and math.isnan(float(d)))
+            if bad:
+                svars += ivar + " "
+        return svars
+
+    def setIC(self, ics, icDict=False, addOnly=False, verbose=False):
+        if icDict:
+            ics = self.updateStringFromDictionary(ics, icDict, "ICs")
+        self.ics = ics
+        ic_lines = splitLines(ics)
+        names = []
+        for eq in ic_lines:
+            names += findVar(eq, "scalar")
+        for evar in sorted(set(names)):
+            self.addVar(evar, kind="conserved")
+        self.allocate()
+        self.initial_conditions.extend(ic_lines)
+        if addOnly:
+            return
+        # ICs are evaluated in FP64 (FP64 coordinates), then stored in the run's dtype
+        scope = {"self": self}
+        for ic in ic_lines:
+            if verbose:
+                self.iprint(ic)
+            try:
+                exec(fortran3d(ic, self.sMap), globals(), scope)
+            except Exception as e:
+                self.iprint("Error: cant parse following string from ICs")
+                self.iprint(ic)
+                self.iprint("Error message: %s" % e)
+                raise
+        self._cast_fields()
+        snans = self.checkForNan(sorted(set(names)))
+        if snans:
+            raise FloatingPointError("Found some nans in inits: %s" % snans)
+        self.updateVars(verbose=verbose)
+        snans = self.checkForNan()
+        if snans:
+            raise FloatingPointError("Found some nans in Update after init: %s" % snans)
+
+    def _cast_fields(self):
+        for name, var in self.variables.items():
+            if name.startswith("mesh"):
+                continue
+            d = var.data
+            if torch.is_tensor(d) and d.is_floating_point() and d.dtype != self.dtype:
+                var.data = d.to(self.dtype)
+
+    def emptyScalar(self, val=0.0):
+        return self.PyMPI.emptyScalar() + val
+
+    # -- time integration (pyranda.py:376-427, 758-810) --------------------------
+    def updateFlux(self):
+        flux = {}
+        for eqo in self.equations:
+            if eqo.kind == "PDE":
+                flux[eqo.LHS[0]] = eqo.RHS(self)
+        return flux
+
+    def updateVars(self, verbose=False):
+        for eq in self.equations:
+            if not eq.active or eq.kind != "ALG":
+                continue
+            if verbose:
+                self.iprint(eq.eqstr)
+            rhs = eq.RHS(self)
+            if not eq.LHS:
+                continue
+            if eq.rank == 1:
+                self.variables[eq.LHS[0]].data = rhs
+            else:
+                for ii in range(len(rhs)):
+                    self.variables[eq.LHS[ii]].data = rhs[ii]
+
+    def rk4(self, time, dt):
+        shape = tuple(self.mesh.shape)
+        fp32 = self.dtype != torch.float64
+        PHI = {}
+        for U in self.conserved:
+            PHI[U] = self.PyMPI.emptyScalar().to(torch.float64) if fp32 else self.PyMPI.emptyScalar()
+            if fp32:
+                self._shadow[U] = self.variables[U].data.to(torch.float64)
+        time_i = time
+        self.deltat = dt
+        for ii in range(5):
+            FLUX = self.updateFlux()
+            for U in self.conserved:
+                tmp1 = RK_A[ii] * PHI[U]
+                PHI[U] = dt * FLUX[U] + tmp1
+                tmp2 = RK_B[ii] * PHI[U]
+                if fp32:
+                    self._shadow[U] = self._shadow[U] + tmp2
+                    self.variables[U].data = self._shadow[U].to(self.dtype)
+                else:
+                    self.variables[U].data = self.variables[U].data + tmp2
+            time = time_i + RK_ETA[ii] * dt
+            self.time = time
+            if fp32:
+                before = {U: self.variables[U].data for U in self.conserved}
+            self.updateVars()
+            if fp32:
+                for U in self.conserved:
+                    if self.variables[U].data is not before[U]:
+                        self._shadow[U] = self.variables[U].data.to(torch.float64)
+        del shape
+        self.cycle += 1
+        return time
+
+    def step(self, nsteps=1):
+        for _ in range(nsteps):
+            self.rk4(self.time, self.deltat)
+
+    def euler(self, time, dt):
+        FLUX = self.updateFlux()
+        for U in self.conserved:
+            self.variables[U].data = self.variables[U].data + FLUX[U] * dt
+        time = time + dt
+        self.time = time
+        self.updateVars()
+        return time
+
+    # -- operators (pyranda.py:607-736) -------------------------------------------
+    def ddx(self, val):
+        if self.nx <= 1:
+            return 0.0
+        return self.PyMPI.der.ddx(val)
+
+    def ddy(self, val):
+        if self.ny <= 1:
+            return 0.0
+        return self.PyMPI.der.ddy(val)
+
+    def ddz(self, val):
+        if self.nz <= 1:
+            return 0.0
+        return self.PyMPI.der.ddz(val)
+
+    def dd4x(self, val): return self.PyMPI.der.dd4x(val)
+    def dd4y(self, val): return self.PyMPI.der.dd4y(val)
+    def dd4z(self, val): return self.PyMPI.der.dd4z(val)
+    def dd8x(self, val): return self.PyMPI.der.dd8x(val)
+    def dd8y(self, val): return self.PyMPI.der.dd8y(val)
+    def dd8z(self, val): return self.PyMPI.der.dd8z(val)
+
+    def div(self, f1, f2=None, f3=None):
+        z = self.zero
+        if f2 is None and f3 is None:
+            if self.nx > 1:
+                return self.PyMPI.der.div(f1, z, z)
+            if self.ny > 1:
+                return self.PyMPI.der.div(z, f1, z)
+            if self.nz > 1:
+                return self.PyMPI.der.div(z, z, f1)
+            return 0.0
+        if f3 is None:
+            if self.nx > 1 and self.ny > 1:
+                return self.PyMPI.der.div(f1, f2, z)
+            if self.nz > 1 and self.ny > 1:
+                return self.PyMPI.der.div(z, f1, f2)
+            if self.nz > 1 and self.nx > 1:
+                return self.PyMPI.der.div(f1, z, f2)
+            if self.nx > 1:
+                return self.PyMPI.der.div(f1, z, z)
+            if self.ny > 1:
+                return self.PyMPI.der.div(z, f1, z)
+            if self.nz > 1:
+                return self.PyMPI.der.div(z, z, f1)
+            return 0.0
+        return self.PyMPI.der.div(f1, f2, f3)
+
+    def divT(self, *args):
+        return self.PyMPI.der.divT(*args)
+
+    def grad(self, val):
+        return self.PyMPI.der.grad(val)
+
+    def laplacian(self, val):
+        return self.PyMPI.der.laplacian(val)
+
+    def ring(self, val):
+        return self.PyMPI.der.ring(val)
+
+    def ringV(self, vx, vy, vz):
+        return self.PyMPI.der.ringV(vx, vy, vz)
+
+    def filter(self, val):
+        return self.PyMPI.fil.filter(val)
+
+    def gfilterx(self, val):
+        return self.PyMPI.gfil.filterDir(val, 1)
+
+    def gfiltery(self, val):
+        return self.PyMPI.gfil.filterDir(val, 2)
+
+    def gfilterz(self, val):
+        return self.PyMPI.gfil.filterDir(val, 3)
+
+    def gfilter(self, val):
+        return self.PyMPI.gfil.filter(val)
+
+    # -- string map (pyranda.py:812-869) -------------------------------------------
+    def get_sMap(self):
+        sMap = {}
+        sMap["div("] = "self.div("
+        sMap["divT("] = "self.divT("
+        sMap["ddx("] = "self.ddx("
+        sMap["ddy("] = "self.ddy("
+        sMap["ddz("] = "self.ddz("
+        sMap["fbar("] = "self.filter("
+        sMap["gbar("] = "self.gfilter("
+        sMap["gbarx("] = "self.gfilterx("
+        sMap["gbary("] = "self.gfiltery("
+        sMap["gbarz("] = "self.gfilterz("
+        sMap["grad("] = "self.grad("
+        sMap["simtime"] = "self.time"
+        sMap["deltat"] = "self.deltat"
+        sMap["lap("] = "self.laplacian("
+        sMap["ring("] = "self.ring("
+        sMap["ringV("] = "self.ringV("
+        sMap["dd4x("] = "self.dd4x("
+        sMap["dd4y("] = "self.dd4y("
+        sMap["dd4z("] = "self.dd4z("
+        sMap["dd8x("] = "self.dd8x("
+        sMap["dd8y("] = "self.dd8y("
+        sMap["dd8z("] = "self.dd8z("
+        sMap["sum("] = "self.PyMPI.sum3D("
+        sMap["mean("] = "1.0/float(self.npts) * self.PyMPI.sum3D("
+        sMap["mean3D("] = "self.PyMPI.mean3D("
+        sMap["sign("] = "numpy.sign("
+        sMap["dot("] = "numpy.dot("
+        sMap["abs("] = "numpy.abs("
+        sMap["sqrt("] = "numpy.sqrt("
+        sMap["sin("] = "numpy.sin("
+        sMap["cos("] = "numpy.cos("
+        sMap["tanh("] = "numpy.tanh("
+        sMap["exp("] = "numpy.exp("
+        sMap["where("] = "numpy.where("
+        sMap["max("] = "self.PyMPI.max3D("
+        sMap["min("] = "self.PyMPI.min3D("
+        sMap["3d("] = "self.emptyScalar("
+        sMap["pi"] = "numpy.pi"
+        sMap["meshVar("] = "self.PyMPI.getVar("
+        sMap["gridLen"] = "self.mesh.GridLen"
+        sMap["random3D()"] = "numpy.random.random( self.mesh.coords[0].data.shape) "
+        sMap["erf("] = "scipy.special.erf("
+        sMap["meshx"] = "self.mesh.coords[0].data"
+        sMap["meshy"] = "self.mesh.coords[1].data"
+        sMap["meshz"] = "self.mesh.coords[2].data"
+        sMap["meshi"] = "self.mesh.indices[0].data"
+        sMap["meshj"] = "self.mesh.indices[1].data"
+        sMap["meshk"] = "self.mesh.indices[2].data"
+        self.sMap = sMap

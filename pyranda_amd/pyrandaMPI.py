@@ -1,0 +1,156 @@
+"""Single-GPU runtime standing where pyranda's pyrandaMPI stands.
+
+pyrandaMPI (pyranda/pyrandaMPI.py) owns the decomposition, the parcop setup
+and the der/fil/gfil operator objects, plus global reductions. On one MI355X
+the whole block is one rank's share, so this class keeps the same attribute
+and method names (PyMPI.der.ddx, PyMPI.sum3D, emptyScalar, x1proc, ...) with
+fields resident in HBM as torch tensors and operators going through
+libpyranda_amd.so. Reductions (sum3D/max3D/min3D, pyrandaMPI.py:307-326)
+return Python floats, as the allreduce does.
+"""
+import torch
+
+from .parcop import ParcopContext, empty_field
+
+
+class _Der:
+    """parcop_der (pyrandaMPI.py:664-712)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def ddx(self, v): return self.ctx.ddx(v)
+    def ddy(self, v): return self.ctx.ddy(v)
+    def ddz(self, v): return self.ctx.ddz(v)
+    def dd8x(self, v): return self.ctx.dd8x(v)
+    def dd8y(self, v): return self.ctx.dd8y(v)
+    def dd8z(self, v): return self.ctx.dd8z(v)
+
+    def dd4x(self, v):
+        raise NotImplementedError("dd4x (explicit 4th derivative e4d4) is outside this build")
+
+    dd4y = dd4z = dd4x
+
+    def div(self, fx, fy, fz): return self.ctx.divergence(fx, fy, fz)
+
+    def divT(self, *a):
+        raise NotImplementedError("divT (tensor divergence) is outside this build")
+
+    def grad(self, v): return self.ctx.grads(v)
+    def laplacian(self, v): return self.ctx.plaplacian(v)
+    def ring(self, v): return self.ctx.pring(v)
+    def ringV(self, vx, vy, vz): return self.ctx.pringv(vx, vy, vz)
+
+
+class _GFil:
+    """parcop_gfil (pyrandaMPI.py:721-730)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def filter(self, v): return self.ctx.gfilter(v)
+    def filterDir(self, v, direction): return self.ctx.gfilterdir(v, direction)
+
+
+class _SFil:
+    """parcop_sfil (pyrandaMPI.py:734-740)."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+
+    def filter(self, v): return self.ctx.sfilter(v)
+
+
+class pyrandaMPI:
+
+    def __init__(self, mesh, comm=None, dtype=torch.float64, device="cuda"):
+        opts = mesh.options
+        self.nx, self.ny, self.nz = (int(n) for n in opts["nn"])
+        x1, xn = opts["x1"], opts["xn"]
+        self.dx = (xn[0] - x1[0]) / max(self.nx - 1, 1)
+        self.dy = (xn[1] - x1[1]) / max(self.ny - 1, 1)
+        self.dz = (xn[2] - x1[2]) / max(self.nz - 1, 1)
+        self.periodic = opts["periodic"]
+        symmetric = opts["symmetric"]
+        self.coordsys = opts["coordsys"]
+        if self.coordsys != 0:
+            raise NotImplementedError("only Cartesian meshes (coordsys=0) are built")
+        self.comm = comm
+        self.master = True
+        self.dtype = dtype
+        self.device = torch.device(device)
+        # BC strings as pyrandaMPI.py:101-131 builds them
+        bcs = []
+        for d in range(3):
+            for side in range(2):
+                b = "NONE"
+                if self.periodic[d]:
+                    b = "PERI"
+                if symmetric[d][side]:
+                    b = "SYMM"
+                bcs.append(b)
+        self.bcs = bcs
+        self.ctx = ParcopContext((self.nx, self.ny, self.nz), (x1[0], x1[1], x1[2]),
+                                 (xn[0], xn[1], xn[2]), bcs, device=self.device)
+        self.ax, self.ay, self.az = self.nx, self.ny, self.nz
+        self.chunk_3d_size = [self.nx, self.ny, self.nz]
+        self.chunk_3d_lo = [0, 0, 0]
+        self.chunk_3d_hi = [self.nx - 1, self.ny - 1, self.nz - 1]
+        # physical-boundary ownership flags (mesh.f90:162-167)
+        self.x1proc = (not self.periodic[0]) and self.nx > 1
+        self.xnproc = self.x1proc
+        self.y1proc = (not self.periodic[1]) and self.ny > 1
+        self.ynproc = self.y1proc
+        self.z1proc = (not self.periodic[2]) and self.nz > 1
+        self.znproc = self.z1proc
+        self.der = _Der(self.ctx)
+        self.gfil = _GFil(self.ctx)
+        self.fil = _SFil(self.ctx)
+
+    # -- allocation (pyrandaMPI.py:249-254) ---------------------------------
+    def emptyScalar(self):
+        return torch.zeros((self.nz, self.ny, self.nx), dtype=self.dtype, device=self.device).permute(2, 1, 0)
+
+    def emptyVector(self, rank=3):
+        return torch.zeros((rank, self.nz, self.ny, self.nx), dtype=self.dtype,
+                           device=self.device).permute(3, 2, 1, 0)
+
+    def setPatch(self):
+        """Operator tables are per context; nothing global to switch."""
+
+    # -- reductions -----------------------------------------------------------
+    def sum3D(self, data):
+        if torch.is_tensor(data):
+            return torch.sum(data, dtype=torch.float64).item()
+        import numpy as np
+        return float(np.sum(data))
+
+    def max3D(self, data):
+        if torch.is_tensor(data):
+            return torch.max(data).item()
+        import numpy as np
+        return float(np.max(data))
+
+    def min3D(self, data):
+        if torch.is_tensor(data):
+            return torch.min(data).item()
+        import numpy as np
+        return float(np.min(data))
+
+    # -- line/plane reductions used by example scripts ------------------------
+    def xsum(self, data):
+        return torch.sum(data, dim=0, dtype=torch.float64).cpu().numpy()
+
+    def ysum(self, data):
+        return torch.sum(data, dim=1, dtype=torch.float64).cpu().numpy()
+
+    def zsum(self, data):
+        return torch.sum(data, dim=2, dtype=torch.float64).cpu().numpy()
+
+    xbar, ybar, zbar = xsum, ysum, zsum
+
+    def subsum3xz(self, data):
+        return torch.sum(data, dim=(0, 2), dtype=torch.float64).cpu().numpy()
+
+    def iprint(self, s):
+        print(s, flush=True)

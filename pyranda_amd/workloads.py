@@ -1,0 +1,122 @@
+"""Workload definitions: Taylor-Green EOM strings and seeded synthetic fields.
+
+Operator microbenchmark field (256^3 periodic): a sum of 64 random Fourier
+modes with integer wavevectors 1 <= |k| <= 32 drawn uniformly in the ball by
+rejection, amplitudes |k|^(-5/3) and uniform phases, seed 1234. Its exact
+derivatives/filters are known mode by mode, which gives size-independent
+checks at full size.
+"""
+import math
+
+import numpy as np
+import torch
+
+
+def fourier_modes(n_modes=64, kmax=32, seed=1234):
+    rng = np.random.default_rng(seed)
+    ks = []
+    while len(ks) < n_modes:
+        k = rng.integers(-kmax, kmax + 1, size=3)
+        r = math.sqrt(float((k * k).sum()))
+        if 1.0 <= r <= kmax:
+            ks.append(k)
+    ks = np.array(ks, dtype=np.int64)
+    amps = np.sqrt((ks * ks).sum(1).astype(np.float64)) ** (-5.0 / 3.0)
+    phis = rng.uniform(0.0, 2.0 * np.pi, n_modes)
+    return ks, amps, phis
+
+
+def periodic_axis(n, device, dtype=torch.float64):
+    """Node coordinates of xdom=(0, 2*pi*(n-1)/n, n, periodic=True)."""
+    return torch.arange(n, device=device, dtype=dtype) * (2.0 * math.pi / n)
+
+
+def fourier_field(n, device, modes=None, weights=None, dtype=torch.float64):
+    """sum_m w_m a_m cos(k_m . x + phi_m) on an n^3 periodic grid, Fortran layout.
+
+    `weights` (per mode) lets callers build exact operator results, e.g. the
+    transfer function of an operator evaluated at each mode's wavenumber.
+    Returns a (n,n,n) view with x fastest.
+    """
+    ks, amps, phis = modes if modes is not None else fourier_modes()
+    x = periodic_axis(n, device, dtype)
+    out = torch.zeros((n, n, n), device=device, dtype=dtype)  # indexed [z, y, x]
+    for m in range(len(amps)):
+        a = amps[m] * (1.0 if weights is None else weights[m])
+        if a == 0.0:
+            continue
+        kx, ky, kz = (int(v) for v in ks[m])
+        arg = (kx * x)[None, None, :] + (ky * x)[None, :, None] + (kz * x)[:, None, None] + phis[m]
+        out += a * torch.cos(arg)
+    return out.permute(2, 1, 0)
+
+
+# Taylor-Green Euler+LAD equations and initial conditions, verbatim from
+# examples/TaylorGreen.py:40-106 (the reference's EOM strings are its API).
+TG_EOM = """
+# Primary Equations of motion here
+ddt(:rho:)  =  -ddx(:rho:*:u:)            - ddy(:rho:*:v:)               - ddz(:rho:*:w:)
+ddt(:rhou:) =  -ddx(:rhou:*:u: - :tauxx:) - ddy(:rhou:*:v: - :tauxy:)    - ddz(:rhou:*:w: - :tauxz:)
+ddt(:rhov:) =  -ddx(:rhov:*:u: - :tauxy:) - ddy(:rhov:*:v: - :tauyy:)    - ddz(:rhov:*:w: - :tauyz:)
+ddt(:rhow:) =  -ddx(:rhow:*:u: - :tauxz:) - ddy(:rhow:*:v: - :tauyz:)    - ddz(:rhow:*:w: - :tauzz:)
+ddt(:Et:)   =  -ddx( (:Et: - :tauxx:)*:u: - :tauxy:*:v: - :tauxz:*:w: )  - ddy( (:Et: - :tauyy:)*:v: -:tauxy:*:u: - :tauyz:*:w:) - ddz( (:Et: - :tauzz:)*:w: - :tauxz:*:u: - :tauyz:*:v: )
+# Conservative filter of the EoM
+:rho:       =  fbar( :rho:  )
+:rhou:      =  fbar( :rhou: )
+:rhov:      =  fbar( :rhov: )
+:rhow:      =  fbar( :rhow: )
+:Et:        =  fbar( :Et:   )
+# Update the primatives and enforce the EOS
+:u:         =  :rhou: / :rho:
+:v:         =  :rhov: / :rho:
+:w:         =  :rhow: / :rho:
+:p:         =  ( :Et: - .5*:rho:*(:u:*:u: + :v:*:v: + :w:*:w:) ) * ( :gamma: - 1.0 )
+# Artificial bulk viscosity
+:ux:        =  ddx(:u:)
+:vy:        =  ddy(:v:)
+:wz:        =  ddz(:w:)
+:div:       =  :ux: + :vy: + :wz:
+# Remaining cross derivatives
+:uy:        =  ddy(:u:)
+:uz:        =  ddz(:u:)
+:vx:        =  ddx(:v:)
+:vz:        =  ddz(:v:)
+:wy:        =  ddy(:w:)
+:wx:        =  ddx(:w:)
+:enst:      = sqrt( (:uy:-:vx:)**2 + (:uz: - :wx:)**2 + (:vz:-:wy:)**2 )
+:tke:       = :rho:*(:u:*:u: + :v:*:v: + :w:*:w:)
+:S:         = sqrt( :ux:*:ux: + :vy:*:vy: + :wz:*:wz: + .5*((:uy:+:vx:)**2 + (:uz: + :wx:)**2 + (:vz:+:wy:)**2) )
+:mu:        =  gbar( abs(ring(:S:  )) ) * :rho: * 1.0e-4
+:beta:      =  gbar( abs(ring(:div:)) * :rho: )  * 7.0e-3
+:taudia:    =  (:beta:-2./3.*:mu:) *:div: - :p:
+:tauxx:     =  2.0*:mu:*:ux:   + :taudia:
+:tauyy:     =  2.0*:mu:*:vy:   + :taudia:
+:tauzz:     =  2.0*:mu:*:wz:   + :taudia:
+:tauxy:     = :mu:*(:uy:+:vx:)
+:tauxz:     = :mu:*(:uz:+:wx:)
+:tauyz:     = :mu:*(:vz:+:wz:)
+:cs:  = sqrt( :p: / :rho: * :gamma: )
+:dt: = dt.courant(:u:,:v:,:w:,:cs:)*1.0
+:dt: = numpy.minimum(:dt:,0.2 * dt.diff(:beta:,:rho:))
+:dt: = numpy.minimum(:dt:,0.2 * dt.diff(:mu:,:rho:))
+"""
+
+TG_IC = """
+:gamma: = 1.4
+u0 = 1.0
+p0 = 100.0
+rho0 = 1.0
+L = 1.0
+:u: =  u0*sin(meshx/L)*cos(meshy/L)*cos(meshz/L)
+:v: = -u0*cos(meshx/L)*sin(meshy/L)*cos(meshz/L)
+:w: = 0.0*:u:
+:p:  = p0 + rho0/16.0*( ( cos(2.*meshx/L) + cos(2.*meshy/L) ) * ( cos(2.*meshz/L) + 2.0 ) - 2.0 )
+:rho: = rho0 + 0.0*:u:
+:rhou: = :rho:*:u:
+:rhov: = :rho:*:v:
+:rhow: = :rho:*:w:
+:Et:  = :p: / (:gamma:-1.0) + 0.5*:rho:*(:u:*:u: + :v:*:v: + :w:*:w:)
+:cs:  = sqrt( :p: / :rho: * :gamma: )
+:tke: = :rho:*(:u:*:u: + :v:*:v: + :w:*:w:)
+:dt: = dt.courant(:u:,:v:,:w:,:cs:)
+"""

@@ -1,0 +1,174 @@
+"""End-to-end parity of the pyrandaSim front end on the GPU.
+
+The reference's own regression scripts, restated through pyranda_amd's
+pyrandaSim (EOM/setIC/rk4/eval), against the values the reference holds:
+unit-test-3D (tests/cases/testUnit.py:3), advec-1d-* (test1DAdvection.py),
+integration-* (testIntegration.py) and taylorgreen (testTaylorGreen.py:3),
+plus the 1-D advection + fbar workload (100 points, 100 RK4 steps) against
+the oracle driven by the same RK4 on the CPU.
+"""
+import json
+import math
+import os
+import sys
+
+import numpy as np
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden", "reference_scalars.json")
+
+
+def _golden(name):
+    with open(GOLDEN) as f:
+        return json.load(f)[name]["value"]
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def pr():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import pyranda_amd
+    return pyranda_amd
+
+
+def test_unit_test_3d(pr):
+    """examples/unit_test.py through pysim.eval."""
+    Nx = 32
+    L = np.pi * 2.0 * (Nx - 1) / Nx
+    domain = """
+    xdom = (0.0 , L , Nx ,periodic=True)
+    ydom = (0.0 , L , Nx ,periodic=True)
+    zdom = (0.0 , L , Nx ,periodic=True)
+    """.replace("L", str(L)).replace("Nx", str(Nx))
+    pysim = pr.pyrandaSim("advection", domain, silent=True)
+    E = 0.0
+    expr = "sum( abs( ddx( sin(meshx) ) - cos(meshx) ))"
+    for e in (expr, expr.replace("x", "y"), expr.replace("x", "z")):
+        E += pysim.eval(e)
+    domain = """
+    xdom = (0.0 , 1.0 , Nx )
+    ydom = (0.0 , 1.0 , Nx )
+    zdom = (0.0 , 1.0 , Nx )
+    """.replace("Nx", str(Nx))
+    pysim2 = pr.pyrandaSim("nonperiodic", domain, silent=True)
+    for expr in ("mean( abs(ddx(meshx*0.0)) )", "mean( abs(ddx(meshx*0.0 + 1.0)) )"):
+        for e in (expr, expr.replace("x", "y"), expr.replace("x", "z")):
+            E += pysim2.eval(e)
+    expr = "mean( abs(ddx(meshx**power) - power*meshx**(power-1) )  )"
+    for power in [1, 2, 3, 4]:
+        ex = expr.replace("power", str(float(power)))
+        for e in (ex, ex.replace("x", "y"), ex.replace("x", "z")):
+            E += pysim2.eval(e)
+    for f in ("fbar", "gbar"):
+        E += pysim2.eval("mean( abs(%s(meshx*0.0)) )" % f)
+        E += pysim2.eval("mean( abs(%s(meshx*0.0 + 1.0) - (meshx*0.0 + 1.0)  ) )" % f)
+        E += pysim2.eval("mean( abs(%s(meshx) - (meshx)  ) )" % f)
+        E += pysim.eval("mean( abs(%s(sin(meshx)) - sin(meshx)  ) )" % f)
+        E += pysim.eval("mean( abs(%s( meshx*0.0 + 1.0) - 1.0  ) )" % f)
+        E += pysim.eval("mean( abs(%s( meshx*0.0 )  ) )" % f)
+    want = _golden("unit-test-3D")
+    _say("unit-test-3D", E, want)
+    assert abs(E - want) / want < 1e-9
+
+
+def advection_error(pr, Npts):
+    """examples/advection.py N 1: one period of periodic d1 + RK4."""
+    L = np.pi * 2.0
+    Lp = L * (Npts - 1.0) / Npts
+    imesh = """
+    Lp = %s
+    Npts = %d
+    xdom = (0.0, Lp,  Npts, periodic=True)
+    """ % (Lp, Npts)
+    ss = pr.pyrandaSim("advection", imesh, silent=True)
+    ss.EOM(" ddt(:phi:)  =  -:c: * ddx(:phi:) ")
+    ic = """
+    r   = sqrt( (meshx-pi)**2  )
+    :phi: = 1.0 + 0.1 * exp( -(r/(pi/4.0))**2 )
+    :phi2: = 1.0*:phi:
+    :c:   = 1.0
+    """
+    ss.setIC(ic)
+    v = 1.0
+    dt_max = v / ss.mesh.nn[0] * L * .90
+    tt = L / v * 1.0
+    dt = dt_max
+    time = 0.0
+    while tt > time:
+        time = ss.rk4(time, dt)
+        dt = min(dt_max, (tt - time))
+    phi = ss.variables["phi"].data
+    phi2 = ss.variables["phi2"].data
+    return torch.sum((phi - phi2) ** 2).item()
+
+
+@pytest.mark.parametrize("n", [50, 100, 200, 300, 400])
+def test_advection_1d(pr, n):
+    got = advection_error(pr, n)
+    want = _golden("advec-1d-%d" % n)
+    _say("advec", n, got, want)
+    # the harness's own tolerance is 1e-4 (tests/run_tests.py:84)
+    assert abs(got - want) / want < 1e-4
+
+
+@pytest.mark.parametrize("n", [25, 50, 100, 200, 400])
+def test_rk4_integration(pr, n):
+    """examples/intTest.py N 1: the 5-stage RK4 on a'=a, b'=cos(6t)."""
+    pysim = pr.pyrandaSim("integralTests", "xdom = (0.0 , 1.0 , 1 )", silent=True)
+    pysim.EOM("""
+    ddt(:a:)  =  :a:
+    ddt(:b:)  =  cos( 6.0*simtime )
+    """)
+    pysim.setIC("""
+    :a: = 1.0
+    :b: = 0.0
+    """)
+    dt = 1.0 / float(n)
+    time = 0.0
+    tt, aa = [time], [1.0]
+    while time < 1.0:
+        time = pysim.rk4(time, dt)
+        tt.append(time)
+        aa.append(pysim.variables["a"].data[0, 0, 0].item())
+    tt, aa = np.array(tt), np.array(aa)
+    err = np.sum(np.abs(aa - (1.0 - 1.0 + np.exp(tt))))
+    want = _golden("integration-%d" % n)
+    _say("integration", n, err, want)
+    assert abs(err - want) / want < 1e-4
+
+
+from pyranda_amd.workloads import TG_EOM, TG_IC  # noqa: E402
+
+
+def taylor_green_enstrophy(pr, Npts, tstop=0.1, precision="fp64"):
+    imesh = """
+    xdom = (0.0, 2*pi*FF,  Npts, periodic=True)
+    ydom = (0.0, 2*pi*FF,  Npts, periodic=True)
+    zdom = (0.0, 2*pi*FF,  Npts, periodic=True)
+    """.replace("Npts", str(Npts)).replace("pi", str(np.pi)).replace("FF", str(float(Npts - 1) / Npts))
+    ss = pr.pyrandaSim("TGvortex", imesh, silent=True, precision=precision)
+    ss.addPackage(pr.pyrandaTimestep(ss))
+    ss.EOM(TG_EOM)
+    ss.setIC(TG_IC)
+    CFL = 0.5
+    dt = ss.variables["dt"].data * CFL
+    enst0 = ss.var("enst").sum()
+    time = 0.0
+    while time < tstop:
+        time = ss.rk4(time, dt)
+        dt = ss.variables["dt"].data * CFL
+    return ss.var("enst").sum() / enst0
+
+
+def test_taylor_green_32(pr):
+    got = taylor_green_enstrophy(pr, 32)
+    want = _golden("taylorgreen")
+    _say("taylorgreen", repr(got), want)
+    assert abs(got - want) / want < 1e-6
