@@ -106,6 +106,26 @@ int pyr_divergence_r32(pyr_ctx* ctx, const float* fx, const float* fy, const flo
 int pyr_apply(pyr_ctx* ctx, int op, int dir, int prec, const void* in, void* out, int epi,
               double epi_scale);
 
+/* --- one RK4 stage update of a conserved variable (pyranda.py:800-804) ----
+ * PHI <- dt*F + A*PHI (in place, FP64), then
+ *   PYR_F64: U_out <- U_in + B*PHI                (double fields)
+ *   PYR_R32: U64 <- U64 + B*PHI (FP64 state), U_out <- (float)U64
+ * F, U_in, U_out have the field precision; n = number of points. */
+int pyr_rk_update(pyr_ctx* ctx, int prec, long long n, const void* F, double* PHI,
+                  const void* U_in, void* U_out, double* U64, double dt, double A, double B);
+
+/* --- fused elementwise kernels for EOM pointwise algebra ----------------
+ * No counterpart in the reference (its pointwise algebra is numpy, pyrandaEq.py:39).
+ * Source is generated by pyranda_amd/fuse.py: one kernel taking a by-value
+ * argument block {long long n; void* p[PYR_PW_MAX_PTRS]; double s[PYR_PW_MAX_SCALARS];}. */
+#define PYR_PW_MAX_PTRS 24
+#define PYR_PW_MAX_SCALARS 16
+int pyr_pw_compile(const char* source, const char* kernel_name, void** fn);
+int pyr_pw_check_source(const char* source, const char* kernel_name);
+int pyr_pw_launch(void* fn, void* stream, long long n, int nptr, void* const* ptrs, int nscal,
+                  const double* scal);
+const char* pyr_pw_last_error(void);
+
 /* --- host-side checks (no GPU needed) ----------------------------------- */
 /* Weight tables as the library restates them (stencils.f90), for comparison
  * with the oracle: ali[5], ari[9], alb1/alb2[3*4*5], arb1/arb2[3*4*9]. */

@@ -3,7 +3,7 @@ import os
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRC = os.path.join(HERE, "csrc", "pyr_core.hip")
+SRCS = [os.path.join(HERE, "csrc", "pyr_core.hip"), os.path.join(HERE, "csrc", "pyr_fuse.cpp")]
 OUT = os.path.join(HERE, "libpyranda_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -16,7 +16,7 @@ FLAGS = [
 
 def build(verbose=False):
     tmp = OUT + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-o", tmp, SRC]
+    cmd = [HIPCC] + FLAGS + ["-o", tmp] + SRCS + ["-lhiprtc"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)

@@ -30,7 +30,15 @@ SIGNATURES = {
     "pyr_apply": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int, ctypes.c_double]),
     "pyr_host_weights": (ctypes.c_int, [ctypes.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _ip]),
     "pyr_host_emulate": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]),
+    "pyr_rk_update": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong, _vp, _vp, _vp, _vp, _vp,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    "pyr_pw_compile": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_vp)]),
+    "pyr_pw_check_source": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p]),
+    "pyr_pw_launch": (ctypes.c_int, [_vp, _vp, ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(_vp),
+                                     ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "pyr_pw_last_error": (ctypes.c_char_p, []),
 }
+PW_MAX_PTRS, PW_MAX_SCALARS = 24, 16
 for _prec in ("f64", "r32"):
     for _name in ("ddx", "ddy", "ddz", "plaplacian", "pring", "sfilter", "gfilter"):
         SIGNATURES["pyr_%s_%s" % (_name, _prec)] = (ctypes.c_int, [_vp, _vp, _vp])
@@ -68,4 +76,10 @@ def lib():
 def check(status):
     if status != 0:
         raise PyrandaAMDError(lib().pyr_last_error().decode())
+    return status
+
+
+def check_pw(status):
+    if status != 0:
+        raise PyrandaAMDError(lib().pyr_pw_last_error().decode())
     return status

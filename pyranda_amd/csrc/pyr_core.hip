@@ -46,52 +46,87 @@ template <> struct SolveT<MODE_R32> { using type = float; };
 
 constexpr int WG_MAX = 512;
 
+// Coefficient tables of one operator, as the solve reads them (LDS copies on
+// the device, plan vectors in the host emulation).
+struct Tabs {
+  const double* lu = nullptr;
+  const double* rc = nullptr;
+  const double* gsel = nullptr;
+  const float* lu32 = nullptr;
+  const float* rc32 = nullptr;
+  const float* gsel32 = nullptr;
+  const double* band = nullptr;
+};
+
+__host__ __device__ inline size_t align16(size_t b) { return (b + 15) & ~(size_t)15; }
+
+// bytes of the staged tables: FP64 lu|rc|gsel, or FP32 lu|rc|gsel + FP64 band
+template <int MODE>
+__host__ __device__ inline size_t stage_bytes(int ntab, int N) {
+  return MODE == MODE_F64 ? align16((size_t)ntab * sizeof(double))
+                          : align16((size_t)ntab * sizeof(float)) + align16((size_t)5 * N * sizeof(double));
+}
+
+template <int MODE>
+__device__ __forceinline__ Tabs stage_tables(const LineArgs& a, unsigned char* dst, int tid, int nthr) {
+  Tabs t;
+  if (MODE == MODE_F64) {
+    double* d = reinterpret_cast<double*>(dst);
+    for (int e = tid; e < a.ntab; e += nthr) d[e] = a.tab64[e];
+    t.lu = d;
+    t.rc = d + 5 * a.N;
+    t.gsel = d + 9 * a.N;
+  } else {
+    float* f = reinterpret_cast<float*>(dst);
+    for (int e = tid; e < a.ntab; e += nthr) f[e] = a.tab32[e];
+    double* bd = reinterpret_cast<double*>(dst + align16((size_t)a.ntab * sizeof(float)));
+    for (int e = tid; e < 5 * a.N; e += nthr) bd[e] = a.band[e];
+    t.lu32 = f;
+    t.rc32 = f + 5 * a.N;
+    t.gsel32 = f + 9 * a.N;
+    t.band = bd;
+  }
+  return t;
+}
+
 // Solve one chunk given its RHS; the LDS exchange of edge values goes through
 // Y (4P x ly entries of the solve type), column `col` belongs to this line.
+// The solution replaces the RHS in `rhs`.
 template <bool IMPL, int MODE, typename CS>
-__device__ __forceinline__ void solve_chunk(const LineArgs& a, int p, int s, int m, CS* Y, int ly,
-                                            int col, double* rhs, double* y) {
-  if (!IMPL) {
-#pragma unroll
-    for (int k = 0; k < CHUNK; ++k) y[k] = rhs[k];
-    return;
-  }
+__device__ __forceinline__ void solve_chunk(const LineArgs& a, const Tabs& t, int p, int s, int m,
+                                            CS* Y, int ly, int col, double* rhs) {
+  if (!IMPL) return;
   auto yr = [&](int c) { return Y[c * ly + col]; };
   if (MODE == MODE_F64) {
-    double x[CHUNK];
-#pragma unroll
-    for (int k = 0; k < CHUNK; ++k) x[k] = rhs[k];
-    local_solve<double>(a.lu, s, m, x);
+    double* x = rhs;  // solve in place
+    local_solve<double>(t.lu, s, m, x);
     double e4[4];
     edges(x, m, e4);
 #pragma unroll
     for (int q = 0; q < 4; ++q) Y[(4 * p + q) * ly + col] = (CS)e4[q];
     __syncthreads();
     double z[4];
-    neighbours<double>(a.gsel, p, a.P, yr, z);
-    spike_correct(a.rc, s, m, z, x);
-#pragma unroll
-    for (int k = 0; k < CHUNK; ++k) y[k] = x[k];
+    neighbours<double>(t.gsel, p, a.P, yr, z);
+    spike_correct(t.rc, s, m, z, x);
   } else {
     float x[CHUNK];
 #pragma unroll
     for (int k = 0; k < CHUNK; ++k) x[k] = (float)rhs[k];
-    local_solve<float>(a.lu32, s, m, x);
+    local_solve<float>(t.lu32, s, m, x);
     float e4[4];
     edges(x, m, e4);
 #pragma unroll
     for (int q = 0; q < 4; ++q) Y[(4 * p + q) * ly + col] = (CS)e4[q];
     __syncthreads();
     float z[4];
-    neighbours<float>(a.gsel32, p, a.P, yr, z);
-    spike_correct(a.rc32, s, m, z, x);
+    neighbours<float>(t.gsel32, p, a.P, yr, z);
+    spike_correct(t.rc32, s, m, z, x);
     // one refinement step: FP64 residual against the same banded operator
-    double res[CHUNK];
-    residual64(a.band, s, m, rhs, x, z, res);
+    residual64(t.band, s, m, rhs, x, z, rhs);
     float d[CHUNK];
 #pragma unroll
-    for (int k = 0; k < CHUNK; ++k) d[k] = (float)res[k];
-    local_solve<float>(a.lu32, s, m, d);
+    for (int k = 0; k < CHUNK; ++k) d[k] = (float)rhs[k];
+    local_solve<float>(t.lu32, s, m, d);
     float f4[4];
     edges(d, m, f4);
     __syncthreads();
@@ -99,25 +134,29 @@ __device__ __forceinline__ void solve_chunk(const LineArgs& a, int p, int s, int
     for (int q = 0; q < 4; ++q) Y[(4 * p + q) * ly + col] = (CS)f4[q];
     __syncthreads();
     float zd[4];
-    neighbours<float>(a.gsel32, p, a.P, yr, zd);
-    spike_correct(a.rc32, s, m, zd, d);
+    neighbours<float>(t.gsel32, p, a.P, yr, zd);
+    spike_correct(t.rc32, s, m, zd, d);
 #pragma unroll
-    for (int k = 0; k < CHUNK; ++k) y[k] = (double)x[k] + (double)d[k];
+    for (int k = 0; k < CHUNK; ++k) rhs[k] = (double)x[k] + (double)d[k];
   }
 }
 
-// y/z lines. block = (LY lanes, P chunks); LY = 64 makes the chunk index
-// wave-uniform so the tables are read with scalar loads.
+// y/z lines. block = (LY lanes, P chunks). With LY = 64 the chunk index is
+// wave-uniform; with LY = 32 a wave holds two chunks of 32 lines each.
 template <typename TS, int KIND, bool IMPL, int MODE, int LY>
 __global__ void __launch_bounds__(WG_MAX) k_line_yz(LineArgs a, const TS* __restrict__ in,
                                                     TS* __restrict__ out) {
   constexpr int R = Halo<KIND>::R;
   using CS = typename SolveT<MODE>::type;
   extern __shared__ __align__(16) unsigned char smem[];
-  CS* Y = reinterpret_cast<CS*>(smem);
   const int lane = threadIdx.x;
   int p = threadIdx.y;
   if (LY == 64) p = __builtin_amdgcn_readfirstlane(p);
+  const int tid = threadIdx.x + LY * threadIdx.y;
+  const int nthr = LY * blockDim.y;
+  CS* Y = reinterpret_cast<CS*>(smem);
+  Tabs t;
+  if (IMPL) t = stage_tables<MODE>(a, smem + align16((size_t)4 * a.P * LY * sizeof(CS)), tid, nthr);
   const long long L = (long long)blockIdx.x * LY + lane;
   const bool valid = L < a.nlines;
   const long long Lc = valid ? L : (long long)a.nlines - 1;
@@ -144,15 +183,16 @@ __global__ void __launch_bounds__(WG_MAX) k_line_yz(LineArgs a, const TS* __rest
   double rhs[CHUNK];
   rhs_interior<KIND>(a, win, rhs, m);
   rhs_boundary(a, s, m, rhs, [&](int i) { return vin[a.stride * i]; });
-  double y[CHUNK];
-  solve_chunk<IMPL, MODE, CS>(a, p, s, m, Y, LY, lane, rhs, y);
+  if (IMPL) __syncthreads();  // staged tables visible
+  solve_chunk<IMPL, MODE, CS>(a, t, p, s, m, Y, LY, lane, rhs);
   if (valid) {
     TS* vout = out + base;
 #pragma unroll
     for (int k = 0; k < CHUNK; ++k) {
       if (k < m) {
-        double val = y[k] * a.scale;
-        if (a.addv) val += (double)win[k + R];
+        double val = rhs[k] * a.scale;
+        // filters add v back; it is re-read (cached) rather than kept live
+        if (a.addv) val += (double)vin[a.stride * (s + k)];
         TS* dst = vout + a.stride * (s + k);
         const TS old = (a.epi == EPI_ADD || a.epi == EPI_MAXABS) ? *dst : (TS)0;
         *dst = epilogue<TS>(a.epi, a.epi_c, val, old);
@@ -162,6 +202,9 @@ __global__ void __launch_bounds__(WG_MAX) k_line_yz(LineArgs a, const TS* __rest
 }
 
 // x lines: LX contiguous lines per workgroup staged in LDS (row pitch N+1).
+// The tile region is reused: once every thread holds its window and RHS in
+// registers, the same LDS holds the coefficient tables and the edge-value
+// exchange, and after the solve it holds the results for coalesced stores.
 template <typename TS, int KIND, bool IMPL, int MODE>
 __global__ void __launch_bounds__(WG_MAX) k_line_x(LineArgs a, const TS* __restrict__ in,
                                                    TS* __restrict__ out) {
@@ -170,15 +213,15 @@ __global__ void __launch_bounds__(WG_MAX) k_line_x(LineArgs a, const TS* __restr
   extern __shared__ __align__(16) unsigned char smem[];
   const int N = a.N, pitch = a.N + 1, LX = a.lx;
   TS* tile = reinterpret_cast<TS*>(smem);
-  const size_t tile_bytes = ((size_t)LX * pitch * sizeof(TS) + 15) & ~(size_t)15;
-  CS* Y = reinterpret_cast<CS*>(smem + tile_bytes);
   const int tid = threadIdx.x;
+  const int nthr = blockDim.x;
   const int l = tid % LX;
   const int p = tid / LX;
   const long long L0 = (long long)blockIdx.x * LX;
-  const int nval = (int)min((long long)LX, (long long)a.nlines - L0);
+  const long long rem = (long long)a.nlines - L0;
+  const int nval = (int)(rem < LX ? rem : LX);
   const TS* src = in + L0 * N;
-  for (int e = tid; e < nval * N; e += blockDim.x) {
+  for (int e = tid; e < nval * N; e += nthr) {
     const int ll = e / N, i = e - ll * N;
     tile[ll * pitch + i] = src[e];
   }
@@ -205,28 +248,37 @@ __global__ void __launch_bounds__(WG_MAX) k_line_x(LineArgs a, const TS* __restr
   double rhs[CHUNK];
   rhs_interior<KIND>(a, win, rhs, m);
   rhs_boundary(a, s, m, rhs, [&](int i) { return row0[i]; });
-  double y[CHUNK];
-  solve_chunk<IMPL, MODE, CS>(a, p, s, m, Y, LX, l, rhs, y);
-  __syncthreads();  // every window has been read from the tile
+  __syncthreads();  // the tile is consumed; reuse its LDS
+  CS* Y = reinterpret_cast<CS*>(smem);
+  Tabs t;
+  if (IMPL) {
+    t = stage_tables<MODE>(a, smem + align16((size_t)4 * a.P * LX * sizeof(CS)), tid, nthr);
+    __syncthreads();
+  }
+  solve_chunk<IMPL, MODE, CS>(a, t, p, s, m, Y, LX, l, rhs);
+  __syncthreads();  // tables and edge values no longer read
+  // results go back through the tile in the compute type so that the store
+  // pass can add v (filters) and apply the epilogue without rounding twice
+  double* dtile = reinterpret_cast<double*>(smem);
   if (l < nval) {
-    TS* rowo = tile + l * pitch;
+    double* rowo = dtile + l * pitch;
 #pragma unroll
-    for (int k = 0; k < CHUNK; ++k) {
-      if (k < m) {
-        double val = y[k] * a.scale;
-        if (a.addv) val += (double)win[k + R];
-        rowo[s + k] = (TS)val;
-      }
-    }
+    for (int k = 0; k < CHUNK; ++k)
+      if (k < m) rowo[s + k] = rhs[k] * a.scale;
   }
   __syncthreads();
   TS* dst = out + L0 * N;
   const bool need_old = (a.epi == EPI_ADD || a.epi == EPI_MAXABS);
-  for (int e = tid; e < nval * N; e += blockDim.x) {
+  for (int e = tid; e < nval * N; e += nthr) {
     const int ll = e / N, i = e - ll * N;
-    const double val = (double)tile[ll * pitch + i];
-    const TS old = need_old ? dst[e] : (TS)0;
-    dst[e] = (a.epi == EPI_STORE) ? tile[ll * pitch + i] : epilogue<TS>(a.epi, a.epi_c, val, old);
+    double val = dtile[ll * pitch + i];
+    if (a.addv) val += (double)src[e];
+    if (a.epi == EPI_STORE) {
+      dst[e] = (TS)val;
+    } else {
+      const TS old = need_old ? dst[e] : (TS)0;
+      dst[e] = epilogue<TS>(a.epi, a.epi_c, val, old);
+    }
   }
 }
 
@@ -241,6 +293,27 @@ __global__ void k_null(long long n, int copy, int epi, double c, const TS* __res
   out[i] = epilogue<TS>(epi, c, val, old);
 }
 
+// RK4 stage update (pyranda.py:800-804), one pass over F, PHI and U
+template <typename TS>
+__global__ void k_rk_update(long long n, const TS* __restrict__ F, double* __restrict__ PHI,
+                            const TS* __restrict__ Uin, TS* __restrict__ Uout,
+                            double* __restrict__ U64, double dt, double A, double B) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const double tmp1 = A * PHI[i];
+    const double phi = dt * (double)F[i] + tmp1;
+    PHI[i] = phi;
+    const double tmp2 = B * phi;
+    if (U64) {
+      const double u = U64[i] + tmp2;
+      U64[i] = u;
+      Uout[i] = (TS)u;
+    } else {
+      Uout[i] = (TS)((double)Uin[i] + tmp2);
+    }
+  }
+}
+
 // ---------------------------------------------------------------------------
 // context
 // ---------------------------------------------------------------------------
@@ -251,6 +324,7 @@ struct DevPlan {
   const float *lu32 = nullptr, *rc32 = nullptr, *gsel32 = nullptr;
   const short* bidx = nullptr;
   const int* bnt = nullptr;
+  int ntab = 0;
 };
 
 struct pyr_ctx {
@@ -275,16 +349,19 @@ static int upload_plan(const OpPlan& pl, DevPlan& d) {
   auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
   size_t off = 0;
   size_t o_cs = off; off = align(off + (P + 1) * sizeof(int));
-  size_t o_lu = off; off = align(off + n * 5 * sizeof(double));
-  size_t o_rc = off; off = align(off + n * 4 * sizeof(double));
-  size_t o_gs = off; off = align(off + P * 16 * P * sizeof(double));
+  // lu|rc|gsel contiguous (one block copied to LDS by the kernels)
+  size_t o_lu = off;
+  size_t o_rc = o_lu + n * 5 * sizeof(double);
+  size_t o_gs = o_rc + n * 4 * sizeof(double);
+  off = align(o_gs + P * 16 * P * sizeof(double));
   size_t o_bd = off; off = align(off + n * 5 * sizeof(double));
   size_t o_bc = off; off = align(off + 2 * NB * MAXT * sizeof(double));
   size_t o_bi = off; off = align(off + 2 * NB * MAXT * 2 * sizeof(short));
   size_t o_bn = off; off = align(off + 2 * NB * sizeof(int));
-  size_t o_lu32 = off; off = align(off + n * 5 * sizeof(float));
-  size_t o_rc32 = off; off = align(off + n * 4 * sizeof(float));
-  size_t o_gs32 = off; off = align(off + P * 16 * P * sizeof(float));
+  size_t o_lu32 = off;
+  size_t o_rc32 = o_lu32 + n * 5 * sizeof(float);
+  size_t o_gs32 = o_rc32 + n * 4 * sizeof(float);
+  off = align(o_gs32 + P * 16 * P * sizeof(float));
   std::vector<unsigned char> host(off, 0);
   std::memcpy(&host[o_cs], pl.cstart.data(), (P + 1) * sizeof(int));
   std::memcpy(&host[o_lu], pl.lu.data(), n * 5 * sizeof(double));
@@ -314,6 +391,7 @@ static int upload_plan(const OpPlan& pl, DevPlan& d) {
   d.lu32 = reinterpret_cast<const float*>(b + o_lu32);
   d.rc32 = reinterpret_cast<const float*>(b + o_rc32);
   d.gsel32 = reinterpret_cast<const float*>(b + o_gs32);
+  d.ntab = (int)(n * 9 + P * 16 * P);
   return 0;
 }
 
@@ -361,6 +439,7 @@ static LineArgs make_args(const pyr_ctx* c, const OpPlan& pl, const DevPlan& d, 
   for (int i = 0; i < 9; ++i) a.w[i] = pl.w[i];
   a.cstart = d.cstart; a.lu = d.lu; a.rc = d.rc; a.gsel = d.gsel; a.band = d.band;
   a.lu32 = d.lu32; a.rc32 = d.rc32; a.gsel32 = d.gsel32;
+  a.tab64 = d.lu; a.tab32 = d.lu32; a.ntab = d.ntab;
   a.bcoef = d.bcoef; a.bidx = d.bidx; a.bnt = d.bnt;
   return a;
 }
@@ -368,28 +447,44 @@ static LineArgs make_args(const pyr_ctx* c, const OpPlan& pl, const DevPlan& d, 
 // ---------------------------------------------------------------------------
 // launch
 // ---------------------------------------------------------------------------
+template <typename K>
+static int set_lds(K kernel, size_t bytes) {
+  if (bytes > 64 * 1024)
+    HIPCHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return 0;
+}
+
 template <typename TS, int KIND, bool IMPL, int MODE>
 static int launch_typed(pyr_ctx* c, int dir, LineArgs a, const TS* in, TS* out) {
   using CS = typename SolveT<MODE>::type;
   const int P = a.P;
+  const size_t stage = IMPL ? stage_bytes<MODE>(a.ntab, a.N) : 0;
   if (dir == 0) {
     int lx = WG_MAX / P;
     if (lx > 64) lx = 64;
-    // keep the staged tile within 48 KiB so several workgroups share a CU
-    while (lx > 4 && (size_t)lx * (a.N + 1) * sizeof(TS) > 48 * 1024) lx /= 2;
+    // keep the staged tile within 40 KiB so several workgroups share a CU
+    while (lx > 4 && (size_t)lx * (a.N + 1) * sizeof(double) > 40 * 1024) lx /= 2;
     a.lx = lx;
-    const size_t tile = ((size_t)lx * (a.N + 1) * sizeof(TS) + 15) & ~(size_t)15;
-    const size_t shm = tile + (size_t)4 * P * lx * sizeof(CS);
+    // the tile region holds the input (TS) and later the results (double)
+    const size_t tile = align16((size_t)lx * (a.N + 1) * sizeof(double));
+    const size_t work = align16((size_t)4 * P * lx * sizeof(CS)) + stage;
+    const size_t shm = tile > work ? tile : work;
     const dim3 grid((a.nlines + lx - 1) / lx), block(lx * P);
-    hipLaunchKernelGGL((k_line_x<TS, KIND, IMPL, MODE>), grid, block, shm, c->stream, a, in, out);
-  } else if (P <= WG_MAX / 64) {
-    const size_t shm = (size_t)4 * P * 64 * sizeof(CS);
+    auto k = k_line_x<TS, KIND, IMPL, MODE>;
+    if (set_lds(k, shm)) return 1;
+    hipLaunchKernelGGL(k, grid, block, shm, c->stream, a, in, out);
+  } else if (P <= 8) {
+    const size_t shm = align16((size_t)4 * P * 64 * sizeof(CS)) + stage;
     const dim3 grid((a.nlines + 63) / 64), block(64, P);
-    hipLaunchKernelGGL((k_line_yz<TS, KIND, IMPL, MODE, 64>), grid, block, shm, c->stream, a, in, out);
+    auto k = k_line_yz<TS, KIND, IMPL, MODE, 64>;
+    if (set_lds(k, shm)) return 1;
+    hipLaunchKernelGGL(k, grid, block, shm, c->stream, a, in, out);
   } else {
-    const size_t shm = (size_t)4 * P * 32 * sizeof(CS);
+    const size_t shm = align16((size_t)4 * P * 32 * sizeof(CS)) + stage;
     const dim3 grid((a.nlines + 31) / 32), block(32, P);
-    hipLaunchKernelGGL((k_line_yz<TS, KIND, IMPL, MODE, 32>), grid, block, shm, c->stream, a, in, out);
+    auto k = k_line_yz<TS, KIND, IMPL, MODE, 32>;
+    if (set_lds(k, shm)) return 1;
+    hipLaunchKernelGGL(k, grid, block, shm, c->stream, a, in, out);
   }
   HIPCHECK(hipGetLastError());
   return 0;
@@ -751,6 +846,25 @@ int pyr_divergence_f64(pyr_ctx* c, const double* fx, const double* fy, const dou
 }
 int pyr_divergence_r32(pyr_ctx* c, const float* fx, const float* fy, const float* fz, float* o) {
   return sum3(c, OP_D1, PYR_R32, fx, fy, fz, o);
+}
+
+int pyr_rk_update(pyr_ctx* c, int prec, long long n, const void* F, double* PHI, const void* Uin,
+                  void* Uout, double* U64, double dt, double A, double B) {
+  if (check_ctx(c)) return 1;
+  if (!F || !PHI || !Uout || (!Uin && !U64)) return fail("null pointer to pyr_rk_update");
+  if (n <= 0) return 0;
+  long long blocks = (n + 1023) / 1024;
+  if (blocks > 8192) blocks = 8192;
+  if (prec == PYR_F64)
+    hipLaunchKernelGGL((k_rk_update<double>), dim3((unsigned)blocks), dim3(256), 0, c->stream, n,
+                       (const double*)F, PHI, (const double*)Uin, (double*)Uout, U64, dt, A, B);
+  else if (prec == PYR_R32)
+    hipLaunchKernelGGL((k_rk_update<float>), dim3((unsigned)blocks), dim3(256), 0, c->stream, n,
+                       (const float*)F, PHI, (const float*)Uin, (float*)Uout, U64, dt, A, B);
+  else
+    return fail("bad precision");
+  HIPCHECK(hipGetLastError());
+  return 0;
 }
 
 int pyr_host_weights(int op, double* ali, double* ari, double* alb1, double* alb2, double* arb1,

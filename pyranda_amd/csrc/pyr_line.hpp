@@ -43,6 +43,11 @@ struct LineArgs {
   const float* lu32;
   const float* rc32;
   const float* gsel32;
+  // the same tables as contiguous blocks, staged into LDS by the kernels:
+  // tab64 = lu|rc|gsel (doubles), tab32 = lu32|rc32|gsel32 (floats)
+  const double* tab64;
+  const float* tab32;
+  int ntab;                // entries of one block: 9N + 16P^2
   const double* bcoef;
   const short* bidx;
   const int* bnt;

@@ -76,6 +76,8 @@ class ParcopContext:
     def __init__(self, nn, x1, xn, bcs, device="cuda"):
         self.nn = tuple(int(n) for n in nn)
         self.device = torch.device(device)
+        if self.device.type == "cuda" and self.device.index is None:
+            self.device = torch.device("cuda", torch.cuda.current_device())
         L = _lib.lib()
         h = ctypes.c_void_p()
         bc6 = (ctypes.c_int * 6)(*[_lib.BC[b] for b in bcs])

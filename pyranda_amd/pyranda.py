@@ -266,11 +266,18 @@ class pyrandaSim:
                     self.variables[eq.LHS[ii]].data = rhs[ii]
 
     def rk4(self, time, dt):
-        shape = tuple(self.mesh.shape)
+        """5-stage 2N-storage RK4 (pyranda.py:758-810).
+
+        Per stage and conserved variable: PHI = dt*F + A*PHI; U = U + B*PHI,
+        as one fused kernel (pyr_rk_update) when F is a field. In fp32 mode
+        PHI and an FP64 shadow of U carry the accumulation.
+        """
         fp32 = self.dtype != torch.float64
+        mpi = self.PyMPI
         PHI = {}
         for U in self.conserved:
-            PHI[U] = self.PyMPI.emptyScalar().to(torch.float64) if fp32 else self.PyMPI.emptyScalar()
+            PHI[U] = torch.zeros((self.nz, self.ny, self.nx), dtype=torch.float64,
+                                 device=mpi.device).permute(2, 1, 0)
             if fp32:
                 self._shadow[U] = self.variables[U].data.to(torch.float64)
         time_i = time
@@ -278,14 +285,7 @@ class pyrandaSim:
         for ii in range(5):
             FLUX = self.updateFlux()
             for U in self.conserved:
-                tmp1 = RK_A[ii] * PHI[U]
-                PHI[U] = dt * FLUX[U] + tmp1
-                tmp2 = RK_B[ii] * PHI[U]
-                if fp32:
-                    self._shadow[U] = self._shadow[U] + tmp2
-                    self.variables[U].data = self._shadow[U].to(self.dtype)
-                else:
-                    self.variables[U].data = self.variables[U].data + tmp2
+                self.variables[U].data = self._rk_update(U, FLUX[U], PHI, dt, RK_A[ii], RK_B[ii], fp32)
             time = time_i + RK_ETA[ii] * dt
             self.time = time
             if fp32:
@@ -295,9 +295,32 @@ class pyrandaSim:
                 for U in self.conserved:
                     if self.variables[U].data is not before[U]:
                         self._shadow[U] = self.variables[U].data.to(torch.float64)
-        del shape
         self.cycle += 1
         return time
+
+    def _rk_update(self, U, F, PHI, dt, A, B, fp32):
+        from . import _lib
+        from .parcop import _contig_equiv
+        u = self.variables[U].data
+        nn = tuple(self.mesh.shape)
+        ok = (torch.is_tensor(F) and tuple(F.shape) == nn and _contig_equiv(F)
+              and F.dtype == self.dtype and torch.is_tensor(u) and tuple(u.shape) == nn
+              and _contig_equiv(u) and u.dtype == self.dtype)
+        if not ok:
+            # scalar fluxes (e.g. ODE right-hand sides) and odd layouts: torch ops
+            PHI[U] = dt * F + A * PHI[U]
+            if fp32:
+                self._shadow[U] = self._shadow[U] + B * PHI[U]
+                return self._shadow[U].to(self.dtype)
+            return u + B * PHI[U]
+        out = torch.empty((self.nz, self.ny, self.nx), dtype=self.dtype, device=u.device).permute(2, 1, 0)
+        ctx = self.PyMPI.ctx
+        ctx._stream()
+        shadow = self._shadow[U].data_ptr() if fp32 else None
+        _lib.check(ctx._L.pyr_rk_update(ctx._h, _lib.R32 if fp32 else _lib.F64, self.npts,
+                                        F.data_ptr(), PHI[U].data_ptr(), u.data_ptr(), out.data_ptr(),
+                                        shadow, float(dt), float(A), float(B)))
+        return out
 
     def step(self, nsteps=1):
         for _ in range(nsteps):

@@ -4,8 +4,15 @@
 `lambda self:` over device fields. The lambda's globals hold the device
 `numpy`/`scipy` namespaces, so calls such as numpy.minimum act on GPU tensors.
 """
+import os
+
 from .pyrandaUtils import fortran3d, findVar
 from .devnumpy import numpy, scipy  # noqa: F401  (names the lambdas resolve)
+from .fuse import FusedRHS
+
+# PYRANDA_AMD_FUSE=0 evaluates right-hand sides with one torch kernel per
+# numpy-level operation instead of generated fused kernels
+FUSE = os.environ.get("PYRANDA_AMD_FUSE", "1") != "0"
 
 
 class pyrandaEq:
@@ -27,6 +34,6 @@ class pyrandaEq:
         else:
             Srhs = fortran3d(self.eqstr, sMap)
         self.sRHS = Srhs
-        self.RHS = eval("lambda self: " + Srhs)
+        self.RHS = FusedRHS(Srhs, globals()) if FUSE else eval("lambda self: " + Srhs)
         if "ddt(" in eqstr:
             self.kind = "PDE"
