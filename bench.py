@@ -44,10 +44,11 @@ def tg_mesh(n):
     """.replace("Npts", str(n)).replace("pi", str(math.pi)).replace("FF", str(float(n - 1) / n))
 
 
-def make_tg(n, precision, p0):
+def make_tg(n, precision, p0, solver="circulant"):
     import pyranda_amd as pr
     from pyranda_amd.workloads import TG_EOM, TG_IC
     ss = pr.pyrandaSim("TGvortex", tg_mesh(n), silent=True, precision=precision)
+    ss.PyMPI.ctx.set_periodic_solver(solver)
     ss.addPackage(pr.pyrandaTimestep(ss))
     ss.EOM(TG_EOM)
     ss.setIC(TG_IC.replace("p0 = 100.0", "p0 = %r" % p0))
@@ -94,7 +95,7 @@ def _symbols(theta):
     return [d1, d2, d8, sf, gf]
 
 
-def ops_bench(n, steps, warmup, dev):
+def ops_bench(n, steps, warmup, dev, only=None, solver="circulant"):
     """Operator microbenchmark: every directional operator on the seeded
     64-mode Fourier field (|k|<=32, |k|^-5/3, seed 1234), FP64 and FP32-refined,
     timed with HIP events, error against the exact per-mode result."""
@@ -104,12 +105,39 @@ def ops_bench(n, steps, warmup, dev):
     from pyranda_amd.workloads import fourier_modes, fourier_field
     L = 2 * math.pi * (n - 1) / n
     ctx = ParcopContext((n, n, n), (0.0, 0.0, 0.0), (L, L, L), ["PERI"] * 6, device=dev)
+    ctx.set_periodic_solver(solver)
     h = ctx.dpatch[0]
     ks, amps, phis = fourier_modes()
     f64 = fourier_field(n, dev, (ks, amps, phis))
     f32 = f64.float()
     out = {}
     reps = max(steps, 1) * 4
+    only = [t for t in (only or "").split(",") if t]
+
+    def wanted(name):
+        return not only or any(name.startswith(t) for t in only)
+
+    # plain device copy of the same field: the practical bandwidth ceiling, and
+    # the kernel whose FETCH_SIZE/WRITE_SIZE counters scale the others
+    for name, f, bpp in (("copy_f64", f64, 16), ("copy_f32", f32, 8)):
+        if not wanted(name):
+            continue
+        o = torch.empty_like(f)
+        for _ in range(warmup + 1):
+            o.copy_(f)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            o.copy_(f)
+        e1.record()
+        torch.cuda.synchronize()
+        us = e0.elapsed_time(e1) * 1e3 / reps
+        gbs = n ** 3 * bpp / (us * 1e-6) / 1e9
+        out[name] = {"us": round(us, 2), "GB_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 3)}
+        print("[ops] %-10s %-4s %8.1f us %7.1f GB/s (%.1f%%)" % (name, "", us, gbs, 100 * gbs / HBM_PEAK_GBS),
+              file=sys.stderr, flush=True)
     for op in range(5):
         for d in range(3):
             th = np.abs(ks[:, d]) * h
@@ -121,6 +149,8 @@ def ops_bench(n, steps, warmup, dev):
             else:
                 exact = fourier_field(n, dev, (ks, amps, phis), weights=sym)
             for name, f, bpp in (("f64", f64, 16), ("r32", f32, 8)):
+                if not wanted("%s_%s_%s" % (OPS[op], DIRS[d], name)):
+                    continue
                 o = ctx.apply(op, d, f)
                 for _ in range(warmup):
                     ctx.apply(op, d, f, out=o)
@@ -135,16 +165,23 @@ def ops_bench(n, steps, warmup, dev):
                 us = e0.elapsed_time(e1) * 1e3 / reps
                 err = ((o.double() - exact).abs().max() / exact.abs().max()).item()
                 gbs = n ** 3 * bpp / (us * 1e-6) / 1e9
-                out["%s_%s_%s" % (OPS[op], DIRS[d], name)] = {
-                    "us": round(us, 2), "GB_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 3),
-                    "rel_max_err": err}
-                print("[ops] %-10s %-4s %8.1f us %7.1f GB/s (%.1f%%) err %.2e" %
-                      (OPS[op] + "_" + DIRS[d], name, us, gbs, 100 * gbs / HBM_PEAK_GBS, err),
+                rec = {"us": round(us, 2), "GB_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 3),
+                       "rel_max_err": err}
+                if name == "r32":
+                    # the refined solve against the FP64 path on the same (rounded) input
+                    ref = ctx.apply(op, d, f.double())
+                    rec["rel_max_err_same_input"] = ((o.double() - ref).abs().max() / ref.abs().max()).item()
+                    del ref
+                out["%s_%s_%s" % (OPS[op], DIRS[d], name)] = rec
+                print("[ops] %-10s %-4s %8.1f us %7.1f GB/s (%.1f%%) err %.2e%s" %
+                      (OPS[op] + "_" + DIRS[d], name, us, gbs, 100 * gbs / HBM_PEAK_GBS, err,
+                       " same-input %.2e" % rec["rel_max_err_same_input"] if name == "r32" else ""),
                       file=sys.stderr, flush=True)
-    best = max(out.items(), key=lambda kv: kv[1]["GB_s"])
+    best = max((kv for kv in out.items() if not kv[0].startswith("copy")),
+               key=lambda kv: kv[1]["GB_s"], default=(None, None))
     print(json.dumps({"metric": "compact operator throughput, 256^3 periodic Fourier field",
                       "unit": "GB/s (algorithmic: one read + one write per point)",
-                      "n": n, "reps": reps, "ops": out,
+                      "n": n, "reps": reps, "solver": solver, "ops": out,
                       "best": best[0]}), flush=True)
 
 
@@ -160,6 +197,10 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--ops", action="store_true",
                     help="operator microbenchmark on the 256^3 seeded Fourier field instead")
+    ap.add_argument("--solver", default="circulant", choices=["circulant", "lu"],
+                    help="solver of periodic implicit lines")
+    ap.add_argument("--only", default=None,
+                    help="--ops: comma-separated name prefixes to run (e.g. d1_x_f64,copy_f64)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -174,7 +215,7 @@ def main():
     dev = torch.device("cuda:0")
     n = args.n
     if args.ops:
-        return ops_bench(n, args.steps, args.warmup, dev)
+        return ops_bench(n, args.steps, args.warmup, dev, args.only, args.solver)
     gamma = 1.4
     p0 = 1.0 / (gamma * 0.1 ** 2)  # Mach 0.1 on the unit velocity
 
@@ -183,7 +224,7 @@ def main():
     states = {}
     dt = None
     for precision in (["fp32"] if args.no_fp64 else ["fp64", "fp32"]):
-        ss = make_tg(n, precision, p0)
+        ss = make_tg(n, precision, p0, args.solver)
         if dt is None:
             dt = float(ss.variables["dt"].data) * 0.5  # CFL 0.5 from the initial state
         ctx = ss.PyMPI.ctx
@@ -246,7 +287,8 @@ def main():
         "dtype": "f32 fields and sweeps, f64 residual refinement, f64 RK4 state",
         "data": "synthetic (Taylor-Green initial condition)",
         "config": {"workload": "taylor_green_euler_lad_%d^3" % n, "grid": [n, n, n], "gamma": gamma,
-                   "mach": 0.1, "cfl": 0.5, "dt": dt, "rk_stages": 5, "lad": True},
+                   "mach": 0.1, "cfl": 0.5, "dt": dt, "rk_stages": 5, "lad": True,
+                   "periodic_solver": args.solver},
         "fp64": ({"value": results["fp64"]["pups"], "ms_per_step": results["fp64"]["ms_per_step"]}
                  if "fp64" in results else None),
         "speedup_fp32r_over_fp64": (head["pups"] / results["fp64"]["pups"]) if "fp64" in results else None,

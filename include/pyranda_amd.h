@@ -57,6 +57,15 @@ int pyr_setup_host(pyr_ctx** ctx, int nx, int ny, int nz, double x1, double xn, 
 int pyr_destroy(pyr_ctx* ctx);
 /* HIP stream (hipStream_t as void*), 0 = null stream */
 int pyr_set_stream(pyr_ctx* ctx, void* stream);
+
+/* Solver of periodic implicit lines. No counterpart in the reference, which
+ * always uses the bordered cyclic LU (pentadiagonal.f90:64-131, 654-825):
+ * PYR_PSOLVE_CIRCULANT (default) factors the circulant LHS as c Q^T Q and
+ * runs constant-coefficient recurrences; PYR_PSOLVE_LU uses the chunked LU
+ * of bounded lines with the cyclic coupling in its reduced system. Both give
+ * the same solution to rounding. */
+enum { PYR_PSOLVE_CIRCULANT = 0, PYR_PSOLVE_LU = 1 };
+int pyr_set_periodic_solver(pyr_ctx* ctx, int mode);
 const char* pyr_last_error(void);
 
 /* --- mesh (parcop.f90 xgrid/ygrid/zgrid/dxgrid/mesh_getgridlen; mesh.f90:173-212)

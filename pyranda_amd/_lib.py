@@ -13,6 +13,7 @@ BC = {"PERI": 0, "NONE": 1, "SYMM": 2}
 OP_D1, OP_D2, OP_D8, OP_SFILTER, OP_GFILTER = 0, 1, 2, 3, 4
 EPI_STORE, EPI_ADD, EPI_ABS_SCALED, EPI_MAXABS_SCALED = 0, 1, 2, 3
 F64, R32 = 0, 1
+PSOLVE = {"circulant": 0, "lu": 1}
 
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -24,6 +25,7 @@ SIGNATURES = {
     "pyr_setup_host": (ctypes.c_int, [ctypes.POINTER(_vp)] + [ctypes.c_int] * 3 + [ctypes.c_double] * 6 + [_ip]),
     "pyr_destroy": (ctypes.c_int, [_vp]),
     "pyr_set_stream": (ctypes.c_int, [_vp, _vp]),
+    "pyr_set_periodic_solver": (ctypes.c_int, [_vp, ctypes.c_int]),
     "pyr_last_error": (ctypes.c_char_p, []),
     "pyr_axis_coords": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "pyr_mesh_scalars": (ctypes.c_int, [_vp, _dp, _dp, _dp]),

@@ -11,7 +11,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstdint>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -89,13 +91,14 @@ __device__ __forceinline__ Tabs stage_tables(const LineArgs& a, unsigned char* d
   return t;
 }
 
-// Solve one chunk given its RHS; the LDS exchange of edge values goes through
-// Y (4P x ly entries of the solve type), column `col` belongs to this line.
-// The solution replaces the RHS in `rhs`.
-template <bool IMPL, int MODE, typename CS>
-__device__ __forceinline__ void solve_chunk(const LineArgs& a, const Tabs& t, int p, int s, int m,
-                                            CS* Y, int ly, int col, double* rhs) {
-  if (!IMPL) return;
+enum Solve { SOLVE_NONE = 0, SOLVE_LU = 1, SOLVE_CIRC = 2 };
+
+// Solve one chunk given its RHS with the chunked LU; the LDS exchange of edge
+// values goes through Y (4P x ly entries of the solve type), column `col`
+// belongs to this line. The solution replaces the RHS in `rhs`.
+template <int MODE, typename CS>
+__device__ __forceinline__ void solve_chunk_lu(const LineArgs& a, const Tabs& t, int p, int s, int m,
+                                               CS* Y, int ly, int col, double* rhs) {
   auto yr = [&](int c) { return Y[c * ly + col]; };
   if (MODE == MODE_F64) {
     double* x = rhs;  // solve in place
@@ -122,7 +125,7 @@ __device__ __forceinline__ void solve_chunk(const LineArgs& a, const Tabs& t, in
     neighbours<float>(t.gsel32, p, a.P, yr, z);
     spike_correct(t.rc32, s, m, z, x);
     // one refinement step: FP64 residual against the same banded operator
-    residual64(t.band, s, m, rhs, x, z, rhs);
+    residual64([&](int k) { return t.band + 5 * (s + k); }, m, rhs, x, z, rhs);
     float d[CHUNK];
 #pragma unroll
     for (int k = 0; k < CHUNK; ++k) d[k] = (float)rhs[k];
@@ -141,12 +144,71 @@ __device__ __forceinline__ void solve_chunk(const LineArgs& a, const Tabs& t, in
   }
 }
 
+// One circulant solve A y = r of the chunk (without the 1/c): forward
+// recurrence with its carry through Xa, backward recurrence with its carry
+// through Xb; the rows of the result go to sink(k, y_k). Two barriers.
+template <typename C, typename Sink>
+__device__ __forceinline__ void circ_solve(const C* kf, const C* kb, C p1, C p2, int p, int P,
+                                           int m, C* Xa, C* Xb, int ly, int col, C* x, Sink sink) {
+  C e0, e1;
+  circ_fwd_exit(x, m, p1, p2, e0, e1);
+  Xa[(2 * p) * ly + col] = e0;
+  Xa[(2 * p + 1) * ly + col] = e1;
+  __syncthreads();
+  circ_carry(kf, p, P, [&](int c) { return Xa[c * ly + col]; }, e0, e1);
+  circ_fwd(x, m, p1, p2, e0, e1);
+  circ_bwd_exit(x, m, p1, p2, e0, e1);
+  Xb[(2 * p) * ly + col] = e0;
+  Xb[(2 * p + 1) * ly + col] = e1;
+  __syncthreads();
+  circ_carry(kb, p, P, [&](int c) { return Xb[c * ly + col]; }, e0, e1);
+  circ_bwd_sink(x, m, p1, p2, e0, e1, sink);
+}
+
+// Circulant solve of one chunk; X0 and X1 hold 4P x ly entries each. Every
+// buffer is written only after a barrier that follows its last read. The
+// solution rows go to emit(k, y_k) in FP64 (before the metric scale).
+template <int MODE, typename CS, typename Emit>
+__device__ __forceinline__ void solve_chunk_circ(const LineArgs& a, int p, int m, CS* X0, CS* X1,
+                                                 int ly, int col, double* rhs, Emit emit) {
+  const int P = a.P;
+  if constexpr (MODE == MODE_F64) {
+    circ_solve<double>(a.kf, a.kb, a.p1, a.p2, p, P, m, X0, X1, ly, col, rhs,
+                       [&](int k, double v) { emit(k, v * a.invc); });
+  } else {
+    float y[CHUNK];
+#pragma unroll
+    for (int k = 0; k < CHUNK; ++k) y[k] = (float)rhs[k];
+    circ_solve<float>(a.kf32, a.kb32, a.p1f, a.p2f, p, P, m, X0, X1, ly, col, y,
+                      [&](int k, float v) { y[k] = v * a.invcf; });
+    // neighbour values of the FP32 solution for the residual
+    float e4[4];
+    edges(y, m, e4);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) X0[(4 * p + q) * ly + col] = e4[q];
+    __syncthreads();
+    const int pm = p == 0 ? P - 1 : p - 1, pp = p == P - 1 ? 0 : p + 1;
+    const float z[4] = {X0[(4 * pm + 2) * ly + col], X0[(4 * pm + 3) * ly + col],
+                        X0[(4 * pp) * ly + col], X0[(4 * pp + 1) * ly + col]};
+    const double row[5] = {a.bd[0], a.bd[1], a.bd[2], a.bd[1], a.bd[0]};
+    residual64([&](int) { return row; }, m, rhs, y, z, rhs);
+    float d[CHUNK];
+#pragma unroll
+    for (int k = 0; k < CHUNK; ++k) d[k] = (float)rhs[k];
+    circ_solve<float>(a.kf32, a.kb32, a.p1f, a.p2f, p, P, m, X1, X0, ly, col, d,
+                      [&](int k, float v) { emit(k, (double)y[k] + (double)(v * a.invcf)); });
+  }
+}
+
 // y/z lines. block = (LY lanes, P chunks). With LY = 64 the chunk index is
 // wave-uniform; with LY = 32 a wave holds two chunks of 32 lines each.
-template <typename TS, int KIND, bool IMPL, int MODE, int LY>
+// FULL: every chunk has CHUNK rows (N a multiple of CHUNK), so the row
+// predicates vanish at compile time. The circulant solve implies periodic lines.
+template <typename TS, int KIND, int SOLVE, int MODE, int LY, bool FULL>
 __global__ void __launch_bounds__(WG_MAX) k_line_yz(LineArgs a, const TS* __restrict__ in,
                                                     TS* __restrict__ out) {
   constexpr int R = Halo<KIND>::R;
+  constexpr bool PER = SOLVE == SOLVE_CIRC;
   using CS = typename SolveT<MODE>::type;
   extern __shared__ __align__(16) unsigned char smem[];
   const int lane = threadIdx.x;
@@ -156,13 +218,13 @@ __global__ void __launch_bounds__(WG_MAX) k_line_yz(LineArgs a, const TS* __rest
   const int nthr = LY * blockDim.y;
   CS* Y = reinterpret_cast<CS*>(smem);
   Tabs t;
-  if (IMPL) t = stage_tables<MODE>(a, smem + align16((size_t)4 * a.P * LY * sizeof(CS)), tid, nthr);
+  if (SOLVE == SOLVE_LU) t = stage_tables<MODE>(a, smem + align16((size_t)4 * a.P * LY * sizeof(CS)), tid, nthr);
   const long long L = (long long)blockIdx.x * LY + lane;
   const bool valid = L < a.nlines;
   const long long Lc = valid ? L : (long long)a.nlines - 1;
   const long long base = (Lc % a.A) + a.B * (Lc / a.A);
-  const int s = a.cstart[p];
-  const int m = a.cstart[p + 1] - s;
+  const int s = FULL ? p * CHUNK : a.cstart[p];
+  const int m = FULL ? CHUNK : a.cstart[p + 1] - s;
   const TS* vin = in + base;
 
   TS win[CHUNK + 2 * R];
@@ -171,7 +233,7 @@ __global__ void __launch_bounds__(WG_MAX) k_line_yz(LineArgs a, const TS* __rest
     int row = s - R + w;
     TS v = (TS)0;
     if (w < m + 2 * R) {
-      if (a.periodic) {
+      if (PER || a.periodic) {
         row = row < 0 ? row + a.N : (row >= a.N ? row - a.N : row);
         v = vin[a.stride * row];
       } else if (row >= 0 && row < a.N) {
@@ -182,36 +244,70 @@ __global__ void __launch_bounds__(WG_MAX) k_line_yz(LineArgs a, const TS* __rest
   }
   double rhs[CHUNK];
   rhs_interior<KIND>(a, win, rhs, m);
-  rhs_boundary(a, s, m, rhs, [&](int i) { return vin[a.stride * i]; });
-  if (IMPL) __syncthreads();  // staged tables visible
-  solve_chunk<IMPL, MODE, CS>(a, t, p, s, m, Y, LY, lane, rhs);
-  if (valid) {
-    TS* vout = out + base;
-#pragma unroll
-    for (int k = 0; k < CHUNK; ++k) {
-      if (k < m) {
-        double val = rhs[k] * a.scale;
-        // filters add v back; it is re-read (cached) rather than kept live
-        if (a.addv) val += (double)vin[a.stride * (s + k)];
-        TS* dst = vout + a.stride * (s + k);
-        const TS old = (a.epi == EPI_ADD || a.epi == EPI_MAXABS) ? *dst : (TS)0;
-        *dst = epilogue<TS>(a.epi, a.epi_c, val, old);
-      }
+  if (!PER) rhs_boundary(a, s, m, rhs, [&](int i) { return vin[a.stride * i]; });
+  TS* vout = out + base;
+  // row k of the result: metric scale, filters add v back (re-read, cached,
+  // rather than kept live), fused epilogue, store
+  auto emit = [&](int k, double y) {
+    if (!valid) return;
+    double val = y * a.scale;
+    if (a.addv) val += (double)vin[a.stride * (s + k)];
+    TS* dst = vout + a.stride * (s + k);
+    const TS old = (a.epi == EPI_ADD || a.epi == EPI_MAXABS) ? *dst : (TS)0;
+    *dst = epilogue<TS>(a.epi, a.epi_c, val, old);
+  };
+  if (SOLVE == SOLVE_CIRC) {
+    solve_chunk_circ<MODE, CS>(a, p, m, Y, Y + 4 * a.P * LY, LY, lane, rhs, emit);
+  } else {
+    if (SOLVE == SOLVE_LU) {
+      __syncthreads();  // staged tables visible
+      solve_chunk_lu<MODE, CS>(a, t, p, s, m, Y, LY, lane, rhs);
     }
+#pragma unroll
+    for (int k = 0; k < CHUNK; ++k)
+      if (k < m) emit(k, rhs[k]);
   }
 }
 
-// x lines: LX contiguous lines per workgroup staged in LDS (row pitch N+1).
-// The tile region is reused: once every thread holds its window and RHS in
-// registers, the same LDS holds the coefficient tables and the edge-value
-// exchange, and after the solve it holds the results for coalesced stores.
-template <typename TS, int KIND, bool IMPL, int MODE>
+// 16-byte vector of the storage type
+template <typename TS> struct Vec16;
+template <> struct Vec16<double> { using type = double2; static constexpr int W = 2; };
+template <> struct Vec16<float> { using type = float4; static constexpr int W = 4; };
+
+// r <- f(r, o) per component, without taking the vectors' addresses (which
+// would move them to scratch)
+template <typename F>
+__device__ __forceinline__ void vzip(double2& r, const double2& o, F f) {
+  r.x = f(r.x, o.x);
+  r.y = f(r.y, o.y);
+}
+template <typename F>
+__device__ __forceinline__ void vzip(float4& r, const float4& o, F f) {
+  r.x = f(r.x, o.x);
+  r.y = f(r.y, o.y);
+  r.z = f(r.z, o.z);
+  r.w = f(r.w, o.w);
+}
+
+#define XU 8  // unroll of the tile load/store loops (copies in flight per thread)
+
+// x lines: LX contiguous lines per workgroup staged in LDS (row pitch
+// a.pitch). Rows move between HBM and LDS as 16-byte vectors when the line
+// length and pointers allow it, XU copies in flight per thread. The tile is
+// reused for the results, which leave through the same coalesced path.
+// LU solve: the tile region also holds the tables and edge exchange once the
+// windows are in registers. Circulant solve: the carry exchange sits after
+// the tile.
+template <typename TS, int KIND, int SOLVE, int MODE, bool FULL>
 __global__ void __launch_bounds__(WG_MAX) k_line_x(LineArgs a, const TS* __restrict__ in,
                                                    TS* __restrict__ out) {
   constexpr int R = Halo<KIND>::R;
+  constexpr bool PER = SOLVE == SOLVE_CIRC;
   using CS = typename SolveT<MODE>::type;
+  using V = typename Vec16<TS>::type;
+  constexpr int VW = Vec16<TS>::W;
   extern __shared__ __align__(16) unsigned char smem[];
-  const int N = a.N, pitch = a.N + 1, LX = a.lx;
+  const int N = a.N, pitch = a.pitch, LX = a.lx;
   TS* tile = reinterpret_cast<TS*>(smem);
   const int tid = threadIdx.x;
   const int nthr = blockDim.x;
@@ -221,22 +317,35 @@ __global__ void __launch_bounds__(WG_MAX) k_line_x(LineArgs a, const TS* __restr
   const long long rem = (long long)a.nlines - L0;
   const int nval = (int)(rem < LX ? rem : LX);
   const TS* src = in + L0 * N;
-  for (int e = tid; e < nval * N; e += nthr) {
-    const int ll = e / N, i = e - ll * N;
-    tile[ll * pitch + i] = src[e];
+  if (a.vec) {
+    const int vpl = N / VW, nvec = nval * vpl;
+    const V* sv = reinterpret_cast<const V*>(src);
+    // global and LDS never alias, so the unrolled loads issue back to back
+#pragma unroll XU
+    for (int e = tid; e < nvec; e += nthr) {
+      const int ll = e / vpl, i = e - ll * vpl;
+      *reinterpret_cast<V*>(tile + ll * pitch + i * VW) = sv[e];
+    }
+  } else {
+    const int ne = nval * N;
+#pragma unroll XU
+    for (int e = tid; e < ne; e += nthr) {
+      const int ll = e / N, i = e - ll * N;
+      tile[ll * pitch + i] = src[e];
+    }
   }
   __syncthreads();
   const int lr = l < nval ? l : 0;
   const TS* row0 = tile + lr * pitch;
-  const int s = a.cstart[p];
-  const int m = a.cstart[p + 1] - s;
+  const int s = FULL ? p * CHUNK : a.cstart[p];
+  const int m = FULL ? CHUNK : a.cstart[p + 1] - s;
   TS win[CHUNK + 2 * R];
 #pragma unroll
   for (int w = 0; w < CHUNK + 2 * R; ++w) {
     int row = s - R + w;
     TS v = (TS)0;
     if (w < m + 2 * R) {
-      if (a.periodic) {
+      if (PER || a.periodic) {
         row = row < 0 ? row + N : (row >= N ? row - N : row);
         v = row0[row];
       } else if (row >= 0 && row < N) {
@@ -247,35 +356,56 @@ __global__ void __launch_bounds__(WG_MAX) k_line_x(LineArgs a, const TS* __restr
   }
   double rhs[CHUNK];
   rhs_interior<KIND>(a, win, rhs, m);
-  rhs_boundary(a, s, m, rhs, [&](int i) { return row0[i]; });
-  __syncthreads();  // the tile is consumed; reuse its LDS
-  CS* Y = reinterpret_cast<CS*>(smem);
-  Tabs t;
-  if (IMPL) {
-    t = stage_tables<MODE>(a, smem + align16((size_t)4 * a.P * LX * sizeof(CS)), tid, nthr);
+  if (!PER) rhs_boundary(a, s, m, rhs, [&](int i) { return row0[i]; });
+  if (SOLVE == SOLVE_LU) {
+    __syncthreads();  // the tile is consumed; reuse its LDS
+    CS* Y = reinterpret_cast<CS*>(smem);
+    const Tabs t = stage_tables<MODE>(a, smem + align16((size_t)4 * a.P * LX * sizeof(CS)), tid, nthr);
     __syncthreads();
+    solve_chunk_lu<MODE, CS>(a, t, p, s, m, Y, LX, l, rhs);
+    __syncthreads();  // tables and edge values no longer read
+  } else if (SOLVE == SOLVE_NONE) {
+    __syncthreads();  // the tile is consumed
   }
-  solve_chunk<IMPL, MODE, CS>(a, t, p, s, m, Y, LX, l, rhs);
-  __syncthreads();  // tables and edge values no longer read
-  // results go back through the tile in the compute type so that the store
-  // pass can add v (filters) and apply the epilogue without rounding twice
-  double* dtile = reinterpret_cast<double*>(smem);
-  if (l < nval) {
-    double* rowo = dtile + l * pitch;
+  TS* rowo = tile + l * pitch;
+  auto emit = [&](int k, double y) {
+    if (l < nval) rowo[s + k] = (TS)(y * a.scale);
+  };
+  if (SOLVE == SOLVE_CIRC) {
+    // the first carry barrier also retires every read of the tile
+    CS* X0 = reinterpret_cast<CS*>(smem + align16((size_t)LX * pitch * sizeof(TS)));
+    solve_chunk_circ<MODE, CS>(a, p, m, X0, X0 + 4 * a.P * LX, LX, l, rhs, emit);
+  } else {
 #pragma unroll
     for (int k = 0; k < CHUNK; ++k)
-      if (k < m) rowo[s + k] = rhs[k] * a.scale;
+      if (k < m) emit(k, rhs[k]);
   }
   __syncthreads();
   TS* dst = out + L0 * N;
   const bool need_old = (a.epi == EPI_ADD || a.epi == EPI_MAXABS);
-  for (int e = tid; e < nval * N; e += nthr) {
-    const int ll = e / N, i = e - ll * N;
-    double val = dtile[ll * pitch + i];
-    if (a.addv) val += (double)src[e];
-    if (a.epi == EPI_STORE) {
-      dst[e] = (TS)val;
-    } else {
+  if (a.vec) {
+    const int vpl = N / VW, nvec = nval * vpl;
+    const V* sv = reinterpret_cast<const V*>(src);
+    V* dv = reinterpret_cast<V*>(dst);
+#pragma unroll XU
+    for (int e = tid; e < nvec; e += nthr) {
+      const int ll = e / vpl, i = e - ll * vpl;
+      V r = *reinterpret_cast<const V*>(tile + ll * pitch + i * VW);
+      // filters: the sum is formed in FP64 and rounded once to the storage type
+      if (a.addv) vzip(r, sv[e], [](TS y, TS v) { return (TS)((double)y + (double)v); });
+      if (a.epi != EPI_STORE) {
+        const V o = need_old ? dv[e] : V{};
+        vzip(r, o, [&](TS y, TS old) { return epilogue<TS>(a.epi, a.epi_c, (double)y, old); });
+      }
+      dv[e] = r;
+    }
+  } else {
+    const int ne = nval * N;
+#pragma unroll XU
+    for (int e = tid; e < ne; e += nthr) {
+      const int ll = e / N, i = e - ll * N;
+      double val = (double)tile[ll * pitch + i];
+      if (a.addv) val = (double)(TS)(val + (double)src[e]);
       const TS old = need_old ? dst[e] : (TS)0;
       dst[e] = epilogue<TS>(a.epi, a.epi_c, val, old);
     }
@@ -325,6 +455,8 @@ struct DevPlan {
   const short* bidx = nullptr;
   const int* bnt = nullptr;
   int ntab = 0;
+  const double *kf = nullptr, *kb = nullptr;
+  const float *kf32 = nullptr, *kb32 = nullptr;
 };
 
 struct pyr_ctx {
@@ -339,6 +471,8 @@ struct pyr_ctx {
   OpPlan plan[NOPS][3];
   DevPlan dplan[NOPS][3];
   void* scratch[2] = {nullptr, nullptr};  // one field-sized temp per precision
+  int periodic_solver = PYR_PSOLVE_CIRCULANT;
+  int x_lds_kb = 40;                        // LDS budget of one x-line workgroup
   size_t npts() const { return (size_t)n[0] * n[1] * n[2]; }
 };
 
@@ -362,6 +496,11 @@ static int upload_plan(const OpPlan& pl, DevPlan& d) {
   size_t o_rc32 = o_lu32 + n * 5 * sizeof(float);
   size_t o_gs32 = o_rc32 + n * 4 * sizeof(float);
   off = align(o_gs32 + P * 16 * P * sizeof(float));
+  const size_t nk = pl.circ ? 4 * P * P : 0;  // 2P x 2P carry rows
+  size_t o_kf = off; off = align(off + nk * sizeof(double));
+  size_t o_kb = off; off = align(off + nk * sizeof(double));
+  size_t o_kf32 = off; off = align(off + nk * sizeof(float));
+  size_t o_kb32 = off; off = align(off + nk * sizeof(float));
   std::vector<unsigned char> host(off, 0);
   std::memcpy(&host[o_cs], pl.cstart.data(), (P + 1) * sizeof(int));
   std::memcpy(&host[o_lu], pl.lu.data(), n * 5 * sizeof(double));
@@ -377,6 +516,16 @@ static int upload_plan(const OpPlan& pl, DevPlan& d) {
   for (size_t i = 0; i < n * 5; ++i) lu32[i] = (float)pl.lu[i];
   for (size_t i = 0; i < n * 4; ++i) rc32[i] = (float)pl.rc[i];
   for (size_t i = 0; i < P * 16 * P; ++i) gs32[i] = (float)pl.gsel[i];
+  if (nk) {
+    std::memcpy(&host[o_kf], pl.kf.data(), nk * sizeof(double));
+    std::memcpy(&host[o_kb], pl.kb.data(), nk * sizeof(double));
+    float* kf32 = reinterpret_cast<float*>(&host[o_kf32]);
+    float* kb32 = reinterpret_cast<float*>(&host[o_kb32]);
+    for (size_t i = 0; i < nk; ++i) {
+      kf32[i] = (float)pl.kf[i];
+      kb32[i] = (float)pl.kb[i];
+    }
+  }
   HIPCHECK(hipMalloc(&d.mem, off));
   HIPCHECK(hipMemcpy(d.mem, host.data(), off, hipMemcpyHostToDevice));
   unsigned char* b = static_cast<unsigned char*>(d.mem);
@@ -392,15 +541,28 @@ static int upload_plan(const OpPlan& pl, DevPlan& d) {
   d.rc32 = reinterpret_cast<const float*>(b + o_rc32);
   d.gsel32 = reinterpret_cast<const float*>(b + o_gs32);
   d.ntab = (int)(n * 9 + P * 16 * P);
+  if (nk) {
+    d.kf = reinterpret_cast<const double*>(b + o_kf);
+    d.kb = reinterpret_cast<const double*>(b + o_kb);
+    d.kf32 = reinterpret_cast<const float*>(b + o_kf32);
+    d.kb32 = reinterpret_cast<const float*>(b + o_kb32);
+  }
   return 0;
 }
 
 // host pointers of the same tables (host emulation)
 static void host_tables(const OpPlan& pl, DevPlan& d, std::vector<float>& lu32,
-                        std::vector<float>& rc32, std::vector<float>& gs32) {
+                        std::vector<float>& rc32, std::vector<float>& gs32,
+                        std::vector<float>& kf32, std::vector<float>& kb32) {
   lu32.assign(pl.lu.begin(), pl.lu.end());
   rc32.assign(pl.rc.begin(), pl.rc.end());
   gs32.assign(pl.gsel.begin(), pl.gsel.end());
+  kf32.assign(pl.kf.begin(), pl.kf.end());
+  kb32.assign(pl.kb.begin(), pl.kb.end());
+  d.kf = pl.kf.data();
+  d.kb = pl.kb.data();
+  d.kf32 = kf32.data();
+  d.kb32 = kb32.data();
   d.cstart = pl.cstart.data();
   d.lu = pl.lu.data();
   d.rc = pl.rc.data();
@@ -441,7 +603,18 @@ static LineArgs make_args(const pyr_ctx* c, const OpPlan& pl, const DevPlan& d, 
   a.lu32 = d.lu32; a.rc32 = d.rc32; a.gsel32 = d.gsel32;
   a.tab64 = d.lu; a.tab32 = d.lu32; a.ntab = d.ntab;
   a.bcoef = d.bcoef; a.bidx = d.bidx; a.bnt = d.bnt;
+  a.p1 = pl.p1; a.p2 = pl.p2; a.invc = pl.invc;
+  a.p1f = (float)pl.p1; a.p2f = (float)pl.p2; a.invcf = (float)pl.invc;
+  for (int i = 0; i < 3; ++i) a.bd[i] = pl.bd[i];
+  a.kf = d.kf; a.kb = d.kb; a.kf32 = d.kf32; a.kb32 = d.kb32;
   return a;
+}
+
+// solve used for a plan: explicit, chunked LU, or circulant (periodic lines)
+static int plan_solve(const pyr_ctx* c, const OpPlan& pl) {
+  if (!pl.implicit) return SOLVE_NONE;
+  if (pl.circ && c->periodic_solver == PYR_PSOLVE_CIRCULANT) return SOLVE_CIRC;
+  return SOLVE_LU;
 }
 
 // ---------------------------------------------------------------------------
@@ -454,40 +627,77 @@ static int set_lds(K kernel, size_t bytes) {
   return 0;
 }
 
-template <typename TS, int KIND, bool IMPL, int MODE>
+template <typename TS, int KIND, int SOLVE, int MODE, bool FULL>
 static int launch_typed(pyr_ctx* c, int dir, LineArgs a, const TS* in, TS* out) {
   using CS = typename SolveT<MODE>::type;
   const int P = a.P;
-  const size_t stage = IMPL ? stage_bytes<MODE>(a.ntab, a.N) : 0;
+  const size_t stage = SOLVE == SOLVE_LU ? stage_bytes<MODE>(a.ntab, a.N) : 0;
   if (dir == 0) {
+    constexpr int VW = Vec16<TS>::W;
+    a.vec = (a.N % VW == 0) && ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 16 == 0);
+    a.pitch = a.vec ? a.N + VW : a.N + 1;
+    auto lds = [&](int lx) {
+      const size_t tile = align16((size_t)lx * a.pitch * sizeof(TS));
+      if (SOLVE == SOLVE_CIRC) return tile + 2 * align16((size_t)4 * P * lx * sizeof(CS));
+      if (SOLVE == SOLVE_LU) {
+        const size_t work = align16((size_t)4 * P * lx * sizeof(CS)) + stage;
+        return tile > work ? tile : work;
+      }
+      return tile;
+    };
     int lx = WG_MAX / P;
     if (lx > 64) lx = 64;
-    // keep the staged tile within 40 KiB so several workgroups share a CU
-    while (lx > 4 && (size_t)lx * (a.N + 1) * sizeof(double) > 40 * 1024) lx /= 2;
+    // keep the workgroup's LDS within the budget so several workgroups share a CU
+    while (lx > 1 && lds(lx) > (size_t)c->x_lds_kb * 1024) lx /= 2;
     a.lx = lx;
-    // the tile region holds the input (TS) and later the results (double)
-    const size_t tile = align16((size_t)lx * (a.N + 1) * sizeof(double));
-    const size_t work = align16((size_t)4 * P * lx * sizeof(CS)) + stage;
-    const size_t shm = tile > work ? tile : work;
+    const size_t shm = lds(lx);
     const dim3 grid((a.nlines + lx - 1) / lx), block(lx * P);
-    auto k = k_line_x<TS, KIND, IMPL, MODE>;
-    if (set_lds(k, shm)) return 1;
-    hipLaunchKernelGGL(k, grid, block, shm, c->stream, a, in, out);
-  } else if (P <= 8) {
-    const size_t shm = align16((size_t)4 * P * 64 * sizeof(CS)) + stage;
-    const dim3 grid((a.nlines + 63) / 64), block(64, P);
-    auto k = k_line_yz<TS, KIND, IMPL, MODE, 64>;
+    auto k = k_line_x<TS, KIND, SOLVE, MODE, FULL>;
     if (set_lds(k, shm)) return 1;
     hipLaunchKernelGGL(k, grid, block, shm, c->stream, a, in, out);
   } else {
-    const size_t shm = align16((size_t)4 * P * 32 * sizeof(CS)) + stage;
-    const dim3 grid((a.nlines + 31) / 32), block(32, P);
-    auto k = k_line_yz<TS, KIND, IMPL, MODE, 32>;
-    if (set_lds(k, shm)) return 1;
-    hipLaunchKernelGGL(k, grid, block, shm, c->stream, a, in, out);
+    const int ly = P <= 8 ? 64 : 32;
+    const size_t ex = align16((size_t)4 * P * ly * sizeof(CS));
+    const size_t shm = SOLVE == SOLVE_CIRC ? 2 * ex : (SOLVE == SOLVE_LU ? ex + stage : 0);
+    const dim3 grid((a.nlines + ly - 1) / ly), block(ly, P);
+    if (ly == 64) {
+      auto k = k_line_yz<TS, KIND, SOLVE, MODE, 64, FULL>;
+      if (set_lds(k, shm)) return 1;
+      hipLaunchKernelGGL(k, grid, block, shm, c->stream, a, in, out);
+    } else {
+      auto k = k_line_yz<TS, KIND, SOLVE, MODE, 32, FULL>;
+      if (set_lds(k, shm)) return 1;
+      hipLaunchKernelGGL(k, grid, block, shm, c->stream, a, in, out);
+    }
   }
   HIPCHECK(hipGetLastError());
   return 0;
+}
+
+template <typename TS, int MODE, bool FULL>
+static int launch_kind(pyr_ctx* c, int dir, int kind, int sv, const LineArgs& a, const TS* in, TS* out) {
+  if constexpr (FULL) {
+    switch (kind) {
+      case K_D1: return launch_typed<TS, K_D1, SOLVE_CIRC, MODE, true>(c, dir, a, in, out);
+      case K_R3: return launch_typed<TS, K_R3, SOLVE_CIRC, MODE, true>(c, dir, a, in, out);
+      default:
+        if (sv == SOLVE_CIRC) return launch_typed<TS, K_R4, SOLVE_CIRC, MODE, true>(c, dir, a, in, out);
+        return launch_typed<TS, K_R4, SOLVE_NONE, MODE, true>(c, dir, a, in, out);
+    }
+  } else {
+    switch (kind) {
+      case K_D1:
+        return sv == SOLVE_CIRC ? launch_typed<TS, K_D1, SOLVE_CIRC, MODE, false>(c, dir, a, in, out)
+                                : launch_typed<TS, K_D1, SOLVE_LU, MODE, false>(c, dir, a, in, out);
+      case K_R3:
+        return sv == SOLVE_CIRC ? launch_typed<TS, K_R3, SOLVE_CIRC, MODE, false>(c, dir, a, in, out)
+                                : launch_typed<TS, K_R3, SOLVE_LU, MODE, false>(c, dir, a, in, out);
+      default:
+        if (sv == SOLVE_CIRC) return launch_typed<TS, K_R4, SOLVE_CIRC, MODE, false>(c, dir, a, in, out);
+        if (sv == SOLVE_LU) return launch_typed<TS, K_R4, SOLVE_LU, MODE, false>(c, dir, a, in, out);
+        return launch_typed<TS, K_R4, SOLVE_NONE, MODE, false>(c, dir, a, in, out);
+    }
+  }
 }
 
 template <typename TS, int MODE>
@@ -501,13 +711,12 @@ static int launch_op(pyr_ctx* c, int op, int dir, const TS* in, TS* out, int epi
     return 0;
   }
   const LineArgs a = make_args(c, pl, c->dplan[op][dir], dir, epi, ec);
-  switch (pl.kind) {
-    case K_D1: return launch_typed<TS, K_D1, true, MODE>(c, dir, a, in, out);
-    case K_R3: return launch_typed<TS, K_R3, true, MODE>(c, dir, a, in, out);
-    default:
-      if (pl.implicit) return launch_typed<TS, K_R4, true, MODE>(c, dir, a, in, out);
-      return launch_typed<TS, K_R4, false, MODE>(c, dir, a, in, out);
-  }
+  const int sv = plan_solve(c, pl);
+  // the balanced partition gives every chunk CHUNK rows exactly when CHUNK | N;
+  // the LU solve keeps runtime row counts (with compile-time counts the
+  // compiler hoists every table read of the chunk and runs out of registers)
+  if (pl.N % CHUNK == 0 && sv != SOLVE_LU) return launch_kind<TS, MODE, true>(c, dir, pl.kind, sv, a, in, out);
+  return launch_kind<TS, MODE, false>(c, dir, pl.kind, sv, a, in, out);
 }
 
 static int check_ctx(pyr_ctx* c) {
@@ -573,6 +782,10 @@ static int setup_common(pyr_ctx** out, int nx, int ny, int nz, double x1, double
   if (!bc6) return fail("null bc array");
   std::unique_ptr<pyr_ctx> c(new pyr_ctx);
   c->n[0] = nx; c->n[1] = ny; c->n[2] = nz;
+  if (const char* e = std::getenv("PYRANDA_AMD_X_LDS_KB")) {
+    const int kb = std::atoi(e);
+    if (kb >= 8 && kb <= 160) c->x_lds_kb = kb;
+  }
   for (int i = 0; i < 6; ++i) {
     if (bc6[i] < PYR_BC_PERI || bc6[i] > PYR_BC_SYMM) return fail("bad boundary code");
     c->bc[i] = bc6[i];
@@ -612,9 +825,10 @@ static int setup_common(pyr_ctx** out, int nx, int ny, int nz, double x1, double
 }
 
 // ---------------------------------------------------------------------------
-// host emulation of the kernels (tests only)
+// host emulation of the kernels (tests only): the same per-thread arithmetic,
+// chunk by chunk, with the LDS exchanges as plain arrays
 // ---------------------------------------------------------------------------
-template <typename TS, int KIND, bool IMPL, int MODE>
+template <typename TS, int KIND, int SOLVE, int MODE>
 static void emulate_line(const LineArgs& a, const TS* vin, long long stride, TS* vout) {
   constexpr int R = Halo<KIND>::R;
   using CS = typename SolveT<MODE>::type;
@@ -626,7 +840,7 @@ static void emulate_line(const LineArgs& a, const TS* vin, long long stride, TS*
     int s, m;
   };
   std::vector<St> st(P);
-  std::vector<CS> Y(4 * P);
+  std::vector<CS> Y(4 * P), Yb(4 * P);
   auto rd = [&](int i) { return vin[stride * i]; };
   for (int p = 0; p < P; ++p) {
     St& t = st[p];
@@ -647,18 +861,34 @@ static void emulate_line(const LineArgs& a, const TS* vin, long long stride, TS*
     }
     rhs_interior<KIND>(a, t.win, t.rhs, t.m);
     rhs_boundary(a, t.s, t.m, t.rhs, rd);
-    if (IMPL) {
+  }
+  // circulant solve of every chunk's vector `v` (without 1/c), as circ_solve
+  auto circ_all = [&](auto getv, const CS* kf, const CS* kb, CS p1, CS p2) {
+    for (int p = 0; p < P; ++p) circ_fwd_exit(getv(p), st[p].m, p1, p2, Y[2 * p], Y[2 * p + 1]);
+    for (int p = 0; p < P; ++p) {
+      CS e0, e1;
+      circ_carry(kf, p, P, [&](int cc) { return Y[cc]; }, e0, e1);
+      circ_fwd(getv(p), st[p].m, p1, p2, e0, e1);
+      circ_bwd_exit(getv(p), st[p].m, p1, p2, Yb[2 * p], Yb[2 * p + 1]);
+    }
+    for (int p = 0; p < P; ++p) {
+      CS e0, e1;
+      circ_carry(kb, p, P, [&](int cc) { return Yb[cc]; }, e0, e1);
+      circ_bwd(getv(p), st[p].m, p1, p2, e0, e1);
+    }
+  };
+  if (SOLVE == SOLVE_LU) {
+    const CS* G = MODE == MODE_F64 ? (const CS*)a.gsel : (const CS*)a.gsel32;
+    const CS* RC = MODE == MODE_F64 ? (const CS*)a.rc : (const CS*)a.rc32;
+    const CS* LU = MODE == MODE_F64 ? (const CS*)a.lu : (const CS*)a.lu32;
+    for (int p = 0; p < P; ++p) {
+      St& t = st[p];
       for (int k = 0; k < CHUNK; ++k) t.x[k] = (CS)t.rhs[k];
-      local_solve<CS>(MODE == MODE_F64 ? (const CS*)a.lu : (const CS*)a.lu32, t.s, t.m, t.x);
+      local_solve<CS>(LU, t.s, t.m, t.x);
       CS e4[4];
       edges(t.x, t.m, e4);
       for (int q = 0; q < 4; ++q) Y[4 * p + q] = e4[q];
     }
-  }
-  if (IMPL) {
-    const CS* G = MODE == MODE_F64 ? (const CS*)a.gsel : (const CS*)a.gsel32;
-    const CS* RC = MODE == MODE_F64 ? (const CS*)a.rc : (const CS*)a.rc32;
-    const CS* LU = MODE == MODE_F64 ? (const CS*)a.lu : (const CS*)a.lu32;
     for (int p = 0; p < P; ++p) {
       St& t = st[p];
       neighbours<CS>(G, p, P, [&](int cc) { return Y[cc]; }, t.z);
@@ -669,7 +899,8 @@ static void emulate_line(const LineArgs& a, const TS* vin, long long stride, TS*
       for (int p = 0; p < P; ++p) {
         St& t = st[p];
         double res[CHUNK];
-        residual64(a.band, t.s, t.m, t.rhs, (const float*)t.x, (const float*)t.z, res);
+        residual64([&](int k) { return a.band + 5 * (t.s + k); }, t.m, t.rhs, (const float*)t.x,
+                   (const float*)t.z, res);
         for (int k = 0; k < CHUNK; ++k) t.d[k] = (CS)res[k];
         local_solve<CS>(LU, t.s, t.m, t.d);
         CS f4[4];
@@ -682,12 +913,42 @@ static void emulate_line(const LineArgs& a, const TS* vin, long long stride, TS*
         spike_correct(RC, t.s, t.m, t.zd, t.d);
       }
     }
+  } else if (SOLVE == SOLVE_CIRC) {
+    for (int p = 0; p < P; ++p)
+      for (int k = 0; k < CHUNK; ++k) st[p].x[k] = (CS)st[p].rhs[k];
+    auto gx = [&](int p) { return st[p].x; };
+    if (MODE == MODE_F64) {
+      circ_all(gx, (const CS*)a.kf, (const CS*)a.kb, (CS)a.p1, (CS)a.p2);
+      for (int p = 0; p < P; ++p)
+        for (int k = 0; k < CHUNK; ++k) st[p].x[k] *= (CS)a.invc;
+    } else {
+      const CS *kf = (const CS*)a.kf32, *kb = (const CS*)a.kb32;
+      const CS p1 = (CS)a.p1f, p2 = (CS)a.p2f, ic = (CS)a.invcf;
+      circ_all(gx, kf, kb, p1, p2);
+      std::vector<CS> E(4 * P);
+      for (int p = 0; p < P; ++p) {
+        for (int k = 0; k < CHUNK; ++k) st[p].x[k] *= ic;
+        edges(st[p].x, st[p].m, &E[4 * p]);
+      }
+      const double row[5] = {a.bd[0], a.bd[1], a.bd[2], a.bd[1], a.bd[0]};
+      for (int p = 0; p < P; ++p) {
+        St& t = st[p];
+        const int pm = (p - 1 + P) % P, pp = (p + 1) % P;
+        const float z[4] = {(float)E[4 * pm + 2], (float)E[4 * pm + 3], (float)E[4 * pp], (float)E[4 * pp + 1]};
+        double res[CHUNK];
+        residual64([&](int) { return row; }, t.m, t.rhs, (const float*)t.x, z, res);
+        for (int k = 0; k < CHUNK; ++k) t.d[k] = (CS)res[k];
+      }
+      circ_all([&](int p) { return st[p].d; }, kf, kb, p1, p2);
+      for (int p = 0; p < P; ++p)
+        for (int k = 0; k < CHUNK; ++k) st[p].d[k] *= ic;
+    }
   }
   for (int p = 0; p < P; ++p) {
     St& t = st[p];
     for (int k = 0; k < t.m; ++k) {
       double y;
-      if (!IMPL) y = t.rhs[k];
+      if (SOLVE == SOLVE_NONE) y = t.rhs[k];
       else if (MODE == MODE_F64) y = (double)t.x[k];
       else y = (double)t.x[k] + (double)t.d[k];
       double val = y * a.scale;
@@ -706,17 +967,27 @@ static int emulate(pyr_ctx* c, int op, int dir, const TS* in, TS* out) {
     return 0;
   }
   DevPlan d;
-  std::vector<float> lu32, rc32, gs32;
-  host_tables(pl, d, lu32, rc32, gs32);
+  std::vector<float> lu32, rc32, gs32, kf32, kb32;
+  host_tables(pl, d, lu32, rc32, gs32, kf32, kb32);
   const LineArgs a = make_args(c, pl, d, dir, EPI_STORE, 0.0);
+  const int sv = plan_solve(c, pl);
   for (long long L = 0; L < a.nlines; ++L) {
     const long long base = (dir == 0) ? L * pl.N : (L % a.A) + a.B * (L / a.A);
+    const TS* vi = in + base;
+    TS* vo = out + base;
     switch (pl.kind) {
-      case K_D1: emulate_line<TS, K_D1, true, MODE>(a, in + base, a.stride, out + base); break;
-      case K_R3: emulate_line<TS, K_R3, true, MODE>(a, in + base, a.stride, out + base); break;
+      case K_D1:
+        if (sv == SOLVE_CIRC) emulate_line<TS, K_D1, SOLVE_CIRC, MODE>(a, vi, a.stride, vo);
+        else emulate_line<TS, K_D1, SOLVE_LU, MODE>(a, vi, a.stride, vo);
+        break;
+      case K_R3:
+        if (sv == SOLVE_CIRC) emulate_line<TS, K_R3, SOLVE_CIRC, MODE>(a, vi, a.stride, vo);
+        else emulate_line<TS, K_R3, SOLVE_LU, MODE>(a, vi, a.stride, vo);
+        break;
       default:
-        if (pl.implicit) emulate_line<TS, K_R4, true, MODE>(a, in + base, a.stride, out + base);
-        else emulate_line<TS, K_R4, false, MODE>(a, in + base, a.stride, out + base);
+        if (sv == SOLVE_CIRC) emulate_line<TS, K_R4, SOLVE_CIRC, MODE>(a, vi, a.stride, vo);
+        else if (sv == SOLVE_LU) emulate_line<TS, K_R4, SOLVE_LU, MODE>(a, vi, a.stride, vo);
+        else emulate_line<TS, K_R4, SOLVE_NONE, MODE>(a, vi, a.stride, vo);
     }
   }
   return 0;
@@ -755,6 +1026,13 @@ int pyr_destroy(pyr_ctx* c) {
 int pyr_set_stream(pyr_ctx* c, void* stream) {
   if (!c) return fail("null context");
   c->stream = (hipStream_t)stream;
+  return 0;
+}
+
+int pyr_set_periodic_solver(pyr_ctx* c, int mode) {
+  if (!c) return fail("null context");
+  if (mode != PYR_PSOLVE_CIRCULANT && mode != PYR_PSOLVE_LU) return fail("bad periodic solver");
+  c->periodic_solver = mode;
   return 0;
 }
 

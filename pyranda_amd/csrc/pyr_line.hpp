@@ -51,6 +51,16 @@ struct LineArgs {
   const double* bcoef;
   const short* bidx;
   const int* bnt;
+  // x kernel: row pitch of the LDS tile, and whether rows move as 16-byte vectors
+  int pitch, vec;
+  // circulant solve (periodic lines): A = c Q^T Q, Q = I + p1 S + p2 S^2
+  double p1, p2, invc;
+  float p1f, p2f, invcf;
+  double bd[3];            // LHS row A[i][i+-2], A[i][i+-1], A[i][i]
+  const double* kf;        // P x 2 x 2P carry rows, forward and backward
+  const double* kb;
+  const float* kf32;
+  const float* kb32;
 };
 
 template <int KIND> struct Halo { static constexpr int R = (KIND == K_R4) ? 4 : 3; };
@@ -183,10 +193,106 @@ PYR_HD void spike_correct(const T* rc, int s, int m, const C* z, C* x) {
   }
 }
 
+// --- circulant solve (periodic lines) -------------------------------------
+// Forward recurrence x_k = r_k - p1 x_{k-1} - p2 x_{k-2} over rows k < m, in
+// place, entered with (x_{-2}, x_{-1}) = (a0, a1).
+template <typename C>
+PYR_HD void circ_fwd(C* x, int m, C p1, C p2, C a0, C a1) {
+#pragma unroll
+  for (int k = 0; k < CHUNK; ++k) {
+    if (k < m) {
+      const C v = x[k] - p1 * a1 - p2 * a0;
+      x[k] = v;
+      a0 = a1;
+      a1 = v;
+    }
+  }
+}
+
+// Exit state (x_{m-2}, x_{m-1}) of the forward recurrence entered with zeros.
+template <typename C>
+PYR_HD void circ_fwd_exit(const C* x, int m, C p1, C p2, C& e0, C& e1) {
+  C a0 = 0, a1 = 0;
+#pragma unroll
+  for (int k = 0; k < CHUNK; ++k) {
+    if (k < m) {
+      const C v = x[k] - p1 * a1 - p2 * a0;
+      a0 = a1;
+      a1 = v;
+    }
+  }
+  e0 = a0;
+  e1 = a1;
+}
+
+// Backward recurrence u_k = w_k - p1 u_{k+1} - p2 u_{k+2}, in place, entered
+// with (u_m, u_{m+1}) = (b1, b2).
+template <typename C>
+PYR_HD void circ_bwd(C* x, int m, C p1, C p2, C b1, C b2) {
+#pragma unroll
+  for (int k = CHUNK - 1; k >= 0; --k) {
+    if (k < m) {
+      const C v = x[k] - p1 * b1 - p2 * b2;
+      x[k] = v;
+      b2 = b1;
+      b1 = v;
+    }
+  }
+}
+
+// Final backward recurrence: each finished row goes to sink(k, u_k) instead
+// of back into x, so no result stays live in registers.
+template <typename C, typename Sink>
+PYR_HD void circ_bwd_sink(const C* x, int m, C p1, C p2, C b1, C b2, Sink sink) {
+#pragma unroll
+  for (int k = CHUNK - 1; k >= 0; --k) {
+    if (k < m) {
+      const C v = x[k] - p1 * b1 - p2 * b2;
+      b2 = b1;
+      b1 = v;
+      sink(k, v);
+    }
+  }
+}
+
+// Exit state (u_0, u_1) of the backward recurrence entered with zeros.
+template <typename C>
+PYR_HD void circ_bwd_exit(const C* x, int m, C p1, C p2, C& e0, C& e1) {
+  C b1 = 0, b2 = 0;
+#pragma unroll
+  for (int k = CHUNK - 1; k >= 0; --k) {
+    if (k < m) {
+      const C v = x[k] - p1 * b1 - p2 * b2;
+      b2 = b1;
+      b1 = v;
+    }
+  }
+  e0 = b1;
+  e1 = b2;
+}
+
+// Entry state of chunk p: rows 2p, 2p+1 of K times the exits yr(c), c < 2P.
+template <typename C, typename T, typename YR>
+PYR_HD void circ_carry(const T* K, int p, int P, YR yr, C& s0, C& s1) {
+  const int n2 = 2 * P;
+  const T* k0 = K + (size_t)(2 * p) * n2;
+  const T* k1 = k0 + n2;
+  C a = 0, b = 0;
+  for (int c = 0; c < n2; ++c) {
+    const C y = yr(c);
+    a += (C)k0[c] * y;
+    b += (C)k1[c] * y;
+  }
+  s0 = a;
+  s1 = b;
+}
+
 // FP64 residual r = rhs - A y for rows of the chunk; y outside the chunk comes
-// from the neighbour values z = (y[s-2], y[s-1], y[e], y[e+1]).
-PYR_HD void residual64(const double* band, int s, int m, const double* rhs, const float* y,
-                       const float* z, double* res) {
+// from the neighbour values z = (y[s-2], y[s-1], y[e], y[e+1]). row(k) gives
+// the five band entries A[s+k][s+k-2 .. s+k+2].
+template <typename Row>
+PYR_HD void residual64(Row row, int m, const double* rhs, const float* y, const float* z,
+                       double* res) {
 #pragma unroll
   for (int k = 0; k < CHUNK; ++k) {
     double yy[5];
@@ -205,7 +311,7 @@ PYR_HD void residual64(const double* band, int s, int m, const double* rhs, cons
     }
     double r = 0.0;
     if (k < m) {
-      const double* b = band + 5 * (s + k);
+      const double* b = row(k);
       r = rhs[k] - (b[0] * yy[0] + b[1] * yy[1] + b[2] * yy[2] + b[3] * yy[3] + b[4] * yy[4]);
     }
     res[k] = r;

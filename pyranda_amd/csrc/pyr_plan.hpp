@@ -12,6 +12,15 @@
 // construction handles the cyclic coupling of periodic lines (chunk 0's left
 // neighbour is chunk P-1), so no Sherman-Morrison step is needed.
 //
+// Periodic lines have a circulant LHS, which factors exactly as
+//     A = c Q^T Q,   Q = I + p1 S + p2 S^2   ((S x)_i = x_{i-1}, cyclic)
+// with both roots of 1 + p1 z + p2 z^2 outside the unit circle. A solve is then
+// a forward and a backward second-order recurrence with constant coefficients;
+// a chunk runs each recurrence from a zero state to get its exit state, the
+// true entry states of all chunks follow from a 2P x 2P carry system (rows
+// `kf`, `kb`), and the chunk re-runs the recurrence from its true entry. No
+// per-row tables are read.
+//
 // RHS rows: interior rows use the interior weights; the up-to-4 rows at each
 // end of a bounded line are stored as term lists coef*(v[a]-v[b]) that keep
 // the reference's difference forms (compact_d1.f90:113-207,
@@ -20,6 +29,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <complex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -51,6 +61,11 @@ struct OpPlan {
   std::vector<double> bcoef;  // 2*NB x MAXT
   std::vector<short> bidx;    // 2*NB x MAXT x 2 (a,b); -1 reads as 0
   std::vector<int> bnt;       // 2*NB
+  // circulant solve of periodic implicit lines
+  int circ = 0;
+  double p1 = 0, p2 = 0, invc = 0;  // A = c Q^T Q, Q = I + p1 S + p2 S^2
+  double bd[3] = {};                // LHS row: A[i][i+-2], A[i][i+-1], A[i][i]
+  std::vector<double> kf, kb;       // P x 2 x 2P: entry states from all exit states
 };
 
 namespace detail {
@@ -173,6 +188,88 @@ inline void invert(std::vector<ld>& a, int n) {
   a.swap(inv);
 }
 
+// Spectral factor of the symmetric pentadiagonal symbol
+//   b/z^2 + a/z + d + a z + b z^2 = c (1 + p1 z + p2 z^2)(1 + p1/z + p2/z^2)
+// with the roots of 1 + p1 z + p2 z^2 outside the unit circle. With w = z + 1/z
+// the quartic becomes b w^2 + a w + (d - 2b) = 0; each w gives the pair
+// z, 1/z, of which the root inside the unit circle is kept. False if the
+// symbol vanishes on the unit circle or b = 0.
+inline bool spectral_factor(ld b, ld a, ld d, ld& p1, ld& p2, ld& c) {
+  using cx = std::complex<ld>;
+  if (b == 0.0L) return false;
+  const cx disc = std::sqrt(cx(a * a - 4.0L * b * (d - 2.0L * b), 0.0L));
+  const cx w[2] = {(cx(-a, 0.0L) + disc) / (2.0L * b), (cx(-a, 0.0L) - disc) / (2.0L * b)};
+  cx r[2];
+  for (int i = 0; i < 2; ++i) {
+    const cx sq = std::sqrt(w[i] * w[i] - cx(4.0L, 0.0L));
+    const cx z1 = (w[i] + sq) / 2.0L, z2 = (w[i] - sq) / 2.0L;
+    r[i] = std::abs(z1) < std::abs(z2) ? z1 : z2;
+    if (!(std::abs(r[i]) < 1.0L - 1e-9L)) return false;
+  }
+  const cx s1 = -(r[0] + r[1]), s2 = r[0] * r[1];
+  if (std::fabs(s1.imag()) > 1e-15L || std::fabs(s2.imag()) > 1e-15L) return false;
+  p1 = s1.real();
+  p2 = s2.real();
+  c = b / p2;
+  const ld ed = c * (1.0L + p1 * p1 + p2 * p2) - d, ea = c * p1 * (1.0L + p2) - a;
+  const ld scale = std::fabs(d) + 2.0L * std::fabs(a) + 2.0L * std::fabs(b);
+  return std::fabs(ed) <= 1e-14L * scale && std::fabs(ea) <= 1e-14L * scale;
+}
+
+// Carry rows of the circulant solve. Forward: chunk q entered with
+// (x[s_q-2], x[s_q-1]) leaves with its zero-entry exit e_q plus H_q times the
+// entry, and chunk q+1 (cyclic) is entered with that exit. Backward likewise
+// from the right with (u[e_q], u[e_q+1]). K maps the exits of all chunks
+// (layout 2q + i) to the entry of each chunk (rows 2p + i).
+inline void circ_carry_rows(int P, const std::vector<int>& cstart, ld p1, ld p2,
+                            std::vector<double>& kf, std::vector<double>& kb) {
+  const int n2 = 2 * P;
+  std::vector<ld> Mf(size_t(n2) * n2, 0.0L), Mb(size_t(n2) * n2, 0.0L);
+  for (int q = 0; q < P; ++q) {
+    const int m = cstart[q + 1] - cstart[q];
+    ld Hf[2][2], Hb[2][2];
+    for (int j = 0; j < 2; ++j) {
+      ld xm2 = j == 0, xm1 = j == 1;  // forward state (x[k-2], x[k-1])
+      for (int k = 0; k < m; ++k) {
+        const ld x = -p1 * xm1 - p2 * xm2;
+        xm2 = xm1;
+        xm1 = x;
+      }
+      Hf[0][j] = xm2;
+      Hf[1][j] = xm1;
+      ld u1 = j == 0, u2 = j == 1;  // backward state (u[k+1], u[k+2])
+      for (int k = m - 1; k >= 0; --k) {
+        const ld u = -p1 * u1 - p2 * u2;
+        u2 = u1;
+        u1 = u;
+      }
+      Hb[0][j] = u1;
+      Hb[1][j] = u2;
+    }
+    const int qn = (q + 1) % P, qp = (q - 1 + P) % P;
+    for (int i = 0; i < 2; ++i) {
+      Mf[size_t(2 * qn + i) * n2 + 2 * qn + i] += 1.0L;
+      Mb[size_t(2 * qp + i) * n2 + 2 * qp + i] += 1.0L;
+      for (int j = 0; j < 2; ++j) {
+        Mf[size_t(2 * qn + i) * n2 + 2 * q + j] -= Hf[i][j];
+        Mb[size_t(2 * qp + i) * n2 + 2 * q + j] -= Hb[i][j];
+      }
+    }
+  }
+  invert(Mf, n2);
+  invert(Mb, n2);
+  // the exit of chunk q is the right-hand side of row block q+1 (forward) or
+  // q-1 (backward)
+  kf.assign(size_t(n2) * n2, 0.0);
+  kb.assign(size_t(n2) * n2, 0.0);
+  for (int r = 0; r < n2; ++r)
+    for (int q = 0; q < P; ++q)
+      for (int i = 0; i < 2; ++i) {
+        kf[size_t(r) * n2 + 2 * q + i] = (double)Mf[size_t(r) * n2 + 2 * ((q + 1) % P) + i];
+        kb[size_t(r) * n2 + 2 * q + i] = (double)Mb[size_t(r) * n2 + 2 * ((q - 1 + P) % P) + i];
+      }
+}
+
 }  // namespace detail
 
 // Build the plan. v1/v2 are the boundary variants (0 NONE, +1 SYMM).
@@ -261,6 +358,20 @@ inline OpPlan build_plan(int op, int n, int periodic, int v1, int v2, double met
   pl.rc.assign(size_t(n) * 4, 0.0);
   pl.gsel.assign(size_t(P) * 4 * 4 * P, 0.0);
   if (!sc.implicit) return pl;
+
+  if (periodic && sc.ncl == 5 && sc.ali[0] == sc.ali[4] && sc.ali[1] == sc.ali[3]) {
+    ld p1, p2, cc;
+    if (detail::spectral_factor(sc.ali[0], sc.ali[1], sc.ali[2], p1, p2, cc)) {
+      pl.circ = 1;
+      pl.p1 = (double)p1;
+      pl.p2 = (double)p2;
+      pl.invc = (double)(1.0L / cc);
+      pl.bd[0] = sc.ali[0];
+      pl.bd[1] = sc.ali[1];
+      pl.bd[2] = sc.ali[2];
+      detail::circ_carry_rows(P, pl.cstart, p1, p2, pl.kf, pl.kb);
+    }
+  }
 
   auto Aent = [&](int i, int col) -> double {  // A[i][col], col may wrap
     const int off = col - i;

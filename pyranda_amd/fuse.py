@@ -31,7 +31,6 @@ _BINARY = {"minimum": "pyr_min", "maximum": "pyr_max", "arctan2": "atan2"}
 _CMP = {ast.Lt: "<", ast.Gt: ">", ast.LtE: "<=", ast.GtE: ">=", ast.Eq: "==", ast.NotEq: "!="}
 
 PRELUDE = r"""
-#include <hip/hip_runtime.h>
 struct PyrPwArgs { long long n; void* p[24]; double s[16]; };
 __device__ __forceinline__ double pyr_min(double a, double b) { return a < b ? a : b; }
 __device__ __forceinline__ double pyr_max(double a, double b) { return a > b ? a : b; }

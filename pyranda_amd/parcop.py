@@ -100,6 +100,10 @@ class ParcopContext:
             _lib.check(L.pyr_axis_coords(h, ax, buf))
             self.coords1d.append(list(buf))
 
+    def set_periodic_solver(self, name):
+        """'circulant' (default) or 'lu': solver of periodic implicit lines."""
+        _lib.check(self._L.pyr_set_periodic_solver(self._h, _lib.PSOLVE[name]))
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             self._L.pyr_destroy(self._h)

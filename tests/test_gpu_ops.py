@@ -73,10 +73,14 @@ def _rel(got, ref):
     return np.abs(got - ref).max() / max(np.abs(ref).max(), 1e-300)
 
 
+@pytest.mark.parametrize("solver", ["circulant", "lu"])
 @pytest.mark.parametrize("nn,kinds", CASES)
-def test_directional_ops_f64_and_r32(dev, oracle_mod, nn, kinds):
-    _say("case", nn, kinds)
+def test_directional_ops_f64_and_r32(dev, oracle_mod, nn, kinds, solver):
+    if solver == "lu" and "PERI" not in kinds:
+        pytest.skip("the periodic solver choice only affects periodic lines")
+    _say("case", nn, kinds, solver)
     ctx, P, v = _setup(nn, kinds, oracle_mod)
+    ctx.set_periodic_solver(solver)
     vd = to_dev(v, dev)
     v32 = v.astype(np.float32).astype(np.float64)
     vd32 = to_dev(v32, dev, torch.float32)

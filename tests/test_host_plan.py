@@ -4,9 +4,10 @@
 * its restated weight tables equal the oracle's (which are pinned to the
   reference build);
 * its mesh scalars/coordinates follow parcop.setup/mesh.f90;
-* the kernels' per-thread arithmetic (chunked line solve, run on the host by
-  pyr_host_emulate) matches the oracle: <=1e-12 relative in FP64 and <=1e-6 in
-  the FP32-refined mode, on periodic, bounded, symmetric and mixed grids.
+* the kernels' per-thread arithmetic (chunked LU and circulant line solves,
+  run on the host by pyr_host_emulate) matches the oracle: <=1e-12 relative in
+  FP64 and <=1e-6 in the FP32-refined mode, on periodic, bounded, symmetric
+  and mixed grids.
 """
 import ctypes
 import os
@@ -113,11 +114,16 @@ CASES = [
     # shortest lines the reference schemes define (r4 boundary rows read up to
     # row 8, compact_r4.f90:117-120) and a null (n<4) direction
     ((8, 8, 3), ("PERI", "NONE", "NONE")),
+    # P = 16 chunks per periodic line
+    ((512, 9, 8), ("PERI", "PERI", "NONE")),
 ]
 
 
+@pytest.mark.parametrize("solver", ["circulant", "lu"])
 @pytest.mark.parametrize("nn,kinds", CASES)
-def test_host_emulation_matches_oracle(amd, oracle_mod, nn, kinds):
+def test_host_emulation_matches_oracle(amd, oracle_mod, nn, kinds, solver):
+    if solver == "lu" and "PERI" not in kinds:
+        pytest.skip("the periodic solver choice only affects periodic lines")
     po = oracle_mod
     periodic = [k == "PERI" for k in kinds]
     sym = [[k == "SYMM", k == "SYMM"] for k in kinds]
@@ -127,6 +133,8 @@ def test_host_emulation_matches_oracle(amd, oracle_mod, nn, kinds):
     for k in kinds:
         bcs += [k, k]
     h = _host_ctx(amd, dom, bcs)
+    L = amd.lib()
+    amd.check(L.pyr_set_periodic_solver(h, amd.PSOLVE[solver]))
     P = po.Oracle(nn, dd, periodic, symmetric=sym)
     rng = np.random.default_rng(sum(nn))
     v = np.asfortranarray(np.sin(X) * np.cos(2 * Y) + 0.3 * np.sin(3 * Z + 1)
