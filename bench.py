@@ -177,6 +177,42 @@ def ops_bench(n, steps, warmup, dev, only=None, solver="circulant"):
                       (OPS[op] + "_" + DIRS[d], name, us, gbs, 100 * gbs / HBM_PEAK_GBS, err,
                        " same-input %.2e" % rec["rel_max_err_same_input"] if name == "r32" else ""),
                       file=sys.stderr, flush=True)
+    # composites of the Taylor-Green step: ring (three d8 sweeps with |.| and
+    # max fused into the stores), laplacian (d2 sweeps accumulated in place),
+    # fbar (x, y, z filters). Bytes: one read and one write per directional sweep.
+    exact = {}
+    sym = [_symbols(np.abs(ks[:, d]) * h) for d in range(3)]
+    exact["lap"] = sum(fourier_field(n, dev, (ks, amps, phis), weights=sym[d][1] / h ** 2) for d in range(3))
+    exact["fbar3"] = fourier_field(n, dev, (ks, amps, phis), weights=sym[0][3] * sym[1][3] * sym[2][3])
+    d8 = [fourier_field(n, dev, (ks, amps, phis), weights=sym[d][2]).abs() * h * h for d in range(3)]
+    exact["ring"] = torch.maximum(torch.maximum(d8[0], d8[1]), d8[2])
+    del d8
+    comps = {"ring": ctx.pring, "lap": ctx.plaplacian, "fbar3": ctx.sfilter}
+    for cname, fn in comps.items():
+        for name, f, bpp in (("f64", f64, 16), ("r32", f32, 8)):
+            key = "%s_%s" % (cname, name)
+            if not wanted(key):
+                continue
+            o = fn(f)
+            for _ in range(warmup):
+                o = fn(f)
+            torch.cuda.synchronize()
+            e0 = torch.cuda.Event(enable_timing=True)
+            e1 = torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(reps):
+                o = fn(f)
+            e1.record()
+            torch.cuda.synchronize()
+            us = e0.elapsed_time(e1) * 1e3 / reps
+            ex = exact[cname]
+            err = ((o.double() - ex).abs().max() / ex.abs().max()).item()
+            gbs = 3 * n ** 3 * bpp / (us * 1e-6) / 1e9
+            out[key] = {"us": round(us, 2), "GB_s": round(gbs, 1), "hbm_frac": round(gbs / HBM_PEAK_GBS, 3),
+                        "rel_max_err": err}
+            print("[ops] %-10s %-4s %8.1f us %7.1f GB/s (%.1f%%) err %.2e" %
+                  (cname, name, us, gbs, 100 * gbs / HBM_PEAK_GBS, err), file=sys.stderr, flush=True)
+    del exact
     best = max((kv for kv in out.items() if not kv[0].startswith("copy")),
                key=lambda kv: kv[1]["GB_s"], default=(None, None))
     print(json.dumps({"metric": "compact operator throughput, 256^3 periodic Fourier field",
@@ -223,7 +259,8 @@ def main():
     results = {}
     states = {}
     dt = None
-    for precision in (["fp32"] if args.no_fp64 else ["fp64", "fp32"]):
+    precs = ["fp32", "fp32-fp64solve"] if args.no_fp64 else ["fp64", "fp32", "fp32-fp64solve"]
+    for precision in precs:
         ss = make_tg(n, precision, p0, args.solver)
         if dt is None:
             dt = float(ss.variables["dt"].data) * 0.5  # CFL 0.5 from the initial state
@@ -258,10 +295,11 @@ def main():
     achieved = alg_bytes / (mean_ms * 1e-3) / 1e9
     op_share = sum(tot.values()) / (head["elapsed_s"] * 1e3)
 
-    acc = {}
+    acc, acc_s = {}, {}
     if "fp64" in states:
         for k in states["fp32"]:
             acc[k] = rel_err(states["fp32"][k], states["fp64"][k])
+            acc_s[k] = rel_err(states["fp32-fp64solve"][k], states["fp64"][k])
 
     cpu = None
     if not args.no_cpu:
@@ -292,6 +330,14 @@ def main():
         "fp64": ({"value": results["fp64"]["pups"], "ms_per_step": results["fp64"]["ms_per_step"]}
                  if "fp64" in results else None),
         "speedup_fp32r_over_fp64": (head["pups"] / results["fp64"]["pups"]) if "fp64" in results else None,
+        # FP32 fields with FP64 solves (pyrandaSim precision "fp32-fp64solve")
+        "fp32_fp64solve": {
+            "value": results["fp32-fp64solve"]["pups"],
+            "ms_per_step": results["fp32-fp64solve"]["ms_per_step"],
+            "speedup_over_fp64": (results["fp32-fp64solve"]["pups"] / results["fp64"]["pups"])
+            if "fp64" in results else None,
+            "accuracy_vs_fp64": {k: {"rel_max": a, "rel_l2": b} for k, (a, b) in acc_s.items()},
+        },
         "accuracy_fp32r_vs_fp64": {k: {"rel_max": a, "rel_l2": b} for k, (a, b) in acc.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,

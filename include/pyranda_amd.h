@@ -66,6 +66,15 @@ int pyr_set_stream(pyr_ctx* ctx, void* stream);
  * the same solution to rounding. */
 enum { PYR_PSOLVE_CIRCULANT = 0, PYR_PSOLVE_LU = 1 };
 int pyr_set_periodic_solver(pyr_ctx* ctx, int mode);
+
+/* Arithmetic of the solves on FP32 fields (PYR_R32 entry points and
+ * pyr_apply with PYR_R32). No counterpart in the reference (FP64 only):
+ * PYR_R32_SOLVE_REFINED (default) solves in FP32 and applies one FP64-residual
+ * refinement; PYR_R32_SOLVE_FP64 forms the RHS and solves in FP64 registers
+ * and rounds once to FP32 (on gfx950 FP32 and FP64 vector FMAs issue at the
+ * same rate, so this is the cheaper of the two). */
+enum { PYR_R32_SOLVE_REFINED = 0, PYR_R32_SOLVE_FP64 = 1 };
+int pyr_set_r32_solve(pyr_ctx* ctx, int mode);
 const char* pyr_last_error(void);
 
 /* --- mesh (parcop.f90 xgrid/ygrid/zgrid/dxgrid/mesh_getgridlen; mesh.f90:173-212)
@@ -116,9 +125,10 @@ int pyr_apply(pyr_ctx* ctx, int op, int dir, int prec, const void* in, void* out
               double epi_scale);
 
 /* --- one RK4 stage update of a conserved variable (pyranda.py:800-804) ----
- * PHI <- dt*F + A*PHI (in place, FP64), then
+ * PHI <- dt*F + A*PHI (in place, FP64; with A == 0 PHI is only written), then
  *   PYR_F64: U_out <- U_in + B*PHI                (double fields)
- *   PYR_R32: U64 <- U64 + B*PHI (FP64 state), U_out <- (float)U64
+ *   PYR_R32: U64 <- U64 + B*PHI (FP64 state), U_out <- (float)U64; with
+ *            U64 == NULL: U_out <- (float)((double)U_in + B*PHI)
  * F, U_in, U_out have the field precision; n = number of points. */
 int pyr_rk_update(pyr_ctx* ctx, int prec, long long n, const void* F, double* PHI,
                   const void* U_in, void* U_out, double* U64, double dt, double A, double B);
@@ -131,8 +141,14 @@ int pyr_rk_update(pyr_ctx* ctx, int prec, long long n, const void* F, double* PH
 #define PYR_PW_MAX_SCALARS 16
 int pyr_pw_compile(const char* source, const char* kernel_name, void** fn);
 int pyr_pw_check_source(const char* source, const char* kernel_name);
+/* Workgroups of PYR_PW_BLOCK threads; pyr_pw_launch picks the grid,
+ * pyr_pw_launch_blocks takes it (reduction kernels write one partial per
+ * workgroup). */
+#define PYR_PW_BLOCK 256
 int pyr_pw_launch(void* fn, void* stream, long long n, int nptr, void* const* ptrs, int nscal,
                   const double* scal);
+int pyr_pw_launch_blocks(void* fn, void* stream, long long n, int blocks, int nptr,
+                         void* const* ptrs, int nscal, const double* scal);
 const char* pyr_pw_last_error(void);
 
 /* --- host-side checks (no GPU needed) ----------------------------------- */

@@ -14,16 +14,18 @@ FLAGS = [
 ]
 
 
-def build(verbose=False):
-    tmp = OUT + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-o", tmp] + SRCS + ["-lhiprtc"]
+def build(verbose=False, out=OUT, defines=()):
+    """Compile the library; `defines` (e.g. ["PYR_PREFETCH=0"]) set the tuning
+    knobs of pyr_core.hip for a variant written to `out`."""
+    tmp = out + ".tmp"
+    cmd = [HIPCC] + FLAGS + ["-D" + d for d in defines] + ["-o", tmp] + SRCS + ["-lhiprtc"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
-    os.replace(tmp, OUT)
+    os.replace(tmp, out)
     if verbose:
-        print("built", OUT)
-    return OUT
+        print("built", out)
+    return out
 
 
 if __name__ == "__main__":

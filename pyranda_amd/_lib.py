@@ -7,13 +7,16 @@ import ctypes
 import os
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libpyranda_amd.so")
+# PYRANDA_AMD_LIB selects another in-tree build of the same library (tuning
+# variants built by _build.build(out=..., defines=...))
+LIB_PATH = os.environ.get("PYRANDA_AMD_LIB") or os.path.join(HERE, "libpyranda_amd.so")
 
 BC = {"PERI": 0, "NONE": 1, "SYMM": 2}
 OP_D1, OP_D2, OP_D8, OP_SFILTER, OP_GFILTER = 0, 1, 2, 3, 4
 EPI_STORE, EPI_ADD, EPI_ABS_SCALED, EPI_MAXABS_SCALED = 0, 1, 2, 3
 F64, R32 = 0, 1
 PSOLVE = {"circulant": 0, "lu": 1}
+R32_SOLVE = {"refined": 0, "fp64": 1}
 
 _vp = ctypes.c_void_p
 _dp = ctypes.POINTER(ctypes.c_double)
@@ -38,7 +41,10 @@ SIGNATURES = {
     "pyr_pw_check_source": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p]),
     "pyr_pw_launch": (ctypes.c_int, [_vp, _vp, ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(_vp),
                                      ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
+    "pyr_pw_launch_blocks": (ctypes.c_int, [_vp, _vp, ctypes.c_longlong, ctypes.c_int, ctypes.c_int,
+                                            ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "pyr_pw_last_error": (ctypes.c_char_p, []),
+    "pyr_set_r32_solve": (ctypes.c_int, [_vp, ctypes.c_int]),
 }
 PW_MAX_PTRS, PW_MAX_SCALARS = 24, 16
 for _prec in ("f64", "r32"):

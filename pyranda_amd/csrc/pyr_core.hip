@@ -44,7 +44,7 @@ static int fail(const std::string& m) {
 // kernels
 // ---------------------------------------------------------------------------
 template <int MODE> struct SolveT { using type = double; };
-template <> struct SolveT<MODE_R32> { using type = float; };
+template <> struct SolveT<MODE_R32> { using type = float; };  // MODE_S32 solves in double
 
 constexpr int WG_MAX = 512;
 
@@ -65,14 +65,14 @@ __host__ __device__ inline size_t align16(size_t b) { return (b + 15) & ~(size_t
 // bytes of the staged tables: FP64 lu|rc|gsel, or FP32 lu|rc|gsel + FP64 band
 template <int MODE>
 __host__ __device__ inline size_t stage_bytes(int ntab, int N) {
-  return MODE == MODE_F64 ? align16((size_t)ntab * sizeof(double))
+  return MODE != MODE_R32 ? align16((size_t)ntab * sizeof(double))
                           : align16((size_t)ntab * sizeof(float)) + align16((size_t)5 * N * sizeof(double));
 }
 
 template <int MODE>
 __device__ __forceinline__ Tabs stage_tables(const LineArgs& a, unsigned char* dst, int tid, int nthr) {
   Tabs t;
-  if (MODE == MODE_F64) {
+  if (MODE != MODE_R32) {
     double* d = reinterpret_cast<double*>(dst);
     for (int e = tid; e < a.ntab; e += nthr) d[e] = a.tab64[e];
     t.lu = d;
@@ -100,7 +100,7 @@ template <int MODE, typename CS>
 __device__ __forceinline__ void solve_chunk_lu(const LineArgs& a, const Tabs& t, int p, int s, int m,
                                                CS* Y, int ly, int col, double* rhs) {
   auto yr = [&](int c) { return Y[c * ly + col]; };
-  if (MODE == MODE_F64) {
+  if (MODE != MODE_R32) {
     double* x = rhs;  // solve in place
     local_solve<double>(t.lu, s, m, x);
     double e4[4];
@@ -172,7 +172,7 @@ template <int MODE, typename CS, typename Emit>
 __device__ __forceinline__ void solve_chunk_circ(const LineArgs& a, int p, int m, CS* X0, CS* X1,
                                                  int ly, int col, double* rhs, Emit emit) {
   const int P = a.P;
-  if constexpr (MODE == MODE_F64) {
+  if constexpr (MODE != MODE_R32) {
     circ_solve<double>(a.kf, a.kb, a.p1, a.p2, p, P, m, X0, X1, ly, col, rhs,
                        [&](int k, double v) { emit(k, v * a.invc); });
   } else {
@@ -202,11 +202,55 @@ __device__ __forceinline__ void solve_chunk_circ(const LineArgs& a, int p, int m
 
 // y/z lines. block = (LY lanes, P chunks). With LY = 64 the chunk index is
 // wave-uniform; with LY = 32 a wave holds two chunks of 32 lines each.
+// What the epilogue of a y/z row needs besides the solution, fetched into
+// registers before the solve so that no store waits on a load: the input
+// value (filters add v back) or the current output (accumulating epilogues).
+// PRE_LATE loads them in the epilogue instead (fewer registers, a wait per row).
+enum Pre { PRE_NONE = 0, PRE_V = 1, PRE_OLD = 2, PRE_LATE = 3 };
+
+// Build-time tuning knobs (register pressure against latency hiding):
+//   PYR_FULL_CHUNKS  compile-time row counts when CHUNK divides N
+//   PYR_PREFETCH     PRE_V / PRE_OLD (1) or PRE_LATE (0)
+//   PYR_YZ_WAVES_PER_EU  minimum waves per SIMD the y/z kernels are built for
+#ifndef PYR_FULL_CHUNKS
+#define PYR_FULL_CHUNKS 1
+#endif
+#ifndef PYR_PREFETCH
+#define PYR_PREFETCH 1
+#endif
+#ifndef PYR_YZ_WAVES_PER_EU
+#define PYR_YZ_WAVES_PER_EU 1
+#endif
+
+// Raw buffer access to one field: a wave's row of 64 adjacent lines is one
+// load/store whose lane part sits in a VGPR and whose row part (uniform across
+// the wave) sits in an SGPR, so no per-row 64-bit address is formed. Fields
+// are below 2 GiB (at most 512^3 FP64). dword3 0x00020000: 32-bit data format,
+// the raw (untyped) configuration of gfx9-family buffer descriptors.
+constexpr int BUF_DWORD3 = 0x00020000;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buf_rsrc(const void* p, int nbytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, nbytes, BUF_DWORD3);
+}
+__device__ __forceinline__ double buf_load(double, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ float buf_load(float, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+}
+__device__ __forceinline__ void buf_store(double v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  using U2 = unsigned int __attribute__((ext_vector_type(2)));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, voff, soff, 0);
+}
+__device__ __forceinline__ void buf_store(float v, __amdgpu_buffer_rsrc_t r, int voff, int soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, 0);
+}
+
 // FULL: every chunk has CHUNK rows (N a multiple of CHUNK), so the row
 // predicates vanish at compile time. The circulant solve implies periodic lines.
-template <typename TS, int KIND, int SOLVE, int MODE, int LY, bool FULL>
-__global__ void __launch_bounds__(WG_MAX) k_line_yz(LineArgs a, const TS* __restrict__ in,
-                                                    TS* __restrict__ out) {
+template <typename TS, int KIND, int SOLVE, int MODE, int LY, bool FULL, int PRE>
+__global__ void __launch_bounds__(WG_MAX) __attribute__((amdgpu_waves_per_eu(PYR_YZ_WAVES_PER_EU, 8)))
+k_line_yz(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
   constexpr int R = Halo<KIND>::R;
   constexpr bool PER = SOLVE == SOLVE_CIRC;
   using CS = typename SolveT<MODE>::type;
@@ -225,7 +269,23 @@ __global__ void __launch_bounds__(WG_MAX) k_line_yz(LineArgs a, const TS* __rest
   const long long base = (Lc % a.A) + a.B * (Lc / a.A);
   const int s = FULL ? p * CHUNK : a.cstart[p];
   const int m = FULL ? CHUNK : a.cstart[p + 1] - s;
-  const TS* vin = in + base;
+  const __amdgpu_buffer_rsrc_t rin = buf_rsrc(in, a.nbytes), rout = buf_rsrc(out, a.nbytes);
+  const int lane_off = (int)(base * (long long)sizeof(TS));
+  const int row_bytes = (int)(a.stride * (long long)sizeof(TS));
+  // row `row` of this line; the row offset is wave-uniform when LY == 64
+  auto ld = [&](__amdgpu_buffer_rsrc_t r, int row) -> TS {
+    if constexpr (LY == 64) return buf_load(TS(), r, lane_off, row * row_bytes);
+    else return buf_load(TS(), r, lane_off + row * row_bytes, 0);
+  };
+  auto st = [&](TS v, int row) {
+    if constexpr (LY == 64) buf_store(v, rout, lane_off, row * row_bytes);
+    else buf_store(v, rout, lane_off + row * row_bytes, 0);
+  };
+  TS pre[CHUNK];
+  if constexpr (PRE == PRE_OLD) {
+#pragma unroll
+    for (int k = 0; k < CHUNK; ++k) pre[k] = k < m ? ld(rout, s + k) : (TS)0;
+  }
 
   TS win[CHUNK + 2 * R];
 #pragma unroll
@@ -235,26 +295,34 @@ __global__ void __launch_bounds__(WG_MAX) k_line_yz(LineArgs a, const TS* __rest
     if (w < m + 2 * R) {
       if (PER || a.periodic) {
         row = row < 0 ? row + a.N : (row >= a.N ? row - a.N : row);
-        v = vin[a.stride * row];
+        v = ld(rin, row);
       } else if (row >= 0 && row < a.N) {
-        v = vin[a.stride * row];
+        v = ld(rin, row);
       }
     }
     win[w] = v;
   }
+  if constexpr (PRE == PRE_V) {
+#pragma unroll
+    for (int k = 0; k < CHUNK; ++k) pre[k] = win[k + R];
+  }
   double rhs[CHUNK];
   rhs_interior<KIND>(a, win, rhs, m);
-  if (!PER) rhs_boundary(a, s, m, rhs, [&](int i) { return vin[a.stride * i]; });
-  TS* vout = out + base;
-  // row k of the result: metric scale, filters add v back (re-read, cached,
-  // rather than kept live), fused epilogue, store
+  if (!PER) rhs_boundary(a, s, m, rhs, [&](int i) { return ld(rin, i); });
+  // row k of the result: metric scale, filters add v back, fused epilogue, store
   auto emit = [&](int k, double y) {
     if (!valid) return;
     double val = y * a.scale;
-    if (a.addv) val += (double)vin[a.stride * (s + k)];
-    TS* dst = vout + a.stride * (s + k);
-    const TS old = (a.epi == EPI_ADD || a.epi == EPI_MAXABS) ? *dst : (TS)0;
-    *dst = epilogue<TS>(a.epi, a.epi_c, val, old);
+    if constexpr (PRE == PRE_V) val += (double)pre[k];
+    if constexpr (PRE == PRE_LATE)
+      if (a.addv) val += (double)ld(rin, s + k);
+    TS old = (TS)0;
+    if constexpr (PRE == PRE_OLD) old = pre[k];
+    if constexpr (PRE == PRE_LATE) old = (a.epi == EPI_ADD || a.epi == EPI_MAXABS) ? ld(rout, s + k) : (TS)0;
+    // a filter with an accumulating epilogue (only through pyr_apply) reads
+    // the output in place
+    if constexpr (PRE == PRE_V) old = (a.epi == EPI_ADD || a.epi == EPI_MAXABS) ? ld(rout, s + k) : (TS)0;
+    st(epilogue<TS>(a.epi, a.epi_c, val, old), s + k);
   };
   if (SOLVE == SOLVE_CIRC) {
     solve_chunk_circ<MODE, CS>(a, p, m, Y, Y + 4 * a.P * LY, LY, lane, rhs, emit);
@@ -423,14 +491,16 @@ __global__ void k_null(long long n, int copy, int epi, double c, const TS* __res
   out[i] = epilogue<TS>(epi, c, val, old);
 }
 
-// RK4 stage update (pyranda.py:800-804), one pass over F, PHI and U
+// RK4 stage update (pyranda.py:800-804), one pass over F, PHI and U. With
+// A == 0 (the first stage) A*PHI is zero and PHI is not read, so it needs no
+// clearing before the step.
 template <typename TS>
 __global__ void k_rk_update(long long n, const TS* __restrict__ F, double* __restrict__ PHI,
                             const TS* __restrict__ Uin, TS* __restrict__ Uout,
                             double* __restrict__ U64, double dt, double A, double B) {
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
-    const double tmp1 = A * PHI[i];
+    const double tmp1 = A == 0.0 ? 0.0 : A * PHI[i];
     const double phi = dt * (double)F[i] + tmp1;
     PHI[i] = phi;
     const double tmp2 = B * phi;
@@ -472,6 +542,7 @@ struct pyr_ctx {
   DevPlan dplan[NOPS][3];
   void* scratch[2] = {nullptr, nullptr};  // one field-sized temp per precision
   int periodic_solver = PYR_PSOLVE_CIRCULANT;
+  int r32_solve = PYR_R32_SOLVE_REFINED;
   int x_lds_kb = 40;                        // LDS budget of one x-line workgroup
   size_t npts() const { return (size_t)n[0] * n[1] * n[2]; }
 };
@@ -577,7 +648,7 @@ static void host_tables(const OpPlan& pl, DevPlan& d, std::vector<float>& lu32,
 }
 
 static LineArgs make_args(const pyr_ctx* c, const OpPlan& pl, const DevPlan& d, int dir, int epi,
-                          double ec) {
+                          double ec, int prec) {
   LineArgs a;
   std::memset(&a, 0, sizeof(a));
   const long long nx = c->n[0], ny = c->n[1], nz = c->n[2];
@@ -594,6 +665,7 @@ static LineArgs make_args(const pyr_ctx* c, const OpPlan& pl, const DevPlan& d, 
     a.nlines = (int)(nx * ny);
     a.A = nx * ny; a.B = 0; a.stride = nx * ny;
   }
+  a.nbytes = (int)(c->npts() * (prec == PYR_F64 ? sizeof(double) : sizeof(float)));
   a.addv = pl.addv;
   a.epi = epi;
   a.scale = pl.scale;
@@ -624,6 +696,26 @@ template <typename K>
 static int set_lds(K kernel, size_t bytes) {
   if (bytes > 64 * 1024)
     HIPCHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+  return 0;
+}
+
+template <typename TS, int KIND, int SOLVE, int MODE, int LY, bool FULL>
+static int launch_yz(pyr_ctx* c, int pre, dim3 grid, dim3 block, size_t shm, const LineArgs& a,
+                     const TS* in, TS* out) {
+  auto k = k_line_yz<TS, KIND, SOLVE, MODE, LY, FULL, PRE_NONE>;
+#if PYR_PREFETCH
+  // filters only exist for the r4 kind; accumulating epilogues for every kind
+  if (pre == PRE_V) {
+    if constexpr (KIND == K_R4) k = k_line_yz<TS, KIND, SOLVE, MODE, LY, FULL, PRE_V>;
+    else return fail("filter epilogue on a non-filter kernel");
+  } else if (pre == PRE_OLD) {
+    k = k_line_yz<TS, KIND, SOLVE, MODE, LY, FULL, PRE_OLD>;
+  }
+#else
+  if (pre != PRE_NONE) k = k_line_yz<TS, KIND, SOLVE, MODE, LY, FULL, PRE_LATE>;
+#endif
+  if (set_lds(k, shm)) return 1;
+  hipLaunchKernelGGL(k, grid, block, shm, c->stream, a, in, out);
   return 0;
 }
 
@@ -660,14 +752,11 @@ static int launch_typed(pyr_ctx* c, int dir, LineArgs a, const TS* in, TS* out) 
     const size_t ex = align16((size_t)4 * P * ly * sizeof(CS));
     const size_t shm = SOLVE == SOLVE_CIRC ? 2 * ex : (SOLVE == SOLVE_LU ? ex + stage : 0);
     const dim3 grid((a.nlines + ly - 1) / ly), block(ly, P);
+    const int pre = a.addv ? PRE_V : ((a.epi == EPI_ADD || a.epi == EPI_MAXABS) ? PRE_OLD : PRE_NONE);
     if (ly == 64) {
-      auto k = k_line_yz<TS, KIND, SOLVE, MODE, 64, FULL>;
-      if (set_lds(k, shm)) return 1;
-      hipLaunchKernelGGL(k, grid, block, shm, c->stream, a, in, out);
+      if (launch_yz<TS, KIND, SOLVE, MODE, 64, FULL>(c, pre, grid, block, shm, a, in, out)) return 1;
     } else {
-      auto k = k_line_yz<TS, KIND, SOLVE, MODE, 32, FULL>;
-      if (set_lds(k, shm)) return 1;
-      hipLaunchKernelGGL(k, grid, block, shm, c->stream, a, in, out);
+      if (launch_yz<TS, KIND, SOLVE, MODE, 32, FULL>(c, pre, grid, block, shm, a, in, out)) return 1;
     }
   }
   HIPCHECK(hipGetLastError());
@@ -676,7 +765,7 @@ static int launch_typed(pyr_ctx* c, int dir, LineArgs a, const TS* in, TS* out) 
 
 template <typename TS, int MODE, bool FULL>
 static int launch_kind(pyr_ctx* c, int dir, int kind, int sv, const LineArgs& a, const TS* in, TS* out) {
-  if constexpr (FULL) {
+  if constexpr (FULL && PYR_FULL_CHUNKS) {
     switch (kind) {
       case K_D1: return launch_typed<TS, K_D1, SOLVE_CIRC, MODE, true>(c, dir, a, in, out);
       case K_R3: return launch_typed<TS, K_R3, SOLVE_CIRC, MODE, true>(c, dir, a, in, out);
@@ -684,7 +773,7 @@ static int launch_kind(pyr_ctx* c, int dir, int kind, int sv, const LineArgs& a,
         if (sv == SOLVE_CIRC) return launch_typed<TS, K_R4, SOLVE_CIRC, MODE, true>(c, dir, a, in, out);
         return launch_typed<TS, K_R4, SOLVE_NONE, MODE, true>(c, dir, a, in, out);
     }
-  } else {
+  } else if constexpr (!FULL) {
     switch (kind) {
       case K_D1:
         return sv == SOLVE_CIRC ? launch_typed<TS, K_D1, SOLVE_CIRC, MODE, false>(c, dir, a, in, out)
@@ -698,6 +787,7 @@ static int launch_kind(pyr_ctx* c, int dir, int kind, int sv, const LineArgs& a,
         return launch_typed<TS, K_R4, SOLVE_NONE, MODE, false>(c, dir, a, in, out);
     }
   }
+  return fail("full-chunk kernels are not built");
 }
 
 template <typename TS, int MODE>
@@ -710,12 +800,13 @@ static int launch_op(pyr_ctx* c, int op, int dir, const TS* in, TS* out, int epi
     HIPCHECK(hipGetLastError());
     return 0;
   }
-  const LineArgs a = make_args(c, pl, c->dplan[op][dir], dir, epi, ec);
+  const LineArgs a = make_args(c, pl, c->dplan[op][dir], dir, epi, ec, MODE == MODE_F64 ? PYR_F64 : PYR_R32);
   const int sv = plan_solve(c, pl);
   // the balanced partition gives every chunk CHUNK rows exactly when CHUNK | N;
   // the LU solve keeps runtime row counts (with compile-time counts the
   // compiler hoists every table read of the chunk and runs out of registers)
-  if (pl.N % CHUNK == 0 && sv != SOLVE_LU) return launch_kind<TS, MODE, true>(c, dir, pl.kind, sv, a, in, out);
+  if (PYR_FULL_CHUNKS && pl.N % CHUNK == 0 && sv != SOLVE_LU)
+    return launch_kind<TS, MODE, true>(c, dir, pl.kind, sv, a, in, out);
   return launch_kind<TS, MODE, false>(c, dir, pl.kind, sv, a, in, out);
 }
 
@@ -733,8 +824,11 @@ static int apply_any(pyr_ctx* c, int op, int dir, int prec, const void* in, void
   if (in == out) return fail("input and output must not alias");
   if (prec == PYR_F64)
     return launch_op<double, MODE_F64>(c, op, dir, (const double*)in, (double*)out, epi, ec);
-  if (prec == PYR_R32)
+  if (prec == PYR_R32) {
+    if (c->r32_solve == PYR_R32_SOLVE_FP64)
+      return launch_op<float, MODE_S32>(c, op, dir, (const float*)in, (float*)out, epi, ec);
     return launch_op<float, MODE_R32>(c, op, dir, (const float*)in, (float*)out, epi, ec);
+  }
   return fail("bad precision");
 }
 
@@ -779,6 +873,8 @@ static int setup_common(pyr_ctx** out, int nx, int ny, int nz, double x1, double
   if (!out) return fail("null output handle");
   *out = nullptr;
   if (nx < 1 || ny < 1 || nz < 1) return fail("grid sizes must be >= 1");
+  // the y/z kernels address a field through a 32-bit buffer range
+  if ((double)nx * ny * nz * sizeof(double) > 2147483647.0) return fail("field exceeds 2 GiB");
   if (!bc6) return fail("null bc array");
   std::unique_ptr<pyr_ctx> c(new pyr_ctx);
   c->n[0] = nx; c->n[1] = ny; c->n[2] = nz;
@@ -878,9 +974,9 @@ static void emulate_line(const LineArgs& a, const TS* vin, long long stride, TS*
     }
   };
   if (SOLVE == SOLVE_LU) {
-    const CS* G = MODE == MODE_F64 ? (const CS*)a.gsel : (const CS*)a.gsel32;
-    const CS* RC = MODE == MODE_F64 ? (const CS*)a.rc : (const CS*)a.rc32;
-    const CS* LU = MODE == MODE_F64 ? (const CS*)a.lu : (const CS*)a.lu32;
+    const CS* G = MODE != MODE_R32 ? (const CS*)a.gsel : (const CS*)a.gsel32;
+    const CS* RC = MODE != MODE_R32 ? (const CS*)a.rc : (const CS*)a.rc32;
+    const CS* LU = MODE != MODE_R32 ? (const CS*)a.lu : (const CS*)a.lu32;
     for (int p = 0; p < P; ++p) {
       St& t = st[p];
       for (int k = 0; k < CHUNK; ++k) t.x[k] = (CS)t.rhs[k];
@@ -917,7 +1013,7 @@ static void emulate_line(const LineArgs& a, const TS* vin, long long stride, TS*
     for (int p = 0; p < P; ++p)
       for (int k = 0; k < CHUNK; ++k) st[p].x[k] = (CS)st[p].rhs[k];
     auto gx = [&](int p) { return st[p].x; };
-    if (MODE == MODE_F64) {
+    if (MODE != MODE_R32) {
       circ_all(gx, (const CS*)a.kf, (const CS*)a.kb, (CS)a.p1, (CS)a.p2);
       for (int p = 0; p < P; ++p)
         for (int k = 0; k < CHUNK; ++k) st[p].x[k] *= (CS)a.invc;
@@ -949,7 +1045,7 @@ static void emulate_line(const LineArgs& a, const TS* vin, long long stride, TS*
     for (int k = 0; k < t.m; ++k) {
       double y;
       if (SOLVE == SOLVE_NONE) y = t.rhs[k];
-      else if (MODE == MODE_F64) y = (double)t.x[k];
+      else if (MODE != MODE_R32) y = (double)t.x[k];
       else y = (double)t.x[k] + (double)t.d[k];
       double val = y * a.scale;
       if (a.addv) val += (double)t.win[k + R];
@@ -969,7 +1065,7 @@ static int emulate(pyr_ctx* c, int op, int dir, const TS* in, TS* out) {
   DevPlan d;
   std::vector<float> lu32, rc32, gs32, kf32, kb32;
   host_tables(pl, d, lu32, rc32, gs32, kf32, kb32);
-  const LineArgs a = make_args(c, pl, d, dir, EPI_STORE, 0.0);
+  const LineArgs a = make_args(c, pl, d, dir, EPI_STORE, 0.0, MODE == MODE_F64 ? PYR_F64 : PYR_R32);
   const int sv = plan_solve(c, pl);
   for (long long L = 0; L < a.nlines; ++L) {
     const long long base = (dir == 0) ? L * pl.N : (L % a.A) + a.B * (L / a.A);
@@ -1026,6 +1122,13 @@ int pyr_destroy(pyr_ctx* c) {
 int pyr_set_stream(pyr_ctx* c, void* stream) {
   if (!c) return fail("null context");
   c->stream = (hipStream_t)stream;
+  return 0;
+}
+
+int pyr_set_r32_solve(pyr_ctx* c, int mode) {
+  if (!c) return fail("null context");
+  if (mode != PYR_R32_SOLVE_REFINED && mode != PYR_R32_SOLVE_FP64) return fail("bad r32 solve mode");
+  c->r32_solve = mode;
   return 0;
 }
 
@@ -1164,7 +1267,10 @@ int pyr_host_emulate(pyr_ctx* c, int op, int dir, int prec, const void* in, void
   if (!c) return fail("null context");
   if (op < 0 || op >= NOPS || dir < 0 || dir > 2) return fail("bad operator or direction");
   if (prec == PYR_F64) return emulate<double, MODE_F64>(c, op, dir, (const double*)in, (double*)out);
-  if (prec == PYR_R32) return emulate<float, MODE_R32>(c, op, dir, (const float*)in, (float*)out);
+  if (prec == PYR_R32) {
+    if (c->r32_solve == PYR_R32_SOLVE_FP64) return emulate<float, MODE_S32>(c, op, dir, (const float*)in, (float*)out);
+    return emulate<float, MODE_R32>(c, op, dir, (const float*)in, (float*)out);
+  }
   return fail("bad precision");
 }
 

@@ -105,27 +105,37 @@ int pyr_pw_check_source(const char* source, const char* kernel_name) {
   return 0;
 }
 
-int pyr_pw_launch(void* fn, void* stream, long long n, int nptr, void* const* ptrs, int nscal,
-                  const double* scal) {
+static int pw_launch(void* fn, void* stream, long long n, long long blocks, int nptr,
+                     void* const* ptrs, int nscal, const double* scal) {
   if (!fn) return ffail("null kernel");
   if (nptr < 0 || nptr > PYR_PW_MAX_PTRS || nscal < 0 || nscal > PYR_PW_MAX_SCALARS)
     return ffail("too many kernel arguments");
   if (n <= 0) return 0;
+  if (blocks < 1 || blocks > 65535) return ffail("bad block count");
   PyrPwArgs a;
   std::memset(&a, 0, sizeof(a));
   a.n = n;
   for (int i = 0; i < nptr; ++i) a.p[i] = ptrs[i];
   for (int i = 0; i < nscal; ++i) a.s[i] = scal[i];
   void* params[] = {&a};
-  const unsigned block = 256;
-  // grid-stride loop: enough blocks to fill 256 CUs several times over
-  long long blocks = (n + block * 4 - 1) / (block * 4);
-  if (blocks > 8192) blocks = 8192;
-  if (blocks < 1) blocks = 1;
-  if (hipModuleLaunchKernel((hipFunction_t)fn, (unsigned)blocks, 1, 1, block, 1, 1, 0,
+  if (hipModuleLaunchKernel((hipFunction_t)fn, (unsigned)blocks, 1, 1, PYR_PW_BLOCK, 1, 1, 0,
                             (hipStream_t)stream, params, nullptr) != hipSuccess)
     return ffail("hipModuleLaunchKernel failed");
   return 0;
+}
+
+int pyr_pw_launch(void* fn, void* stream, long long n, int nptr, void* const* ptrs, int nscal,
+                  const double* scal) {
+  // grid-stride loop: enough blocks to fill 256 CUs several times over
+  long long blocks = (n + PYR_PW_BLOCK * 4 - 1) / (PYR_PW_BLOCK * 4);
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  return pw_launch(fn, stream, n, blocks, nptr, ptrs, nscal, scal);
+}
+
+int pyr_pw_launch_blocks(void* fn, void* stream, long long n, int blocks, int nptr,
+                         void* const* ptrs, int nscal, const double* scal) {
+  return pw_launch(fn, stream, n, blocks, nptr, ptrs, nscal, scal);
 }
 
 }  // extern "C"

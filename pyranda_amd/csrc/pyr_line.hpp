@@ -24,12 +24,15 @@
 
 namespace pyr {
 
-enum Mode { MODE_F64 = 0, MODE_R32 = 1 };
+// MODE_F64: FP64 fields and solve. MODE_R32: FP32 fields, FP32 solve + one
+// FP64-residual refinement. MODE_S32: FP32 fields, FP64 solve.
+enum Mode { MODE_F64 = 0, MODE_R32 = 1, MODE_S32 = 2 };
 enum Epi { EPI_STORE = 0, EPI_ADD = 1, EPI_ABS = 2, EPI_MAXABS = 3 };
 
 // Everything a kernel needs about one operator/direction, passed by value.
 struct LineArgs {
   int N, P, nlines, periodic;
+  int nbytes;              // bytes of one field (buffer range of the y/z kernels)
   long long A, B, stride;  // y/z: line L starts at (L % A) + B*(L / A), rows `stride` apart
   int lx;                  // x kernel: lines per workgroup
   int addv, epi;

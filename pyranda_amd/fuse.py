@@ -17,6 +17,7 @@ whole fields (slices, vectors) make that subtree evaluate with torch ops.
 """
 import ast
 import ctypes
+import ctypes
 import hashlib
 
 import torch
@@ -275,11 +276,26 @@ def _copy(n):
     return copy.deepcopy(n)
 
 
+class _DropAbsOfRing(ast.NodeTransformer):
+    """numpy.abs(self.ring(...)) -> self.ring(...): ring and ringV are maxima of
+    |d8|-scaled values (operators.f90:615-699), never negative, so the abs is
+    the identity and would cost one more pass over the field."""
+
+    def visit_Call(self, node):
+        self.generic_visit(node)
+        if (_is_numpy_call(node) and node.func.attr in ("abs", "absolute") and len(node.args) == 1
+                and isinstance(node.args[0], ast.Call) and isinstance(node.args[0].func, ast.Attribute)
+                and isinstance(node.args[0].func.value, ast.Name) and node.args[0].func.value.id == "self"
+                and node.args[0].func.attr in ("ring", "ringV")):
+            return node.args[0]
+        return node
+
+
 class FusedRHS:
     """Callable replacing `lambda self: <rhs>` with fused pointwise kernels."""
 
     def __init__(self, src, globals_):
-        tree = ast.parse(src, mode="eval")
+        tree = _DropAbsOfRing().visit(ast.parse(src, mode="eval"))
         f = _Fuser(src[:40])
         body = f.visit(tree.body)
         self.subtrees = f.subtrees
@@ -294,3 +310,309 @@ class FusedRHS:
 
     def __call__(self, sim):
         return self._fn(sim, tuple(sim.mesh.shape), self.subtrees)
+
+
+_RED = {
+    # identity, combine(acc, v); max/min propagate NaN as numpy's max/min do
+    "max": ("-__builtin_huge_val()", "((v > acc || v != v || acc != acc) ? (acc != acc ? acc : v) : acc)"),
+    "min": ("__builtin_huge_val()", "((v < acc || v != v || acc != acc) ? (acc != acc ? acc : v) : acc)"),
+}
+_RED_BLOCKS = 1024
+
+
+class FusedReduce:
+    """max/min over the grid of one pointwise expression of fields and
+    scalars, in one pass: each workgroup of PYR_PW_BLOCK threads reduces its
+    grid-stride share (wave shuffles, then LDS) to one FP64 partial, and the
+    host reduces the partials. `body` is C over x<i> (inputs, fields first,
+    then scalars) computed in FP64."""
+
+    def __init__(self, op, body, nfields, nscalars, tag=""):
+        self.op, self.body, self.nf, self.ns, self.tag = op, body, nfields, nscalars, tag
+        self._kernels = {}
+
+    def _source(self, sig):
+        ident, comb = _RED[self.op]
+        name = "pyr_red_" + hashlib.sha1((self.op + self.body + repr(sig)).encode()).hexdigest()[:16]
+        L = [PRELUDE, 'extern "C" __global__ void __launch_bounds__(256) %s(PyrPwArgs a) {' % name]
+        for i, t in enumerate(sig):
+            ct = "double" if t == "T64" else "float"
+            L.append("  const %s* __restrict__ p%d = (const %s*)a.p[%d];" % (ct, i, ct, i))
+        for j in range(self.ns):
+            L.append("  const double x%d = a.s[%d];" % (self.nf + j, j))
+        L.append("  double* __restrict__ part = (double*)a.p[%d];" % len(sig))
+        L.append("  double acc = %s;" % ident)
+        L.append("  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;"
+                 " i += (long long)gridDim.x * blockDim.x) {")
+        for i in range(len(sig)):
+            L.append("    const double x%d = (double)p%d[i];" % (i, i))
+        L.append("    const double v = (%s);" % self.body)
+        L.append("    acc = %s;" % comb)
+        L.append("  }")
+        L.append("  for (int off = 32; off > 0; off >>= 1) {")
+        L.append("    const double v = __shfl_xor(acc, off, 64);")
+        L.append("    acc = %s;" % comb)
+        L.append("  }")
+        L.append("  __shared__ double wsum[4];")
+        L.append("  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;")
+        L.append("  __syncthreads();")
+        L.append("  if (threadIdx.x == 0) {")
+        L.append("    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { const double v = wsum[w]; acc = %s; }" % comb)
+        L.append("    part[blockIdx.x] = acc;")
+        L.append("  }")
+        L.append("}")
+        return name, "\n".join(L)
+
+    def __call__(self, shape, fields, scalars):
+        sig = tuple("T64" if f.dtype == torch.float64 else "T32" for f in fields)
+        fn = self._kernels.get(sig)
+        if fn is None:
+            name, src = self._source(sig)
+            h = ctypes.c_void_p()
+            L = _lib.lib()
+            if L.pyr_pw_compile(src.encode(), name.encode(), ctypes.byref(h)) != 0:
+                raise _lib.PyrandaAMDError(L.pyr_pw_last_error().decode())
+            fn = self._kernels[sig] = h
+        dev = fields[0].device
+        n = shape[0] * shape[1] * shape[2]
+        blocks = max(1, min(_RED_BLOCKS, (n + 1023) // 1024))
+        part = torch.empty(blocks, dtype=torch.float64, device=dev)
+        ptrs = (ctypes.c_void_p * (len(fields) + 1))(*[f.data_ptr() for f in fields], part.data_ptr())
+        sc = (ctypes.c_double * max(1, len(scalars)))(*[float(x) for x in scalars])
+        stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        _lib.check_pw(_lib.lib().pyr_pw_launch_blocks(fn, stream, n, blocks, len(fields) + 1, ptrs,
+                                                      len(scalars), sc))
+        import numpy as _np
+        vals = part.cpu().numpy()
+        return float(_np.max(vals) if self.op == "max" else _np.min(vals))
+
+
+def fields_ok(shape, *vals):
+    """All values are FP32/FP64 device fields of `shape` in the field layout."""
+    return all(_field_ok(v, tuple(shape)) for v in vals)
+
+
+# --- runs of pointwise equations -------------------------------------------
+_VAR = None
+
+
+def _var_name(node):
+    """'X' for the AST of self.variables["X"].data, else None."""
+    if (isinstance(node, ast.Attribute) and node.attr == "data" and isinstance(node.value, ast.Subscript)
+            and isinstance(node.value.value, ast.Attribute) and node.value.value.attr == "variables"
+            and isinstance(node.value.value.value, ast.Name) and node.value.value.value.id == "self"
+            and isinstance(node.value.slice, ast.Constant) and isinstance(node.value.slice.value, str)):
+        return node.value.slice.value
+    return None
+
+
+def pure_pointwise(src):
+    """Variables read by `src` if it is pointwise algebra over variables and
+    constants only (no operator call, no reduction), else None."""
+    try:
+        node = ast.parse(src, mode="eval").body
+    except SyntaxError:
+        return None
+    if not _pointwise(node):
+        return None
+    g = _Gen()
+    g.expr(node)
+    names = []
+    for leaf in g.leaves:
+        nm = _var_name(leaf)
+        if nm is None:
+            return None
+        names.append(nm)
+    return names
+
+
+class PointwiseRun:
+    """Consecutive pointwise equations `lhs_i = rhs_i` as one kernel: reads
+    each outside variable once, writes every lhs. A value produced inside the
+    run is rounded to its field type before later lines use it, so the results
+    are those of evaluating the lines one after another."""
+
+    def __init__(self, lines, tag=""):
+        self.lines = lines  # [(lhs, src)]
+        self.tag = tag
+        self.outputs = [l for l, _ in lines]
+        self.inputs = []      # outside variables in first-use order
+        self.bodies = []      # per line: expression over x<i> (inputs) and l<j> (earlier lines)
+        self.deps = []        # per line: (input indices, earlier-line indices)
+        produced = {}
+        for j, (lhs, src) in enumerate(lines):
+            node = ast.parse(src, mode="eval").body
+            g = _Gen()
+            body = g.expr(node)
+            ins, ints = set(), set()
+            # leaves are x<k> in g's numbering: map them to run inputs / earlier lines
+            ren = {}
+            for k, leaf in enumerate(g.leaves):
+                nm = _var_name(leaf)
+                if nm in produced:
+                    ren["x%d" % k] = "l%d" % produced[nm]
+                    ints.add(produced[nm])
+                else:
+                    if nm not in self.inputs:
+                        self.inputs.append(nm)
+                    ii = self.inputs.index(nm)
+                    ren["x%d" % k] = "X%d" % ii
+                    ins.add(ii)
+            import re as _re
+            body = _re.sub(r"\bx(\d+)\b", lambda m: ren["x" + m.group(1)], body)
+            self.bodies.append(body.replace("X", "x"))
+            self.deps.append((ins, ints))
+            produced[lhs] = j
+        self._kernels = {}
+
+    def _types(self, sig):
+        """Output type of every line ('f64'/'f32'), None if a line has no field input."""
+        out = []
+        for ins, ints in self.deps:
+            kinds = [sig[i] for i in ins] + ["T64" if out[j] == "f64" else "T32" for j in ints]
+            fields = [k for k in kinds if k != "S"]
+            if not fields:
+                return None
+            out.append("f64" if "T64" in fields else "f32")
+        return out
+
+    def _source(self, sig, types):
+        key = repr(self.bodies) + repr(sig) + repr(types)
+        name = "pyr_run_" + hashlib.sha1(key.encode()).hexdigest()[:16]
+        L = [PRELUDE, 'extern "C" __global__ void %s(PyrPwArgs a) {' % name]
+        ip = isc = 0
+        for i, t in enumerate(sig):
+            if t == "S":
+                L.append("  const double x%d = a.s[%d];" % (i, isc))
+                isc += 1
+            else:
+                ct = "double" if t == "T64" else "float"
+                L.append("  const %s* __restrict__ p%d = (const %s*)a.p[%d];" % (ct, i, ct, ip))
+                ip += 1
+        for j, t in enumerate(types):
+            ot = "double" if t == "f64" else "float"
+            L.append("  %s* __restrict__ o%d = (%s*)a.p[%d];" % (ot, j, ot, ip + j))
+        L.append("  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;"
+                 " i += (long long)gridDim.x * blockDim.x) {")
+        for i, t in enumerate(sig):
+            if t != "S":
+                L.append("    const double x%d = (double)p%d[i];" % (i, i))
+        for j, t in enumerate(types):
+            ot = "double" if t == "f64" else "float"
+            L.append("    const %s r%d = (%s)(%s);" % (ot, j, ot, self.bodies[j]))
+            L.append("    o%d[i] = r%d;" % (j, j))
+            L.append("    const double l%d = (double)r%d;" % (j, j))
+        L.append("  }")
+        L.append("}")
+        return name, "\n".join(L)
+
+    def __call__(self, sim):
+        """Evaluate the run on `sim`; False if an input is not a field or scalar
+        (the caller then evaluates the lines one by one)."""
+        shape = tuple(sim.mesh.shape)
+        sig, fields, scalars = [], [], []
+        for nm in self.inputs:
+            v = sim.variables[nm].data
+            if torch.is_tensor(v) and v.dim() == 0:
+                v = v.item()
+            if isinstance(v, (int, float)) and not isinstance(v, bool):
+                sig.append("S")
+                scalars.append(float(v))
+            elif _field_ok(v, shape):
+                sig.append("T64" if v.dtype == torch.float64 else "T32")
+                fields.append(v)
+            else:
+                return False
+        sig = tuple(sig)
+        types = self._types(sig)
+        if types is None or not fields:
+            return False
+        if len(fields) + len(types) > _lib.PW_MAX_PTRS or len(scalars) > _lib.PW_MAX_SCALARS:
+            return False
+        key = (sig, tuple(types))
+        fn = self._kernels.get(key)
+        if fn is None:
+            name, src = self._source(sig, types)
+            h = ctypes.c_void_p()
+            L = _lib.lib()
+            if L.pyr_pw_compile(src.encode(), name.encode(), ctypes.byref(h)) != 0:
+                raise _lib.PyrandaAMDError(L.pyr_pw_last_error().decode())
+            fn = self._kernels[key] = h
+        dev = fields[0].device
+        nx, ny, nz = shape
+        outs = [torch.empty((nz, ny, nx), dtype=torch.float64 if t == "f64" else torch.float32,
+                            device=dev).permute(2, 1, 0) for t in types]
+        ptrs = (ctypes.c_void_p * (len(fields) + len(outs)))(*[f.data_ptr() for f in fields],
+                                                             *[o.data_ptr() for o in outs])
+        sc = (ctypes.c_double * max(1, len(scalars)))(*scalars)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        _lib.check_pw(_lib.lib().pyr_pw_launch(fn, stream, nx * ny * nz, len(fields) + len(outs), ptrs,
+                                               len(scalars), sc))
+        for lhs, o in zip(self.outputs, outs):
+            sim.variables[lhs].data = o
+        return True
+
+
+_SAFE_CALLS = {"ddx", "ddy", "ddz", "dd4x", "dd4y", "dd4z", "ring", "ringV", "gfilter", "gbar",
+               "gbarx", "gbary", "gbarz", "fbar", "sfilter", "filter", "lap", "div", "grad",
+               "laplacian", "d2x", "d2y", "d2z"}
+
+
+def reads_of(src):
+    """Variables an equation reads, or None if it calls anything whose reads
+    are not visible in its text (packages, user functions)."""
+    try:
+        node = ast.parse(src, mode="eval").body
+    except SyntaxError:
+        return None
+    names = set()
+    for n in ast.walk(node):
+        nm = _var_name(n)
+        if nm is not None:
+            names.add(nm)
+        if isinstance(n, ast.Call):
+            f = n.func
+            if isinstance(f, ast.Attribute) and isinstance(f.value, ast.Name):
+                if f.value.id == "self" and f.attr not in _SAFE_CALLS:
+                    return None
+                if f.value.id not in ("self", "numpy"):
+                    return None
+            elif not (isinstance(f, ast.Attribute)):
+                return None
+        if isinstance(n, ast.Name) and n.id not in ("self", "numpy"):
+            return None
+    return names
+
+
+def plan_alg(eqs):
+    """Execution plan of the ALG equations: a list of ('eq', eq) and
+    ('run', PointwiseRun, [eqs]) items. Pointwise equations are grouped into
+    runs; an equation between them may execute first when it neither reads a
+    value the run produces nor writes anything the run reads or writes."""
+    plan = []
+    cur, cur_reads, cur_writes = [], set(), set()
+
+    def flush():
+        nonlocal cur, cur_reads, cur_writes
+        if len(cur) == 1:
+            plan.append(("eq", cur[0]))
+        elif cur:
+            plan.append(("run", PointwiseRun([(e.LHS[0], e.sRHS) for e in cur]), list(cur)))
+        cur, cur_reads, cur_writes = [], set(), set()
+
+    for eq in eqs:
+        pw = (eq.rank == 1 and eq.LHS and pure_pointwise(eq.sRHS)) if eq.kind == "ALG" else None
+        if pw is not None and pw is not False:
+            cur.append(eq)
+            cur_reads |= set(pw)
+            cur_writes.add(eq.LHS[0])
+            continue
+        reads = reads_of(eq.sRHS) if eq.kind == "ALG" else None
+        writes = set(eq.LHS or [])
+        if cur and reads is not None and not (reads & cur_writes) and not (writes & (cur_reads | cur_writes)):
+            plan.append(("eq", eq))
+            continue
+        flush()
+        plan.append(("eq", eq))
+    flush()
+    return plan

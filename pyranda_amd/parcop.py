@@ -100,6 +100,10 @@ class ParcopContext:
             _lib.check(L.pyr_axis_coords(h, ax, buf))
             self.coords1d.append(list(buf))
 
+    def set_r32_solve(self, name):
+        """'refined' (default) or 'fp64': arithmetic of the solves on FP32 fields."""
+        _lib.check(self._L.pyr_set_r32_solve(self._h, _lib.R32_SOLVE[name]))
+
     def set_periodic_solver(self, name):
         """'circulant' (default) or 'lu': solver of periodic implicit lines."""
         _lib.check(self._L.pyr_set_periodic_solver(self._h, _lib.PSOLVE[name]))

@@ -10,6 +10,10 @@ Precision: `precision="fp64"` (default) keeps every field in FP64 like the
 reference. `precision="fp32"` stores fields in FP32 and runs the operators as
 FP32 sweeps plus one FP64-residual refinement; the RK4 state accumulation
 (PHI and the conserved variables) stays in FP64 shadows.
+`precision="fp32-fp64solve"` stores fields in FP32 like "fp32" but forms the
+operators' right-hand sides and solves in FP64 registers (one rounding to
+FP32 per result): on gfx950 FP32 and FP64 vector FMAs issue at the same rate,
+so this keeps the halved memory traffic without the refinement's second solve.
 """
 import math
 import random
@@ -54,8 +58,8 @@ class pyrandaSim:
             self.mesh.coordsys = opts["coordsys"]
         else:
             raise ValueError("No suitable mesh type specified.")
-        if precision not in ("fp64", "fp32"):
-            raise ValueError("precision must be 'fp64' or 'fp32'")
+        if precision not in ("fp64", "fp32", "fp32-fp64solve"):
+            raise ValueError("precision must be 'fp64', 'fp32' or 'fp32-fp64solve'")
         self.precision = precision
         self.dtype = torch.float64 if precision == "fp64" else torch.float32
         if not torch.cuda.is_available():
@@ -73,6 +77,8 @@ class pyrandaSim:
         self.nPDE = self.nALG = self.nELP = 0
         self.mesh.dims = opts["dim"]
         self.PyMPI = pyrandaMPI(self.mesh, comm=opts.get("comm"), dtype=self.dtype, device=device)
+        if precision == "fp32-fp64solve":
+            self.PyMPI.ctx.set_r32_solve("fp64")
         self.mesh.PyMPI = self.PyMPI
         self.mesh.makeMesh()
         set_default(self.PyMPI.device, self.dtype)
@@ -250,20 +256,37 @@ class pyrandaSim:
                 flux[eqo.LHS[0]] = eqo.RHS(self)
         return flux
 
+    def _alg_plan(self):
+        from . import fuse
+        from .pyrandaEq import FUSE
+        eqs = [eq for eq in self.equations if eq.kind == "ALG"]
+        key = tuple((id(eq), eq.active) for eq in eqs)
+        if getattr(self, "_plan_key", None) != key:
+            active = [eq for eq in eqs if eq.active]
+            self._plan = fuse.plan_alg(active) if FUSE else [("eq", eq) for eq in active]
+            self._plan_key = key
+        return self._plan
+
+    def _eval_alg(self, eq, verbose):
+        if verbose:
+            self.iprint(eq.eqstr)
+        rhs = eq.RHS(self)
+        if not eq.LHS:
+            return
+        if eq.rank == 1:
+            self.variables[eq.LHS[0]].data = rhs
+        else:
+            for ii in range(len(rhs)):
+                self.variables[eq.LHS[ii]].data = rhs[ii]
+
     def updateVars(self, verbose=False):
-        for eq in self.equations:
-            if not eq.active or eq.kind != "ALG":
-                continue
-            if verbose:
-                self.iprint(eq.eqstr)
-            rhs = eq.RHS(self)
-            if not eq.LHS:
-                continue
-            if eq.rank == 1:
-                self.variables[eq.LHS[0]].data = rhs
-            else:
-                for ii in range(len(rhs)):
-                    self.variables[eq.LHS[ii]].data = rhs[ii]
+        # pointwise ALG lines run as fused multi-output kernels where possible
+        for item in self._alg_plan():
+            if item[0] == "eq":
+                self._eval_alg(item[1], verbose)
+            elif verbose or not item[1](self):
+                for eq in item[2]:
+                    self._eval_alg(eq, verbose)
 
     def rk4(self, time, dt):
         """5-stage 2N-storage RK4 (pyranda.py:758-810).
@@ -275,11 +298,16 @@ class pyrandaSim:
         fp32 = self.dtype != torch.float64
         mpi = self.PyMPI
         PHI = {}
+        # conserved variables that an ALG line overwrites after every stage (the
+        # conservative filter): their FP64 shadow would be re-derived from the
+        # FP32 field each stage, which is what the update does without one
+        rewritten = self._rewritten_conserved() if fp32 else set()
         for U in self.conserved:
-            PHI[U] = torch.zeros((self.nz, self.ny, self.nx), dtype=torch.float64,
+            # the first stage has A = 0 and only writes PHI
+            PHI[U] = torch.empty((self.nz, self.ny, self.nx), dtype=torch.float64,
                                  device=mpi.device).permute(2, 1, 0)
             if fp32:
-                self._shadow[U] = self.variables[U].data.to(torch.float64)
+                self._shadow[U] = None if U in rewritten else self.variables[U].data.to(torch.float64)
         time_i = time
         self.deltat = dt
         for ii in range(5):
@@ -293,10 +321,17 @@ class pyrandaSim:
             self.updateVars()
             if fp32:
                 for U in self.conserved:
-                    if self.variables[U].data is not before[U]:
+                    if U not in rewritten and self.variables[U].data is not before[U]:
                         self._shadow[U] = self.variables[U].data.to(torch.float64)
         self.cycle += 1
         return time
+
+    def _rewritten_conserved(self):
+        out = set()
+        for eq in self.equations:
+            if eq.active and eq.kind == "ALG" and eq.LHS:
+                out.update(U for U in eq.LHS if U in self.conserved)
+        return out
 
     def _rk_update(self, U, F, PHI, dt, A, B, fp32):
         from . import _lib
@@ -308,15 +343,16 @@ class pyrandaSim:
               and _contig_equiv(u) and u.dtype == self.dtype)
         if not ok:
             # scalar fluxes (e.g. ODE right-hand sides) and odd layouts: torch ops
-            PHI[U] = dt * F + A * PHI[U]
+            PHI[U] = dt * F + A * (PHI[U] if A != 0.0 else torch.zeros_like(PHI[U]))
             if fp32:
-                self._shadow[U] = self._shadow[U] + B * PHI[U]
+                base = self._shadow[U] if self._shadow[U] is not None else u.to(torch.float64)
+                self._shadow[U] = base + B * PHI[U]
                 return self._shadow[U].to(self.dtype)
             return u + B * PHI[U]
         out = torch.empty((self.nz, self.ny, self.nx), dtype=self.dtype, device=u.device).permute(2, 1, 0)
         ctx = self.PyMPI.ctx
         ctx._stream()
-        shadow = self._shadow[U].data_ptr() if fp32 else None
+        shadow = self._shadow[U].data_ptr() if fp32 and self._shadow[U] is not None else None
         _lib.check(ctx._L.pyr_rk_update(ctx._h, _lib.R32 if fp32 else _lib.F64, self.npts,
                                         F.data_ptr(), PHI[U].data_ptr(), u.data_ptr(), out.data_ptr(),
                                         shadow, float(dt), float(A), float(B)))

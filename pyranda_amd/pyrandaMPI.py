@@ -121,19 +121,26 @@ class pyrandaMPI:
     # -- reductions -----------------------------------------------------------
     def sum3D(self, data):
         if torch.is_tensor(data):
-            return torch.sum(data, dtype=torch.float64).item()
+            return torch.sum(self._flat(data), dtype=torch.float64).item()
         import numpy as np
         return float(np.sum(data))
 
+    @staticmethod
+    def _flat(data):
+        # the field layout (x fastest) is the contiguous order of the reversed view
+        if data.dim() == 3 and data.permute(2, 1, 0).is_contiguous():
+            return data.permute(2, 1, 0)
+        return data
+
     def max3D(self, data):
         if torch.is_tensor(data):
-            return torch.max(data).item()
+            return torch.max(self._flat(data)).item()
         import numpy as np
         return float(np.max(data))
 
     def min3D(self, data):
         if torch.is_tensor(data):
-            return torch.min(data).item()
+            return torch.min(self._flat(data)).item()
         import numpy as np
         return float(np.min(data))
 

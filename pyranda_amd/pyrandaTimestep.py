@@ -4,8 +4,14 @@ courant: 1/max(|u|/dx + |v|/dy + |w|/dz + |c|/GridLen) over the grid;
 diff: min(rho*GridLen^2 / max(1e-12, bulk)). Both are device reductions
 returning Python floats, as the reference's MPI allreduce does.
 """
+from . import fuse
 from .devnumpy import numpy
 from .pyrandaPackage import pyrandaPackage
+
+# one-pass device reductions of the same expressions (same operation order)
+_COURANT = fuse.FusedReduce("max", "(((fabs(x0) / x4 + fabs(x1) / x5) + fabs(x2) / x6) + fabs(x3) / x7)",
+                            4, 4, "courant")
+_DIFF = fuse.FusedReduce("min", "((x1 * x2) * x2) / pyr_max(1.0e-12, x0)", 2, 1, "diff")
 
 
 class pyrandaTimestep(pyrandaPackage):
@@ -26,12 +32,18 @@ class pyrandaTimestep(pyrandaPackage):
         }
 
     def courant(self, u, v, w, c):
+        shape = self.pyranda.mesh.shape
+        if fuse.fields_ok(shape, u, v, w, c):
+            return 1.0 / _COURANT(shape, [u, v, w, c], [self.dx, self.dy, self.dz, self.GridLen])
         vrate = numpy.abs(u) / self.dx + numpy.abs(v) / self.dy + numpy.abs(w) / self.dz
         crate = numpy.abs(c) / self.GridLen
         return 1.0 / self.pyranda.PyMPI.max3D(vrate + crate)
 
     def diff(self, bulk, density):
         delta = self.GridLen
+        shape = self.pyranda.mesh.shape
+        if fuse.fields_ok(shape, bulk, density):
+            return _DIFF(shape, [bulk, density], [delta])
         drate = density * delta * delta / numpy.maximum(1.0e-12, bulk)
         return self.pyranda.PyMPI.min3D(drate)
 

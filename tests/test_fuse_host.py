@@ -5,7 +5,10 @@
 * the rewritten expression keeps the semantics of the original right-hand
   side: evaluated on CPU tensors (where every fused subtree takes its torch
   path) it equals the plain `lambda self: <rhs>` bit for bit;
-* the generated C++ expression of a subtree is the expected one.
+* the generated C++ expression of a subtree is the expected one;
+* the planner groups the Taylor-Green pointwise lines into runs, and running
+  its plan gives the variables of the written order; the run and reduction
+  kernels compile.
 """
 import ast
 import ctypes
@@ -98,3 +101,83 @@ def test_fused_rewrite_preserves_semantics():
             assert torch.equal(a, b), src
         else:
             assert a == b, src
+
+
+def _tg_eqs():
+    from pyranda_amd import pyranda as P
+    from pyranda_amd.pyrandaEq import pyrandaEq
+    from pyranda_amd.pyrandaTimestep import pyrandaTimestep
+    holder = types.SimpleNamespace(packages={})
+    sMap = P.pyrandaSim.get_sMap(holder) or holder.sMap
+    ts = pyrandaTimestep.__new__(pyrandaTimestep)
+    ts.get_sMap()
+    sMap.update(ts.sMap)
+    return [pyrandaEq(line.strip(), sMap, None) for line in TG_EOM.split("\n")
+            if line.strip() and not line.strip().startswith("#")]
+
+
+def test_plan_groups_pointwise_runs():
+    plan = fuse.plan_alg([e for e in _tg_eqs() if e.kind == "ALG"])
+    runs = [[e.LHS[0] for e in item[2]] for item in plan if item[0] == "run"]
+    assert runs == [["u", "v", "w", "p"], ["div", "enst", "tke", "S"],
+                    ["taudia", "tauxx", "tauyy", "tauzz", "tauxy", "tauxz", "tauyz", "cs"]]
+    # every ALG equation appears exactly once
+    seen = _tg_eqs_alg_cache(plan)
+    assert len(seen) == len(set(map(id, seen))) == len([e for e in _tg_eqs() if e.kind == "ALG"])
+
+
+def _tg_eqs_alg_cache(plan):
+    return [e for item in plan for e in ([item[1]] if item[0] == "eq" else item[2])]
+
+
+def test_plan_order_preserves_results():
+    """Executing the plan (runs line by line, as their CPU fallback does) gives
+    the same variables as the equations in their written order."""
+    from pyranda_amd import devnumpy
+    alg = [e for e in _tg_eqs() if e.kind == "ALG"]
+    names = set()
+    for e in alg:
+        names |= set(e.LHS)
+        names |= set(fuse.reads_of(e.sRHS) or [])
+    names |= {"rho", "rhou", "rhov", "rhow", "Et", "gamma", "dt"}
+    g = {"numpy": devnumpy.numpy}
+
+    def run(order):
+        sim = _Sim(sorted(names))
+        sim.variables["gamma"].data = 1.4
+        for e in order:
+            r = eval("lambda self: " + e.sRHS, g)(sim)
+            sim.variables[e.LHS[0]].data = r
+        return sim
+
+    ref = run(alg)
+    got = run(_tg_eqs_alg_cache(fuse.plan_alg(alg)))
+    for k in names:
+        a, b = ref.variables[k].data, got.variables[k].data
+        if torch.is_tensor(a):
+            torch.testing.assert_close(a, b, rtol=0, atol=0, equal_nan=True, msg=k)
+        else:
+            assert a == b, k
+
+
+def test_run_and_reduction_kernels_compile(amd):
+    L = amd.lib()
+    plan = fuse.plan_alg([e for e in _tg_eqs() if e.kind == "ALG"])
+    n = 0
+    for item in plan:
+        if item[0] != "run":
+            continue
+        run = item[1]
+        for t in ("T32", "T64"):
+            sig = tuple("S" if nm == "gamma" else t for nm in run.inputs)
+            types_ = run._types(sig)
+            name, code = run._source(sig, types_)
+            assert L.pyr_pw_check_source(code.encode(), name.encode()) == 0, L.pyr_pw_last_error().decode()
+            n += 1
+    assert n == 6
+    import importlib
+    T = importlib.import_module("pyranda_amd.pyrandaTimestep")
+    for red, nf in ((T._COURANT, 4), (T._DIFF, 2)):
+        for t in ("T32", "T64"):
+            name, code = red._source(tuple(t for _ in range(nf)))
+            assert L.pyr_pw_check_source(code.encode(), name.encode()) == 0, L.pyr_pw_last_error().decode()

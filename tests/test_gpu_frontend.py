@@ -172,3 +172,72 @@ def test_taylor_green_32(pr):
     want = _golden("taylorgreen")
     _say("taylorgreen", repr(got), want)
     assert abs(got - want) / want < 1e-6
+
+
+@pytest.mark.parametrize("precision", ["fp32", "fp32-fp64solve"])
+def test_taylor_green_32_fp32_fields(pr, precision):
+    """FP32 fields: the enstrophy golden (11 digits) to FP32 storage accuracy."""
+    got = taylor_green_enstrophy(pr, 32, precision=precision)
+    want = _golden("taylorgreen")
+    _say("taylorgreen", precision, repr(got), want)
+    assert abs(got - want) / want < 1e-5
+
+
+EULER_EOM = """
+# Primary Equations of motion here
+ddt(:rho:)  =  -ddx(:rho:*:u:)                  - ddy(:rho:*:v:)
+ddt(:rhou:) =  -ddx(:rhou:*:u: + :p: - :tau:)   - ddy(:rhou:*:v:)
+ddt(:rhov:) =  -ddx(:rhov:*:u:)                 - ddy(:rhov:*:v: + :p: - :tau:)
+ddt(:Et:)   =  -ddx( (:Et: + :p: - :tau:)*:u: ) - ddy( (:Et: + :p: - :tau:)*:v: )
+# Conservative filter of the EoM
+:rho:       =  fbar( :rho:  )
+:rhou:      =  fbar( :rhou: )
+:rhov:      =  fbar( :rhov: )
+:Et:        =  fbar( :Et:   )
+# Update the primatives and enforce the EOS
+:u:         =  :rhou: / :rho:
+:v:         =  :rhov: / :rho:
+:p:         =  ( :Et: - .5*:rho:*(:u:*:u: + :v:*:v:) ) * ( :gamma: - 1.0 )
+# Artificial bulk viscosity (old school way)
+:div:       =  ddx(:u:) + ddy(:v:)
+:beta:      =  gbar(abs(ring(:div:))) * :rho: * 7.0e-2
+:tau:       =  :beta:*:div:
+# Apply constant BCs
+bc.extrap(['rho','Et'],['x1','xn','y1','yn'])
+bc.const(['u','v'],['x1','xn','y1','yn'],0.0)
+"""
+
+
+@pytest.mark.parametrize("npts", [64, 128])
+def test_euler_2d_curve(pr, npts):
+    """examples/euler.py (2-D Sod, bounded x and y, BC lines) against the
+    reference-held curve tests/baselines/euler-2d-<n>.dat."""
+    import numpy
+    L = numpy.pi * 2.0
+    Lp = L * (npts - 1.0) / npts
+    mesh_options = {"coordsys": 0, "periodic": numpy.array([False, False, True]), "dim": 3,
+                    "x1": [0.0, 0.0, 0.0], "xn": [Lp, Lp, Lp], "nn": [npts, npts, 1]}
+    ss = pr.pyrandaSim("advection", mesh_options, silent=True)
+    ss.addPackage(pr.pyrandaBC(ss))
+    ss.EOM(EULER_EOM)
+    ss.setIC("""rad = sqrt( (meshx-pi)**2  +  (meshy-pi)**2 )
+    :gamma: = 1.4
+    :Et:  = gbar( where( rad < pi/2.0, 1.0/(:gamma:-1.0) , .1 /(:gamma:-1.0) ) )
+    :rho: = gbar( where( rad < pi/2.0, 1.0    , .125 ) )
+    """)
+    x = ss.mesh.coords[0].data
+    xx = ss.PyMPI.zbar(x)
+    ny = ss.PyMPI.ny
+    dt_max = 1.0 / ss.mesh.nn[0] * 0.75
+    tt = L / 1.0 * .125
+    time, dt = 0.0, dt_max
+    while tt > time:
+        time = ss.rk4(time, dt)
+        dt = min(dt_max, (tt - time))
+    v = ss.PyMPI.zbar(ss.variables["rho"].data)
+    base = numpy.loadtxt(os.path.join(os.path.dirname(GOLDEN), "euler-2d-%d.dat" % npts))
+    assert numpy.allclose(xx[:, ny // 2], base[0], rtol=0, atol=1e-15)
+    err = numpy.abs(v[:, ny // 2] - base[1]).max() / numpy.abs(base[1]).max()
+    _say("euler-2d", npts, "max rel diff", err)
+    # the reference harness accepts 1e-4 (tests/testObj.py:52-58)
+    assert err <= 1e-9

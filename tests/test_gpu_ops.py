@@ -92,6 +92,11 @@ def test_directional_ops_f64_and_r32(dev, oracle_mod, nn, kinds, solver):
             ref32 = P.apply(op, d, v32)
             got32 = to_host(ctx.apply(op, d, vd32))
             assert _rel(got32, ref32) <= 1e-6, ("r32", op, d, _rel(got32, ref32))
+            # FP32 fields with the FP64 solve: one rounding of each result
+            ctx.set_r32_solve("fp64")
+            got32 = to_host(ctx.apply(op, d, vd32))
+            ctx.set_r32_solve("refined")
+            assert _rel(got32, ref32) <= 1.2e-7, ("r32-fp64solve", op, d, _rel(got32, ref32))
     ctx.close()
 
 

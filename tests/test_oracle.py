@@ -11,7 +11,11 @@ metric) fails at least one of them:
 * bounded and symmetric operators: dense numpy solves of the assembled
   band, built here independently of the oracle's LU and RHS forms;
 * the whole chain: the reference-held regression value of
-  examples/unit_test.py (tests/cases/testUnit.py:3).
+  examples/unit_test.py (tests/cases/testUnit.py:3), and the reference-held
+  density curves of the 2-D Euler (Sod) case at 64^2 and 128^2
+  (tests/baselines/euler-2d-*.dat, tests/cases/test2deuler.py:14-22), which
+  run bounded lines, LAD ring/gbar, the filter and the pyrandaBC
+  extrapolation/constant boundaries through oracle/euler_cpu.py.
 """
 import json
 import os
@@ -279,3 +283,16 @@ def test_planted_mistake_is_caught(oracle_mod, monkeypatch):
     got = unit_test_value(po)
     want = _golden()["unit-test-3D"]["value"]
     assert abs(got - want) / abs(want) > 1e-3
+
+
+@pytest.mark.parametrize("n", [64, 128])
+def test_euler_2d_curve_golden(oracle_mod, n):
+    """Reference-held curve (x, rho(x, ny/2)) of examples/euler.py at t = L/8."""
+    from oracle.euler_cpu import Euler2DCPU
+    base = np.loadtxt(os.path.join(GOLDEN, "euler-2d-%d.dat" % n))
+    x, rho = Euler2DCPU(n).run_case()
+    assert np.array_equal(x, base[0])
+    # the reference harness accepts 1e-4 (tests/testObj.py:52-58); the
+    # restatement reproduces the curve to the last bit
+    err = np.abs(rho - base[1]).max() / np.abs(base[1]).max()
+    assert err <= 1e-13, err
