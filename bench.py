@@ -32,6 +32,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+
+# HBM bytes per dispatch of the line kernels at 256^3, measured with rocprofv3
+# --pmc FETCH_SIZE / WRITE_SIZE (KiB per dispatch), FETCH_SIZE scaled by 2.0 and
+# WRITE_SIZE by 1.0 from a plain device copy of known size; the table and the
+# commands are in profiles/line_ops_hbm_traffic_pmc.md. Key: (op, direction,
+# field dtype) of the circulant solve.
+MEASURED_TRAFFIC = {
+    ("d1", "x", "torch.float64"): (131401 + 131072) * 1024,
+    ("d1", "x", "torch.float32"): (65950 + 65536) * 1024,
+    ("d1", "y", "torch.float64"): (131468 + 131072) * 1024,
+    ("d1", "y", "torch.float32"): (65848 + 65536) * 1024,
+    ("fbar", "y", "torch.float64"): (131510 + 131072) * 1024,
+    ("fbar", "y", "torch.float32"): (65952 + 65536) * 1024,
+    ("gbar", "y", "torch.float64"): (140248 + 131072) * 1024,
+    ("gbar", "y", "torch.float32"): (65844 + 65536) * 1024,
+}
 OPS = ["d1", "d2", "d8", "fbar", "gbar"]
 DIRS = "xyz"
 
@@ -95,7 +111,7 @@ def _symbols(theta):
     return [d1, d2, d8, sf, gf]
 
 
-def ops_bench(n, steps, warmup, dev, only=None, solver="circulant"):
+def ops_bench(n, steps, warmup, dev, only=None, solver="circulant", r32_solve="refined"):
     """Operator microbenchmark: every directional operator on the seeded
     64-mode Fourier field (|k|<=32, |k|^-5/3, seed 1234), FP64 and FP32-refined,
     timed with HIP events, error against the exact per-mode result."""
@@ -106,6 +122,7 @@ def ops_bench(n, steps, warmup, dev, only=None, solver="circulant"):
     L = 2 * math.pi * (n - 1) / n
     ctx = ParcopContext((n, n, n), (0.0, 0.0, 0.0), (L, L, L), ["PERI"] * 6, device=dev)
     ctx.set_periodic_solver(solver)
+    ctx.set_r32_solve(r32_solve)
     h = ctx.dpatch[0]
     ks, amps, phis = fourier_modes()
     f64 = fourier_field(n, dev, (ks, amps, phis))
@@ -235,6 +252,10 @@ def main():
                     help="operator microbenchmark on the 256^3 seeded Fourier field instead")
     ap.add_argument("--solver", default="circulant", choices=["circulant", "lu"],
                     help="solver of periodic implicit lines")
+    ap.add_argument("--r32-solve", default="refined", choices=["refined", "fp64"],
+                    help="--ops: arithmetic of the solves on FP32 fields")
+    ap.add_argument("--precision", default="fp64,fp32,fp32-fp64solve",
+                    help="comma-separated pyrandaSim precisions to time (the headline is fp32)")
     ap.add_argument("--only", default=None,
                     help="--ops: comma-separated name prefixes to run (e.g. d1_x_f64,copy_f64)")
     args = ap.parse_args()
@@ -251,7 +272,7 @@ def main():
     dev = torch.device("cuda:0")
     n = args.n
     if args.ops:
-        return ops_bench(n, args.steps, args.warmup, dev, args.only, args.solver)
+        return ops_bench(n, args.steps, args.warmup, dev, args.only, args.solver, args.r32_solve)
     gamma = 1.4
     p0 = 1.0 / (gamma * 0.1 ** 2)  # Mach 0.1 on the unit velocity
 
@@ -259,7 +280,9 @@ def main():
     results = {}
     states = {}
     dt = None
-    precs = ["fp32", "fp32-fp64solve"] if args.no_fp64 else ["fp64", "fp32", "fp32-fp64solve"]
+    precs = [p for p in args.precision.split(",") if p and not (args.no_fp64 and p == "fp64")]
+    if "fp32" not in precs:
+        precs.append("fp32")
     for precision in precs:
         ss = make_tg(n, precision, p0, args.solver)
         if dt is None:
@@ -299,7 +322,8 @@ def main():
     if "fp64" in states:
         for k in states["fp32"]:
             acc[k] = rel_err(states["fp32"][k], states["fp64"][k])
-            acc_s[k] = rel_err(states["fp32-fp64solve"][k], states["fp64"][k])
+            if "fp32-fp64solve" in states:
+                acc_s[k] = rel_err(states["fp32-fp64solve"][k], states["fp64"][k])
 
     cpu = None
     if not args.no_cpu:
@@ -337,10 +361,11 @@ def main():
             "speedup_over_fp64": (results["fp32-fp64solve"]["pups"] / results["fp64"]["pups"])
             if "fp64" in results else None,
             "accuracy_vs_fp64": {k: {"rel_max": a, "rel_l2": b} for k, (a, b) in acc_s.items()},
-        },
+        } if "fp32-fp64solve" in results else None,
         "accuracy_fp32r_vs_fp64": {k: {"rel_max": a, "rel_l2": b} for k, (a, b) in acc.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": (MEASURED_TRAFFIC.get((OPS[dom[0]], DIRS[dom[1]], dom[2])) if n == 256 else None),
                      "kernel": "%s_%s (%s)" % (OPS[dom[0]], DIRS[dom[1]], dom[2].replace("torch.", "")),
                      "launches": count, "mean_us": mean_ms * 1e3, "bytes_per_launch": alg_bytes,
                      "operator_time_share": op_share},

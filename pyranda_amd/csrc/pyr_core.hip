@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 #include <memory>
@@ -168,18 +169,27 @@ __device__ __forceinline__ void circ_solve(const C* kf, const C* kb, C p1, C p2,
 // Circulant solve of one chunk; X0 and X1 hold 4P x ly entries each. Every
 // buffer is written only after a barrier that follows its last read. The
 // solution rows go to emit(k, y_k) in FP64 (before the metric scale).
+// carry rows of the circulant solve in the solve type: 0 forward, 1 backward
+template <int MODE>
+__device__ __forceinline__ const typename SolveT<MODE>::type* kmat(const LineArgs& a, int which) {
+  if constexpr (MODE == MODE_R32) return which ? a.kb32 : a.kf32;
+  else return which ? a.kb : a.kf;
+}
+
+// kf/kb: the carry rows in the solve type (global memory or an LDS copy)
 template <int MODE, typename CS, typename Emit>
-__device__ __forceinline__ void solve_chunk_circ(const LineArgs& a, int p, int m, CS* X0, CS* X1,
-                                                 int ly, int col, double* rhs, Emit emit) {
+__device__ __forceinline__ void solve_chunk_circ(const LineArgs& a, const CS* kf, const CS* kb, int p,
+                                                 int m, CS* X0, CS* X1, int ly, int col, double* rhs,
+                                                 Emit emit) {
   const int P = a.P;
   if constexpr (MODE != MODE_R32) {
-    circ_solve<double>(a.kf, a.kb, a.p1, a.p2, p, P, m, X0, X1, ly, col, rhs,
+    circ_solve<double>(kf, kb, a.p1, a.p2, p, P, m, X0, X1, ly, col, rhs,
                        [&](int k, double v) { emit(k, v * a.invc); });
   } else {
     float y[CHUNK];
 #pragma unroll
     for (int k = 0; k < CHUNK; ++k) y[k] = (float)rhs[k];
-    circ_solve<float>(a.kf32, a.kb32, a.p1f, a.p2f, p, P, m, X0, X1, ly, col, y,
+    circ_solve<float>(kf, kb, a.p1f, a.p2f, p, P, m, X0, X1, ly, col, y,
                       [&](int k, float v) { y[k] = v * a.invcf; });
     // neighbour values of the FP32 solution for the residual
     float e4[4];
@@ -195,7 +205,7 @@ __device__ __forceinline__ void solve_chunk_circ(const LineArgs& a, int p, int m
     float d[CHUNK];
 #pragma unroll
     for (int k = 0; k < CHUNK; ++k) d[k] = (float)rhs[k];
-    circ_solve<float>(a.kf32, a.kb32, a.p1f, a.p2f, p, P, m, X1, X0, ly, col, d,
+    circ_solve<float>(kf, kb, a.p1f, a.p2f, p, P, m, X1, X0, ly, col, d,
                       [&](int k, float v) { emit(k, (double)y[k] + (double)(v * a.invcf)); });
   }
 }
@@ -211,7 +221,8 @@ enum Pre { PRE_NONE = 0, PRE_V = 1, PRE_OLD = 2, PRE_LATE = 3 };
 // Build-time tuning knobs (register pressure against latency hiding):
 //   PYR_FULL_CHUNKS  compile-time row counts when CHUNK divides N
 //   PYR_PREFETCH     PRE_V / PRE_OLD (1) or PRE_LATE (0)
-//   PYR_YZ_WAVES_PER_EU  minimum waves per SIMD the y/z kernels are built for
+//   PYR_YZ_WAVES_PER_EU, PYR_X_WAVES_PER_EU  minimum waves per SIMD the y/z
+//                    and x kernels are built for
 #ifndef PYR_FULL_CHUNKS
 #define PYR_FULL_CHUNKS 1
 #endif
@@ -220,6 +231,9 @@ enum Pre { PRE_NONE = 0, PRE_V = 1, PRE_OLD = 2, PRE_LATE = 3 };
 #endif
 #ifndef PYR_YZ_WAVES_PER_EU
 #define PYR_YZ_WAVES_PER_EU 1
+#endif
+#ifndef PYR_X_WAVES_PER_EU
+#define PYR_X_WAVES_PER_EU 1
 #endif
 
 // Raw buffer access to one field: a wave's row of 64 adjacent lines is one
@@ -246,10 +260,23 @@ __device__ __forceinline__ void buf_store(float v, __amdgpu_buffer_rsrc_t r, int
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, voff, soff, 0);
 }
 
+// Occupancy the y/z kernels are built for (waves per SIMD). Measured on
+// gfx950 at 256^3: capping the FP64-solve circulant kernels without epilogue
+// prefetch and the FP32 explicit filter at 128 VGPRs (4 waves) is faster
+// despite a few spilled registers (d1 y 39 -> 33 us, gbar y 62 -> 54 us);
+// for every other instance the spills cost more than the occupancy gains.
+template <typename TS, int SOLVE, int MODE, int PRE>
+struct YzWaves {
+  static constexpr int v =
+      PYR_YZ_WAVES_PER_EU > 1 ? PYR_YZ_WAVES_PER_EU
+      : (((MODE == MODE_S32 && SOLVE == SOLVE_CIRC && PRE == PRE_NONE) ||
+          (SOLVE == SOLVE_NONE && sizeof(TS) == 4)) ? 4 : 1);
+};
+
 // FULL: every chunk has CHUNK rows (N a multiple of CHUNK), so the row
 // predicates vanish at compile time. The circulant solve implies periodic lines.
 template <typename TS, int KIND, int SOLVE, int MODE, int LY, bool FULL, int PRE>
-__global__ void __launch_bounds__(WG_MAX) __attribute__((amdgpu_waves_per_eu(PYR_YZ_WAVES_PER_EU, 8)))
+__global__ void __launch_bounds__(WG_MAX) __attribute__((amdgpu_waves_per_eu(YzWaves<TS, SOLVE, MODE, PRE>::v, 8)))
 k_line_yz(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
   constexpr int R = Halo<KIND>::R;
   constexpr bool PER = SOLVE == SOLVE_CIRC;
@@ -325,7 +352,8 @@ k_line_yz(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
     st(epilogue<TS>(a.epi, a.epi_c, val, old), s + k);
   };
   if (SOLVE == SOLVE_CIRC) {
-    solve_chunk_circ<MODE, CS>(a, p, m, Y, Y + 4 * a.P * LY, LY, lane, rhs, emit);
+    solve_chunk_circ<MODE, CS>(a, kmat<MODE>(a, 0), kmat<MODE>(a, 1), p, m, Y, Y + 4 * a.P * LY, LY, lane,
+                               rhs, emit);
   } else {
     if (SOLVE == SOLVE_LU) {
       __syncthreads();  // staged tables visible
@@ -334,6 +362,121 @@ k_line_yz(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
 #pragma unroll
     for (int k = 0; k < CHUNK; ++k)
       if (k < m) emit(k, rhs[k]);
+  }
+}
+
+// 16-byte copy from HBM straight into LDS (global_load_lds_dwordx4): lane i's
+// bytes land at lds + 16 i. The builtin is given to the device pass only:
+// with it in a kernel body the host pass drops the kernel's launch stub.
+__device__ __forceinline__ void lds_copy16(const void* g, void* lds) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  __builtin_amdgcn_global_load_lds(g, (__attribute__((address_space(3))) void*)lds, 16, 0, 0);
+#endif
+}
+
+// y/z lines, software-pipelined. A persistent workgroup walks groups of LG
+// adjacent lines (LG lanes x P chunks). The N rows of the next group are
+// copied from HBM into one of two LDS tiles by global->LDS DMA (no registers,
+// no waits) while the current group is solved from the other tile, so HBM
+// reads overlap the solve even with few waves per CU. Results leave as
+// coalesced buffer stores. Requirements, checked on the host: periodic lines,
+// CHUNK | N, the LG lines of a group adjacent in memory (LG | A), 16-byte
+// aligned rows and both tiles within the LDS budget.
+template <typename TS, int KIND, int SOLVE, int MODE, int LG, int PRE>
+__global__ void __launch_bounds__(WG_MAX) k_line_yz_pipe(LineArgs a, const TS* __restrict__ in,
+                                                         TS* __restrict__ out) {
+  constexpr int R = Halo<KIND>::R;
+  using CS = typename SolveT<MODE>::type;
+  constexpr int VE = 16 / (int)sizeof(TS);        // elements one lane moves
+  constexpr int LPR = LG * (int)sizeof(TS) / 16;  // lanes per tile row
+  constexpr int RPI = 64 / LPR;                   // tile rows per wave instruction (1 KiB)
+  static_assert(LPR >= 1 && 64 % LPR == 0, "tile row must be a whole number of 16-byte pieces");
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = a.N, P = a.P;
+  const int lane = threadIdx.x;
+  const int p = threadIdx.y;
+  const int flat = threadIdx.x + LG * threadIdx.y;
+  const int wave = __builtin_amdgcn_readfirstlane(flat >> 6);
+  const int nwaves = (LG * (int)blockDim.y) >> 6;
+  const int lane64 = flat & 63;
+  // tile rows: R wrapped rows, the N rows of the lines, R wrapped rows, and
+  // up to RPI-1 surplus rows (copies are issued in runs of RPI rows)
+  const int NT = ((N + 2 * R + RPI - 1) / RPI) * RPI;
+  TS* tiles = reinterpret_cast<TS*>(smem);
+  const int tile_elems = NT * LG;
+  CS* X0 = reinterpret_cast<CS*>(smem + align16(2 * (size_t)tile_elems * sizeof(TS)));
+  CS* X1 = X0 + 4 * P * LG;
+  CS* Kl = X1 + 4 * P * LG;  // carry rows, forward then backward (4P^2 each)
+  if constexpr (SOLVE == SOLVE_CIRC) {
+    const CS* kf = kmat<MODE>(a, 0);
+    const CS* kb = kmat<MODE>(a, 1);
+    const int nk = 4 * P * P;
+    for (int e = flat; e < nk; e += LG * (int)blockDim.y) {
+      Kl[e] = kf[e];
+      Kl[nk + e] = kb[e];
+    }
+  }
+  const int ngroups = a.nlines / LG;
+  const __amdgpu_buffer_rsrc_t rout = buf_rsrc(out, a.nbytes);
+  const int row_bytes = (int)(a.stride * (long long)sizeof(TS));
+  const int s = p * CHUNK;
+  constexpr int m = CHUNK;
+
+  // tile row j (0 <= j < NT) holds line row (j - R) mod N
+  auto issue = [&](int g, int t) {
+    const long long L0 = (long long)g * LG;
+    const TS* src = in + (L0 % a.A) + a.B * (L0 / a.A);
+    TS* dst = tiles + t * tile_elems;
+    const int c = (lane64 % LPR) * VE;
+    for (int j0 = wave * RPI; j0 < NT; j0 += nwaves * RPI) {
+      int r = j0 + lane64 / LPR - R;
+      r = r < 0 ? r + N : (r >= N ? r - N : r);
+      lds_copy16(src + (long long)a.stride * r + c, dst + j0 * LG);
+    }
+  };
+
+  int t = 0;
+  if ((int)blockIdx.x < ngroups) issue(blockIdx.x, 0);
+  for (int g = blockIdx.x; g < ngroups; g += gridDim.x, t ^= 1) {
+    // this wave's copies (and last group's stores) are done; the barrier makes
+    // every wave's copies visible and retires all reads of the other tile
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    const long long L = (long long)g * LG + lane;
+    const int lane_off = (int)(((L % a.A) + a.B * (L / a.A)) * (long long)sizeof(TS));
+    // accumulating epilogues read the current output before the next copies are
+    // queued, so waiting for them never waits for the copies
+    TS pre[CHUNK];
+    if constexpr (PRE == PRE_OLD) {
+#pragma unroll
+      for (int k = 0; k < CHUNK; ++k) pre[k] = buf_load(TS(), rout, lane_off + (s + k) * row_bytes, 0);
+    }
+    const int gn = g + gridDim.x;
+    if (gn < ngroups) issue(gn, t ^ 1);
+    // window rows s-R .. s+CHUNK+R-1 are tile rows s .. s+CHUNK+2R-1 (no wrap)
+    const TS* col0 = tiles + t * tile_elems + s * LG + lane;
+    TS win[CHUNK + 2 * R];
+#pragma unroll
+    for (int w = 0; w < CHUNK + 2 * R; ++w) win[w] = col0[w * LG];
+    if constexpr (PRE == PRE_V) {
+#pragma unroll
+      for (int k = 0; k < CHUNK; ++k) pre[k] = win[k + R];
+    }
+    double rhs[CHUNK];
+    rhs_interior<KIND>(a, win, rhs, m);
+    auto emit = [&](int k, double y) {
+      double val = y * a.scale;
+      if constexpr (PRE == PRE_V) val += (double)pre[k];
+      TS old = (TS)0;
+      if constexpr (PRE == PRE_OLD) old = pre[k];
+      buf_store(epilogue<TS>(a.epi, a.epi_c, val, old), rout, lane_off + (s + k) * row_bytes, 0);
+    };
+    if constexpr (SOLVE == SOLVE_CIRC) {
+      solve_chunk_circ<MODE, CS>(a, Kl, Kl + 4 * P * P, p, m, X0, X1, LG, lane, rhs, emit);
+    } else {
+#pragma unroll
+      for (int k = 0; k < CHUNK; ++k) emit(k, rhs[k]);
+    }
   }
 }
 
@@ -367,8 +510,8 @@ __device__ __forceinline__ void vzip(float4& r, const float4& o, F f) {
 // windows are in registers. Circulant solve: the carry exchange sits after
 // the tile.
 template <typename TS, int KIND, int SOLVE, int MODE, bool FULL>
-__global__ void __launch_bounds__(WG_MAX) k_line_x(LineArgs a, const TS* __restrict__ in,
-                                                   TS* __restrict__ out) {
+__global__ void __launch_bounds__(WG_MAX) __attribute__((amdgpu_waves_per_eu(PYR_X_WAVES_PER_EU, 8)))
+k_line_x(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
   constexpr int R = Halo<KIND>::R;
   constexpr bool PER = SOLVE == SOLVE_CIRC;
   using CS = typename SolveT<MODE>::type;
@@ -442,7 +585,8 @@ __global__ void __launch_bounds__(WG_MAX) k_line_x(LineArgs a, const TS* __restr
   if (SOLVE == SOLVE_CIRC) {
     // the first carry barrier also retires every read of the tile
     CS* X0 = reinterpret_cast<CS*>(smem + align16((size_t)LX * pitch * sizeof(TS)));
-    solve_chunk_circ<MODE, CS>(a, p, m, X0, X0 + 4 * a.P * LX, LX, l, rhs, emit);
+    solve_chunk_circ<MODE, CS>(a, kmat<MODE>(a, 0), kmat<MODE>(a, 1), p, m, X0, X0 + 4 * a.P * LX, LX, l,
+                               rhs, emit);
   } else {
 #pragma unroll
     for (int k = 0; k < CHUNK; ++k)
@@ -544,6 +688,8 @@ struct pyr_ctx {
   int periodic_solver = PYR_PSOLVE_CIRCULANT;
   int r32_solve = PYR_R32_SOLVE_REFINED;
   int x_lds_kb = 40;                        // LDS budget of one x-line workgroup
+  int yz_pipe = 1;                           // pipelined y/z kernels where the shape allows
+  int ncu = 256;                             // compute units of the device
   size_t npts() const { return (size_t)n[0] * n[1] * n[2]; }
 };
 
@@ -719,6 +865,42 @@ static int launch_yz(pyr_ctx* c, int pre, dim3 grid, dim3 block, size_t shm, con
   return 0;
 }
 
+// Pipelined y/z launch when the line shape allows it (see k_line_yz_pipe);
+// returns 1 if launched, 0 if the caller should use k_line_yz, -1 on error.
+template <typename TS, int KIND, int SOLVE, int MODE>
+static int launch_yz_pipe(pyr_ctx* c, int pre, const LineArgs& a, const TS* in, TS* out) {
+  if constexpr (SOLVE != SOLVE_CIRC && SOLVE != SOLVE_NONE) {
+    return 0;
+  } else {
+    using CS = typename SolveT<MODE>::type;
+    constexpr int LG = sizeof(TS) == 4 ? 32 : 16;
+    // whole waves (the copy loop strides by the wave count)
+    if (!c->yz_pipe || !a.periodic || a.N % CHUNK != 0 || (LG * a.P) % 64 != 0) return 0;
+    if (a.A % LG != 0 || a.nlines % LG != 0) return 0;
+    if ((uintptr_t)in % 16 != 0 || (a.stride * sizeof(TS)) % 16 != 0 || (a.B * sizeof(TS)) % 16 != 0) return 0;
+    constexpr int R = Halo<KIND>::R;
+    constexpr int RPI = 64 / (LG * (int)sizeof(TS) / 16);
+    // NT tile rows as in the kernel: N lines rows + 2R wrapped rows, in runs of RPI
+    const size_t tile_rows = (size_t)((a.N + 2 * R + RPI - 1) / RPI) * RPI;
+    const size_t tiles = align16(2 * tile_rows * LG * sizeof(TS));
+    const size_t shm = tiles + (2 * (size_t)4 * a.P * LG + 2 * (size_t)4 * a.P * a.P) * sizeof(CS);
+    if (shm > 80 * 1024) return 0;
+    const int per_cu = (int)((160 * 1024) / shm);
+    const int ngroups = a.nlines / LG;
+    const int nblocks = std::min(ngroups, std::max(1, c->ncu * per_cu));
+    auto k = k_line_yz_pipe<TS, KIND, SOLVE, MODE, LG, PRE_NONE>;
+    if (pre == PRE_V) {
+      if constexpr (KIND == K_R4) k = k_line_yz_pipe<TS, KIND, SOLVE, MODE, LG, PRE_V>;
+      else return 0;
+    } else if (pre == PRE_OLD) {
+      k = k_line_yz_pipe<TS, KIND, SOLVE, MODE, LG, PRE_OLD>;
+    }
+    if (set_lds(k, shm)) return -1;
+    hipLaunchKernelGGL(k, dim3(nblocks), dim3(LG, a.P), shm, c->stream, a, in, out);
+    return 1;
+  }
+}
+
 template <typename TS, int KIND, int SOLVE, int MODE, bool FULL>
 static int launch_typed(pyr_ctx* c, int dir, LineArgs a, const TS* in, TS* out) {
   using CS = typename SolveT<MODE>::type;
@@ -753,6 +935,17 @@ static int launch_typed(pyr_ctx* c, int dir, LineArgs a, const TS* in, TS* out) 
     const size_t shm = SOLVE == SOLVE_CIRC ? 2 * ex : (SOLVE == SOLVE_LU ? ex + stage : 0);
     const dim3 grid((a.nlines + ly - 1) / ly), block(ly, P);
     const int pre = a.addv ? PRE_V : ((a.epi == EPI_ADD || a.epi == EPI_MAXABS) ? PRE_OLD : PRE_NONE);
+    if constexpr (FULL) {
+      // a filter with an accumulating epilogue keeps the plain kernel
+      if (!(a.addv && pre == PRE_V && (a.epi == EPI_ADD || a.epi == EPI_MAXABS))) {
+        const int r = launch_yz_pipe<TS, KIND, SOLVE, MODE>(c, pre, a, in, out);
+        if (r < 0) return 1;
+        if (r > 0) {
+          HIPCHECK(hipGetLastError());
+          return 0;
+        }
+      }
+    }
     if (ly == 64) {
       if (launch_yz<TS, KIND, SOLVE, MODE, 64, FULL>(c, pre, grid, block, shm, a, in, out)) return 1;
     } else {
@@ -878,6 +1071,7 @@ static int setup_common(pyr_ctx** out, int nx, int ny, int nz, double x1, double
   if (!bc6) return fail("null bc array");
   std::unique_ptr<pyr_ctx> c(new pyr_ctx);
   c->n[0] = nx; c->n[1] = ny; c->n[2] = nz;
+  if (const char* e = std::getenv("PYRANDA_AMD_YZ_PIPE")) c->yz_pipe = std::atoi(e) != 0;
   if (const char* e = std::getenv("PYRANDA_AMD_X_LDS_KB")) {
     const int kb = std::atoi(e);
     if (kb >= 8 && kb <= 160) c->x_lds_kb = kb;
@@ -911,6 +1105,11 @@ static int setup_common(pyr_ctx** out, int nx, int ny, int nz, double x1, double
     return fail(std::string("setup: ") + e.what());
   }
   if (device) {
+    int dev = 0;
+    hipDeviceProp_t prop;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
+        prop.multiProcessorCount > 0)
+      c->ncu = prop.multiProcessorCount;
     for (int op = 0; op < NOPS; ++op)
       for (int d = 0; d < 3; ++d)
         if (upload_plan(c->plan[op][d], c->dplan[op][d])) return 1;

@@ -49,6 +49,9 @@ CASES = [
     ((24, 9, 12), ("SYMM", "SYMM", "NONE")),
     ((130, 70, 3), ("PERI", "NONE", "NONE")),
     ((300, 20, 10), ("PERI", "PERI", "NONE")),  # 10 chunks per x line
+    # periodic y/z lines of whole 32-row chunks in whole line groups: the
+    # pipelined y/z kernels (global->LDS copies of the next line group)
+    ((64, 128, 128), ("PERI", "PERI", "PERI")),
 ]
 
 
