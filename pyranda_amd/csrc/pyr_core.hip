@@ -873,7 +873,15 @@ static int launch_yz_pipe(pyr_ctx* c, int pre, const LineArgs& a, const TS* in, 
     return 0;
   } else {
     using CS = typename SolveT<MODE>::type;
-    constexpr int LG = sizeof(TS) == 4 ? 32 : 16;
+    constexpr int LG = 32;
+    // Measured on gfx950 at 256^3: with FP32 fields the pipelined kernel wins
+    // for the explicit filter (gbar y 54 -> 28 us) and the refined circulant
+    // solve (d1 y 51 -> 41 us), but not with accumulating epilogues; FP64
+    // fields leave too few waves per CU once two tiles fit in LDS.
+    if constexpr (sizeof(TS) != 4 || (SOLVE == SOLVE_CIRC && MODE != MODE_R32)) {
+      return 0;
+    } else {
+    if (pre == PRE_OLD) return 0;
     // whole waves (the copy loop strides by the wave count)
     if (!c->yz_pipe || !a.periodic || a.N % CHUNK != 0 || (LG * a.P) % 64 != 0) return 0;
     if (a.A % LG != 0 || a.nlines % LG != 0) return 0;
@@ -883,7 +891,8 @@ static int launch_yz_pipe(pyr_ctx* c, int pre, const LineArgs& a, const TS* in, 
     // NT tile rows as in the kernel: N lines rows + 2R wrapped rows, in runs of RPI
     const size_t tile_rows = (size_t)((a.N + 2 * R + RPI - 1) / RPI) * RPI;
     const size_t tiles = align16(2 * tile_rows * LG * sizeof(TS));
-    const size_t shm = tiles + (2 * (size_t)4 * a.P * LG + 2 * (size_t)4 * a.P * a.P) * sizeof(CS);
+    // carry exchange and carry rows only for the circulant solve
+    const size_t shm = tiles + (SOLVE == SOLVE_CIRC ? (2 * (size_t)4 * a.P * LG + 2 * (size_t)4 * a.P * a.P) * sizeof(CS) : 0);
     if (shm > 80 * 1024) return 0;
     const int per_cu = (int)((160 * 1024) / shm);
     const int ngroups = a.nlines / LG;
@@ -892,12 +901,11 @@ static int launch_yz_pipe(pyr_ctx* c, int pre, const LineArgs& a, const TS* in, 
     if (pre == PRE_V) {
       if constexpr (KIND == K_R4) k = k_line_yz_pipe<TS, KIND, SOLVE, MODE, LG, PRE_V>;
       else return 0;
-    } else if (pre == PRE_OLD) {
-      k = k_line_yz_pipe<TS, KIND, SOLVE, MODE, LG, PRE_OLD>;
     }
     if (set_lds(k, shm)) return -1;
     hipLaunchKernelGGL(k, dim3(nblocks), dim3(LG, a.P), shm, c->stream, a, in, out);
     return 1;
+    }
   }
 }
 

@@ -120,3 +120,109 @@ L = 1.0
 :tke: = :rho:*(:u:*:u: + :v:*:v: + :w:*:w:)
 :dt: = dt.courant(:u:,:v:,:w:,:cs:)
 """
+
+
+# Kelvin-Helmholtz shear layers (McNally et al.), 2-D doubly periodic, verbatim
+# from examples/KH.py:56-118 (EOM and IC strings are the reference's API);
+# the dictionaries are applied by pyrandaSim.EOM / setIC as in the script.
+KH_EOM = """
+# Primary Equations of motion here
+ddt(:rho:)  =  -ddx(:rho:*:u:)            - ddy(:rho:*:v:)
+ddt(:rhou:) =  -ddx(:rhou:*:u: - :tauxx:) - ddy(:rhou:*:v: - :tauxy:)
+ddt(:rhov:) =  -ddx(:rhov:*:u: - :tauxy:) - ddy(:rhov:*:v: - :tauyy:)
+ddt(:Et:)   =  -ddx( (:Et: - :tauxx:)*:u: - :tauxy:*:v:  - :tx:*:kappa: )  - ddy( (:Et: - :tauyy:)*:v: -:tauxy:*:u:- :ty:*:kappa: )
+# Conservative filter of the EoM
+:rho:       =  fbar( :rho:  )
+:rhou:      =  fbar( :rhou: )
+:rhov:      =  fbar( :rhov: )
+:Et:        =  fbar( :Et:   )
+# Update the primatives and enforce the EOS
+:u:         =  :rhou: / :rho:
+:v:         =  :rhov: / :rho:
+:p:         =  ( :Et: - .5*:rho:*(:u:*:u: + :v:*:v:) ) * ( gamma - 1.0 )
+# Artificial bulk viscosity
+:ux:        =  ddx(:u:)
+:vy:        =  ddy(:v:)
+:div:       =  :ux: + :vy:
+# Remaining cross derivatives
+:uy:        =  ddy(:u:)
+:vx:        =  ddx(:v:)
+:enst:      = sqrt( (:uy:-:vx:)**2 )
+:tke:       = :rho:*(:u:*:u: + :v:*:v: )
+:S:         = sqrt( :ux:*:ux: + :vy:*:vy:  + .5*((:uy:+:vx:)**2  ) )
+:mu:        =  gbar( abs(ring(:S:  )) ) * :rho: * 1.0e-4
+:beta:      =  gbar( abs(ring(:div:)) * :rho: )  * 7.0e-3
+:taudia:    =  (:beta:-2./3.*:mu:) *:div: - :p:
+:tauxx:     =  2.0*:mu:*:ux:   + :taudia:
+:tauyy:     =  2.0*:mu:*:vy:   + :taudia:
+:tauxy:     = :mu:*(:uy:+:vx:)
+:T:         = :p: / (:rho: * R0 )
+[:tx:,:ty:,:tz:] = grad(:T:)
+:kappa:     = gbar( ring(:T:)* :rho:*cv/(:T: * :dt: ) ) * 1.0e-3
+:cs:  = sqrt( :p: / :rho: * gamma )
+:dt: = dt.courant(:u:,:v:,:w:,:cs:)*1.0
+:dt: = numpy.minimum(:dt:,0.2 * dt.diff(:beta:,:rho:))
+:dt: = numpy.minimum(:dt:,0.2 * dt.diff(:mu:,:rho:))
+"""
+
+KH_IC = """
+Um = (u1-u2)/2.0
+rhoM = (rho1-rho2)/2.0
+:u: =                     u1-Um*exp( -(meshy-.75)/L)
+:u: = where( meshy < .75, u2+Um*exp( -(.75-meshy)/L) , :u: )
+:u: = where( meshy < .50, u2+Um*exp( (-meshy+.25)/L) , :u: )
+:u: = where( meshy < .25, u1-Um*exp(  (meshy-.25)/L) , :u: )
+:v: = 0.01*sin( 4.*pi*meshx ) + Vmean
+:rho: =                     rho1-rhoM*exp( -(meshy-.75)/L)
+:rho: = where( meshy < .75, rho2+rhoM*exp( -(.75-meshy)/L) , :rho: )
+:rho: = where( meshy < .50, rho2+rhoM*exp( (-meshy+.25)/L) , :rho: )
+:rho: = where( meshy < .25, rho1-rhoM*exp(  (meshy-.25)/L) , :rho: )
+:p: += p0
+# Form conserved variables
+:Et: = :p:/( gamma - 1.0 ) + .5*:rho:*(:u:*:u: + :v:*:v:)
+:rhou: = :rho:*:u:
+:rhov: = :rho:*:v:
+:cs:  = sqrt( :p: / :rho: * gamma )
+:dt: = dt.courant(:u:,:v:,:w:,:cs:)*.1
+"""
+
+
+def kh_setup(pr, nx, ny=None, vmean=0.0, precision="fp64", name="KelvinHelmholtz", device=None):
+    """A KH.py simulation on an nx x ny grid (2-D, doubly periodic, the
+    script's unit box with spacing 1/n), returned after setIC. The script
+    uses a square grid (KH.py:52-55); ny != nx gives the uneven grids of an
+    ensemble."""
+    ny = nx if ny is None else ny
+    gamma, R = 5. / 3., 1.0
+    cp = R / (1.0 - 1.0 / gamma)
+    cv = cp - R
+    mesh = """
+    xdom = (0.0, LX,  NX, periodic=True)
+    ydom = (0.0, LY,  NY, periodic=True)
+    """.replace("LX", str(1.0 * (nx - 1.0) / nx)).replace("LY", str(1.0 * (ny - 1.0) / ny)) \
+       .replace("NX", str(nx)).replace("NY", str(ny))
+    kw = {"silent": True, "precision": precision}
+    if device is not None:
+        kw["device"] = device
+    ss = pr.pyrandaSim(name, mesh, **kw)
+    ss.addPackage(pr.pyrandaTimestep(ss))
+    ss.EOM(KH_EOM, {"gamma": gamma, "R0": R, "cv": cv})
+    ss.setIC(KH_IC, {"gamma": gamma, "u1": 0.5, "u2": -0.5, "p0": 2.5, "rho1": 1.0,
+                     "rho2": 2.0, "L": 0.025, "Vmean": vmean})
+    return ss
+
+
+def kh_run(ss, t_final=1.5, dt_max=1.0, max_steps=None):
+    """The KH.py time loop (KH.py:130-151): dt grows by at most 10% a step."""
+    time = 0.0
+    dt = float(ss.var("dt").data)
+    steps = 0
+    while t_final > time:
+        time = ss.rk4(time, dt)
+        dt = min(float(ss.variables["dt"].data), 1.1 * dt)
+        dt = min(dt_max, dt)
+        dt = min(dt, (t_final - time))
+        steps += 1
+        if max_steps is not None and steps >= max_steps:
+            break
+    return time, steps

@@ -241,3 +241,24 @@ def test_euler_2d_curve(pr, npts):
     _say("euler-2d", npts, "max rel diff", err)
     # the reference harness accepts 1e-4 (tests/testObj.py:52-58)
     assert err <= 1e-9
+
+
+def test_kh_2d_curve(pr):
+    """examples/KH.py at 64^2 (doubly periodic shear layers, LAD with the
+    kappa line reading the dt variable, dt from dt.courant/dt.diff in every
+    stage) against the reference-held curve tests/baselines/
+    KelvinHelmholtzKH-2d-64.dat: rho(x, 3ny/4) at t = 1.5 after 249 steps."""
+    import numpy
+    from pyranda_amd.workloads import kh_run, kh_setup
+    ss = kh_setup(pr, 64, name="KelvinHelmholtzKH-2d-64")
+    time, steps = kh_run(ss)
+    ny = ss.PyMPI.ny
+    xx = ss.PyMPI.zbar(ss.mesh.coords[0].data)
+    v = ss.PyMPI.zbar(ss.variables["rho"].data)
+    base = numpy.loadtxt(os.path.join(os.path.dirname(GOLDEN), "KH-2d-64.dat"))
+    assert numpy.allclose(xx[:, 3 * ny // 4], base[0], rtol=0, atol=1e-15)
+    err = numpy.abs(v[:, 3 * ny // 4] - base[1]).max() / numpy.abs(base[1]).max()
+    _say("kh-2d-64 steps", steps, "max rel diff", err)
+    assert steps == 249
+    # the reference harness accepts 1e-4 (tests/testObj.py:52-58)
+    assert err <= 1e-9
