@@ -238,6 +238,61 @@ def ops_bench(n, steps, warmup, dev, only=None, solver="circulant", r32_solve="r
                       "best": best[0]}), flush=True)
 
 
+def ensemble_bench(args, dev):
+    """KH ensemble: independent 2-D shear layers (examples/KH.py) on uneven
+    grids, FP32-refined, advanced member by member with eager launches and
+    then as one captured HIP graph per time step for all members."""
+    import torch
+    import pyranda_amd as pr
+    from pyranda_amd.ensemble import pyrandaEnsemble
+    from pyranda_amd.workloads import ensemble_sizes, kh_advance, kh_ensemble
+    sizes = ensemble_sizes(args.members, 128, 512)
+    npts = sum(nx * ny for nx, ny in sizes)
+    print("[ensemble] %d members, %d points, sizes %s" % (len(sizes), npts, sizes), file=sys.stderr, flush=True)
+    sims, state = kh_ensemble(pr, sizes, precision=args.ens_precision, device=dev)
+    ens = pyrandaEnsemble(sims, kh_advance, state)
+    ens.step_eager(max(1, args.warmup))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ens.step_eager(args.steps)
+    torch.cuda.synchronize()
+    eager = time.perf_counter() - t0
+    print("[ensemble] eager %.2f ms/step" % (1e3 * eager / args.steps), file=sys.stderr, flush=True)
+    out = {"eager": {"ms_per_step": 1e3 * eager / args.steps, "value": npts * args.steps / eager}}
+    for mode in ("concurrent", "serial"):
+        t0 = time.perf_counter()
+        ens.capture(concurrent=(mode == "concurrent"))
+        torch.cuda.synchronize()
+        cap = time.perf_counter() - t0
+        ens.step(max(1, args.warmup))
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ens.step(args.steps)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        out["graph_" + mode] = {"ms_per_step": 1e3 * el / args.steps, "value": npts * args.steps / el,
+                                "capture_s": cap}
+        print("[ensemble] graph (%s) %.2f ms/step, capture %.1f s" % (mode, 1e3 * el / args.steps, cap),
+              file=sys.stderr, flush=True)
+        ens.graph = None
+    best = max(("graph_concurrent", "graph_serial"), key=lambda k: out[k]["value"])
+    line = {
+        "metric": "KH ensemble point-updates/s (%d independent 2-D Euler+LAD layers, uneven grids)" % len(sizes),
+        "value": out[best]["value"], "unit": "point-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": out[best]["ms_per_step"], "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 fields and sweeps, f64 residual refinement, f64 RK4 state"
+        if args.ens_precision == "fp32" else args.ens_precision,
+        "data": "synthetic (KH.py initial condition on each member grid)",
+        "config": {"workload": "kh_ensemble", "members": len(sizes), "points": npts,
+                   "sizes": "log-uniform per axis in [128, 512], multiples of 8, seed 2024",
+                   "graph": best}, "modes": out,
+        "speedup_graph_over_eager": out[best]["value"] / out["eager"]["value"],
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -256,6 +311,10 @@ def main():
                     help="--ops: arithmetic of the solves on FP32 fields")
     ap.add_argument("--precision", default="fp64,fp32,fp32-fp64solve",
                     help="comma-separated pyrandaSim precisions to time (the headline is fp32)")
+    ap.add_argument("--ensemble", action="store_true",
+                    help="KH ensemble (independent 2-D layers on uneven grids, one HIP graph per step)")
+    ap.add_argument("--members", type=int, default=64, help="--ensemble: number of members")
+    ap.add_argument("--ens-precision", default="fp32", choices=["fp64", "fp32", "fp32-fp64solve"])
     ap.add_argument("--only", default=None,
                     help="--ops: comma-separated name prefixes to run (e.g. d1_x_f64,copy_f64)")
     args = ap.parse_args()
@@ -273,6 +332,8 @@ def main():
     n = args.n
     if args.ops:
         return ops_bench(n, args.steps, args.warmup, dev, args.only, args.solver, args.r32_solve)
+    if args.ensemble:
+        return ensemble_bench(args, dev)
     gamma = 1.4
     p0 = 1.0 / (gamma * 0.1 ** 2)  # Mach 0.1 on the unit velocity
 

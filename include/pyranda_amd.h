@@ -132,6 +132,13 @@ int pyr_apply(pyr_ctx* ctx, int op, int dir, int prec, const void* in, void* out
  * F, U_in, U_out have the field precision; n = number of points. */
 int pyr_rk_update(pyr_ctx* ctx, int prec, long long n, const void* F, double* PHI,
                   const void* U_in, void* U_out, double* U64, double dt, double A, double B);
+/* The same update with dt read from device memory (one FP64 value) when the
+ * kernel runs, so a time step whose dt comes from device reductions needs no
+ * host round trip and can be captured in a HIP graph and replayed. No
+ * counterpart in the reference (its dt is a host float, pyranda.py:758). */
+int pyr_rk_update_dt(pyr_ctx* ctx, int prec, long long n, const void* F, double* PHI,
+                     const void* U_in, void* U_out, double* U64, const double* dt, double A,
+                     double B);
 
 /* --- fused elementwise kernels for EOM pointwise algebra ----------------
  * No counterpart in the reference (its pointwise algebra is numpy, pyrandaEq.py:39).

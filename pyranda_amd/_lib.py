@@ -37,6 +37,8 @@ SIGNATURES = {
     "pyr_host_emulate": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp]),
     "pyr_rk_update": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong, _vp, _vp, _vp, _vp, _vp,
                                      ctypes.c_double, ctypes.c_double, ctypes.c_double]),
+    "pyr_rk_update_dt": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong, _vp, _vp, _vp, _vp, _vp,
+                                        _vp, ctypes.c_double, ctypes.c_double]),
     "pyr_pw_compile": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "pyr_pw_check_source": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p]),
     "pyr_pw_launch": (ctypes.c_int, [_vp, _vp, ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(_vp),

@@ -637,11 +637,13 @@ __global__ void k_null(long long n, int copy, int epi, double c, const TS* __res
 
 // RK4 stage update (pyranda.py:800-804), one pass over F, PHI and U. With
 // A == 0 (the first stage) A*PHI is zero and PHI is not read, so it needs no
-// clearing before the step.
+// clearing before the step. dtp (if not null) holds dt in device memory.
 template <typename TS>
 __global__ void k_rk_update(long long n, const TS* __restrict__ F, double* __restrict__ PHI,
                             const TS* __restrict__ Uin, TS* __restrict__ Uout,
-                            double* __restrict__ U64, double dt, double A, double B) {
+                            double* __restrict__ U64, const double* __restrict__ dtp, double dt,
+                            double A, double B) {
+  if (dtp) dt = *dtp;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
     const double tmp1 = A == 0.0 ? 0.0 : A * PHI[i];
@@ -1436,8 +1438,8 @@ int pyr_divergence_r32(pyr_ctx* c, const float* fx, const float* fy, const float
   return sum3(c, OP_D1, PYR_R32, fx, fy, fz, o);
 }
 
-int pyr_rk_update(pyr_ctx* c, int prec, long long n, const void* F, double* PHI, const void* Uin,
-                  void* Uout, double* U64, double dt, double A, double B) {
+static int rk_update(pyr_ctx* c, int prec, long long n, const void* F, double* PHI, const void* Uin,
+                     void* Uout, double* U64, const double* dtp, double dt, double A, double B) {
   if (check_ctx(c)) return 1;
   if (!F || !PHI || !Uout || (!Uin && !U64)) return fail("null pointer to pyr_rk_update");
   if (n <= 0) return 0;
@@ -1445,14 +1447,25 @@ int pyr_rk_update(pyr_ctx* c, int prec, long long n, const void* F, double* PHI,
   if (blocks > 8192) blocks = 8192;
   if (prec == PYR_F64)
     hipLaunchKernelGGL((k_rk_update<double>), dim3((unsigned)blocks), dim3(256), 0, c->stream, n,
-                       (const double*)F, PHI, (const double*)Uin, (double*)Uout, U64, dt, A, B);
+                       (const double*)F, PHI, (const double*)Uin, (double*)Uout, U64, dtp, dt, A, B);
   else if (prec == PYR_R32)
     hipLaunchKernelGGL((k_rk_update<float>), dim3((unsigned)blocks), dim3(256), 0, c->stream, n,
-                       (const float*)F, PHI, (const float*)Uin, (float*)Uout, U64, dt, A, B);
+                       (const float*)F, PHI, (const float*)Uin, (float*)Uout, U64, dtp, dt, A, B);
   else
     return fail("bad precision");
   HIPCHECK(hipGetLastError());
   return 0;
+}
+
+int pyr_rk_update(pyr_ctx* c, int prec, long long n, const void* F, double* PHI, const void* Uin,
+                  void* Uout, double* U64, double dt, double A, double B) {
+  return rk_update(c, prec, n, F, PHI, Uin, Uout, U64, nullptr, dt, A, B);
+}
+
+int pyr_rk_update_dt(pyr_ctx* c, int prec, long long n, const void* F, double* PHI, const void* Uin,
+                     void* Uout, double* U64, const double* dt, double A, double B) {
+  if (!dt) return fail("null dt pointer to pyr_rk_update_dt");
+  return rk_update(c, prec, n, F, PHI, Uin, Uout, U64, dt, 0.0, A, B);
 }
 
 int pyr_host_weights(int op, double* ali, double* ari, double* alb1, double* alb2, double* arb1,

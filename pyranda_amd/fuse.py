@@ -129,6 +129,11 @@ class _Gen:
         raise AssertionError(name)
 
 
+def _device_scalar(v):
+    return (torch.is_tensor(v) and v.dim() == 0 and v.is_cuda
+            and v.dtype in (torch.float64, torch.float32))
+
+
 def _field_ok(t, shape):
     if not torch.is_tensor(t) or t.dtype not in (torch.float64, torch.float32) or not t.is_cuda:
         return False
@@ -166,6 +171,10 @@ class FusedSubtree:
             if s == "S":
                 decl.append("  const double x%d = a.s[%d];" % (i, isc))
                 isc += 1
+            elif s in ("D64", "D32"):
+                ct = "double" if s == "D64" else "float"
+                decl.append("  const double x%d = (double)*(const %s*)a.p[%d];" % (i, ct, ip))
+                ip += 1
             else:
                 ct = "double" if s == "T64" else "float"
                 decl.append("  const %s* __restrict__ p%d = (const %s*)a.p[%d];" % (ct, i, ct, ip))
@@ -176,7 +185,7 @@ class FusedSubtree:
         lines.append("  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;"
                      " i += (long long)gridDim.x * blockDim.x) {")
         for i, s in enumerate(sig):
-            if s != "S":
+            if s in ("T64", "T32"):
                 lines.append("    const double x%d = (double)p%d[i];" % (i, i))
         lines.append("    out[i] = (%s)(%s);" % (ot, self.body))
         lines.append("  }")
@@ -198,29 +207,38 @@ class FusedSubtree:
 
     def __call__(self, shape, *vals):
         sig = []
-        fields = []
+        fields = []     # pointer operands: fields and device scalars, in order
         scalars = []
         any64 = False
+        nfield = 0
+        dev = None
         for v in vals:
-            if torch.is_tensor(v) and v.dim() == 0:
+            if torch.is_tensor(v) and v.dim() == 0 and not _device_scalar(v):
                 v = v.item()
             if isinstance(v, (int, float)) or (hasattr(v, "dtype") and not torch.is_tensor(v)
                                                 and getattr(v, "shape", None) == ()):
                 sig.append("S")
                 scalars.append(float(v))
+            elif _device_scalar(v):
+                # a 0-d device value (a reduction result kept on the GPU):
+                # read once by the kernel, no host round trip; like a Python
+                # scalar it does not decide the result type
+                sig.append("D64" if v.dtype == torch.float64 else "D32")
+                fields.append(v)
             elif _field_ok(v, shape):
                 sig.append("T64" if v.dtype == torch.float64 else "T32")
                 any64 |= v.dtype == torch.float64
                 fields.append(v)
+                nfield += 1
+                dev = v.device
             else:
                 return self._py(*vals)
-        if not fields or self.is_bool:
+        if not nfield or self.is_bool:
             return self._py(*vals)
         if len(fields) + 1 > _lib.PW_MAX_PTRS or len(scalars) > _lib.PW_MAX_SCALARS:
             return self._py(*vals)
         out_dt = "f64" if any64 else "f32"
         fn = self.kernel(tuple(sig), out_dt)
-        dev = fields[0].device
         nx, ny, nz = shape
         out = torch.empty((nz, ny, nx), dtype=torch.float64 if any64 else torch.float32,
                           device=dev).permute(2, 1, 0)
@@ -363,7 +381,7 @@ class FusedReduce:
         L.append("}")
         return name, "\n".join(L)
 
-    def __call__(self, shape, fields, scalars):
+    def _launch(self, n, fields, scalars, blocks):
         sig = tuple("T64" if f.dtype == torch.float64 else "T32" for f in fields)
         fn = self._kernels.get(sig)
         if fn is None:
@@ -374,17 +392,35 @@ class FusedReduce:
                 raise _lib.PyrandaAMDError(L.pyr_pw_last_error().decode())
             fn = self._kernels[sig] = h
         dev = fields[0].device
-        n = shape[0] * shape[1] * shape[2]
-        blocks = max(1, min(_RED_BLOCKS, (n + 1023) // 1024))
         part = torch.empty(blocks, dtype=torch.float64, device=dev)
         ptrs = (ctypes.c_void_p * (len(fields) + 1))(*[f.data_ptr() for f in fields], part.data_ptr())
         sc = (ctypes.c_double * max(1, len(scalars)))(*[float(x) for x in scalars])
         stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
         _lib.check_pw(_lib.lib().pyr_pw_launch_blocks(fn, stream, n, blocks, len(fields) + 1, ptrs,
                                                       len(scalars), sc))
+        return part
+
+    def __call__(self, shape, fields, scalars, device=False):
+        """The reduction as a Python float, or with device=True as a 0-d FP64
+        device tensor (a second one-workgroup pass over the partials; no host
+        round trip, so it can sit inside a captured HIP graph)."""
+        n = shape[0] * shape[1] * shape[2]
+        blocks = max(1, min(_RED_BLOCKS, (n + 1023) // 1024))
+        part = self._launch(n, fields, scalars, blocks)
+        if device:
+            return _final(self.op)._launch(blocks, [part], [], 1).reshape(())
         import numpy as _np
         vals = part.cpu().numpy()
         return float(_np.max(vals) if self.op == "max" else _np.min(vals))
+
+
+_FINALS = {}
+
+
+def _final(op):
+    if op not in _FINALS:
+        _FINALS[op] = FusedReduce(op, "x0", 1, 0, "final")
+    return _FINALS[op]
 
 
 def fields_ok(shape, *vals):
@@ -470,7 +506,7 @@ class PointwiseRun:
         out = []
         for ins, ints in self.deps:
             kinds = [sig[i] for i in ins] + ["T64" if out[j] == "f64" else "T32" for j in ints]
-            fields = [k for k in kinds if k != "S"]
+            fields = [k for k in kinds if k[0] == "T"]
             if not fields:
                 return None
             out.append("f64" if "T64" in fields else "f32")
@@ -485,6 +521,10 @@ class PointwiseRun:
             if t == "S":
                 L.append("  const double x%d = a.s[%d];" % (i, isc))
                 isc += 1
+            elif t in ("D64", "D32"):
+                ct = "double" if t == "D64" else "float"
+                L.append("  const double x%d = (double)*(const %s*)a.p[%d];" % (i, ct, ip))
+                ip += 1
             else:
                 ct = "double" if t == "T64" else "float"
                 L.append("  const %s* __restrict__ p%d = (const %s*)a.p[%d];" % (ct, i, ct, ip))
@@ -495,7 +535,7 @@ class PointwiseRun:
         L.append("  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;"
                  " i += (long long)gridDim.x * blockDim.x) {")
         for i, t in enumerate(sig):
-            if t != "S":
+            if t in ("T64", "T32"):
                 L.append("    const double x%d = (double)p%d[i];" % (i, i))
         for j, t in enumerate(types):
             ot = "double" if t == "f64" else "float"
@@ -511,21 +551,26 @@ class PointwiseRun:
         (the caller then evaluates the lines one by one)."""
         shape = tuple(sim.mesh.shape)
         sig, fields, scalars = [], [], []
+        dev = None
         for nm in self.inputs:
             v = sim.variables[nm].data
-            if torch.is_tensor(v) and v.dim() == 0:
+            if torch.is_tensor(v) and v.dim() == 0 and not _device_scalar(v):
                 v = v.item()
             if isinstance(v, (int, float)) and not isinstance(v, bool):
                 sig.append("S")
                 scalars.append(float(v))
+            elif _device_scalar(v):
+                sig.append("D64" if v.dtype == torch.float64 else "D32")
+                fields.append(v)
             elif _field_ok(v, shape):
                 sig.append("T64" if v.dtype == torch.float64 else "T32")
                 fields.append(v)
+                dev = v.device
             else:
                 return False
         sig = tuple(sig)
         types = self._types(sig)
-        if types is None or not fields:
+        if types is None or dev is None:
             return False
         if len(fields) + len(types) > _lib.PW_MAX_PTRS or len(scalars) > _lib.PW_MAX_SCALARS:
             return False
@@ -538,7 +583,6 @@ class PointwiseRun:
             if L.pyr_pw_compile(src.encode(), name.encode(), ctypes.byref(h)) != 0:
                 raise _lib.PyrandaAMDError(L.pyr_pw_last_error().decode())
             fn = self._kernels[key] = h
-        dev = fields[0].device
         nx, ny, nz = shape
         outs = [torch.empty((nz, ny, nx), dtype=torch.float64 if t == "f64" else torch.float32,
                             device=dev).permute(2, 1, 0) for t in types]

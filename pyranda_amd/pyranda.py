@@ -14,6 +14,12 @@ FP32 sweeps plus one FP64-residual refinement; the RK4 state accumulation
 operators' right-hand sides and solves in FP64 registers (one rounding to
 FP32 per result): on gfx950 FP32 and FP64 vector FMAs issue at the same rate,
 so this keeps the halved memory traffic without the refinement's second solve.
+
+`device_scalars=True` keeps the grid reductions (dt.courant, dt.diff, sum,
+max, min) on the GPU as 0-d FP64 tensors instead of Python floats, and rk4
+accepts such a tensor as dt: a time step then makes no host round trip and
+can be captured in a HIP graph (pyranda_amd.ensemble). Values are the same
+as with host scalars, bit for bit.
 """
 import math
 import random
@@ -42,7 +48,8 @@ RK_ETA = [494393426753. / 4806282396855., 4702696611523. / 9636871101405.,
 
 class pyrandaSim:
 
-    def __init__(self, name, meshOptions, silent=False, precision="fp64", device="cuda"):
+    def __init__(self, name, meshOptions, silent=False, precision="fp64", device="cuda",
+                 device_scalars=False):
         self.name = name
         self.silent = silent
         self.mesh = pyrandaMesh()
@@ -77,6 +84,7 @@ class pyrandaSim:
         self.nPDE = self.nALG = self.nELP = 0
         self.mesh.dims = opts["dim"]
         self.PyMPI = pyrandaMPI(self.mesh, comm=opts.get("comm"), dtype=self.dtype, device=device)
+        self.PyMPI.device_scalars = bool(device_scalars)
         if precision == "fp32-fp64solve":
             self.PyMPI.ctx.set_r32_solve("fp64")
         self.mesh.PyMPI = self.PyMPI
@@ -242,7 +250,9 @@ class pyrandaSim:
             if name.startswith("mesh"):
                 continue
             d = var.data
-            if torch.is_tensor(d) and d.is_floating_point() and d.dtype != self.dtype:
+            # fields take the run's dtype; 0-d device scalars (reductions kept
+            # on the GPU) stay FP64 like the host floats they stand for
+            if torch.is_tensor(d) and d.dim() > 0 and d.is_floating_point() and d.dtype != self.dtype:
                 var.data = d.to(self.dtype)
 
     def emptyScalar(self, val=0.0):
@@ -353,9 +363,17 @@ class pyrandaSim:
         ctx = self.PyMPI.ctx
         ctx._stream()
         shadow = self._shadow[U].data_ptr() if fp32 and self._shadow[U] is not None else None
-        _lib.check(ctx._L.pyr_rk_update(ctx._h, _lib.R32 if fp32 else _lib.F64, self.npts,
-                                        F.data_ptr(), PHI[U].data_ptr(), u.data_ptr(), out.data_ptr(),
-                                        shadow, float(dt), float(A), float(B)))
+        prec = _lib.R32 if fp32 else _lib.F64
+        if torch.is_tensor(dt):
+            # dt in device memory (device_scalars): read by the kernel
+            dt64 = dt.to(device=u.device, dtype=torch.float64).reshape(())
+            _lib.check(ctx._L.pyr_rk_update_dt(ctx._h, prec, self.npts, F.data_ptr(), PHI[U].data_ptr(),
+                                               u.data_ptr(), out.data_ptr(), shadow, dt64.data_ptr(),
+                                               float(A), float(B)))
+        else:
+            _lib.check(ctx._L.pyr_rk_update(ctx._h, prec, self.npts, F.data_ptr(), PHI[U].data_ptr(),
+                                            u.data_ptr(), out.data_ptr(), shadow, float(dt), float(A),
+                                            float(B)))
         return out
 
     def step(self, nsteps=1):

@@ -6,7 +6,8 @@ the whole block is one rank's share, so this class keeps the same attribute
 and method names (PyMPI.der.ddx, PyMPI.sum3D, emptyScalar, x1proc, ...) with
 fields resident in HBM as torch tensors and operators going through
 libpyranda_amd.so. Reductions (sum3D/max3D/min3D, pyrandaMPI.py:307-326)
-return Python floats, as the allreduce does.
+return Python floats, as the allreduce does, or 0-d device tensors when
+`device_scalars` is set (no host round trip).
 """
 import torch
 
@@ -103,6 +104,7 @@ class pyrandaMPI:
         self.ynproc = self.y1proc
         self.z1proc = (not self.periodic[2]) and self.nz > 1
         self.znproc = self.z1proc
+        self.device_scalars = False
         self.der = _Der(self.ctx)
         self.gfil = _GFil(self.ctx)
         self.fil = _SFil(self.ctx)
@@ -119,9 +121,12 @@ class pyrandaMPI:
         """Operator tables are per context; nothing global to switch."""
 
     # -- reductions -----------------------------------------------------------
+    def _scalar(self, t):
+        return t if self.device_scalars else t.item()
+
     def sum3D(self, data):
         if torch.is_tensor(data):
-            return torch.sum(self._flat(data), dtype=torch.float64).item()
+            return self._scalar(torch.sum(self._flat(data), dtype=torch.float64))
         import numpy as np
         return float(np.sum(data))
 
@@ -134,13 +139,13 @@ class pyrandaMPI:
 
     def max3D(self, data):
         if torch.is_tensor(data):
-            return torch.max(self._flat(data)).item()
+            return self._scalar(torch.max(self._flat(data)))
         import numpy as np
         return float(np.max(data))
 
     def min3D(self, data):
         if torch.is_tensor(data):
-            return torch.min(self._flat(data)).item()
+            return self._scalar(torch.min(self._flat(data)))
         import numpy as np
         return float(np.min(data))
 

@@ -2,8 +2,11 @@
 
 courant: 1/max(|u|/dx + |v|/dy + |w|/dz + |c|/GridLen) over the grid;
 diff: min(rho*GridLen^2 / max(1e-12, bulk)). Both are device reductions
-returning Python floats, as the reference's MPI allreduce does.
+returning Python floats, as the reference's MPI allreduce does, or 0-d FP64
+device tensors when the simulation keeps its scalars on the GPU.
 """
+import torch
+
 from . import fuse
 from .devnumpy import numpy
 from .pyrandaPackage import pyrandaPackage
@@ -34,7 +37,10 @@ class pyrandaTimestep(pyrandaPackage):
     def courant(self, u, v, w, c):
         shape = self.pyranda.mesh.shape
         if fuse.fields_ok(shape, u, v, w, c):
-            return 1.0 / _COURANT(shape, [u, v, w, c], [self.dx, self.dy, self.dz, self.GridLen])
+            r = _COURANT(shape, [u, v, w, c], [self.dx, self.dy, self.dz, self.GridLen],
+                         device=self.pyranda.PyMPI.device_scalars)
+            # 1/x on the device is the correctly rounded quotient, as on the host
+            return r.reciprocal() if isinstance(r, torch.Tensor) else 1.0 / r
         vrate = numpy.abs(u) / self.dx + numpy.abs(v) / self.dy + numpy.abs(w) / self.dz
         crate = numpy.abs(c) / self.GridLen
         return 1.0 / self.pyranda.PyMPI.max3D(vrate + crate)
@@ -43,7 +49,7 @@ class pyrandaTimestep(pyrandaPackage):
         delta = self.GridLen
         shape = self.pyranda.mesh.shape
         if fuse.fields_ok(shape, bulk, density):
-            return _DIFF(shape, [bulk, density], [delta])
+            return _DIFF(shape, [bulk, density], [delta], device=self.pyranda.PyMPI.device_scalars)
         drate = density * delta * delta / numpy.maximum(1.0e-12, bulk)
         return self.pyranda.PyMPI.min3D(drate)
 

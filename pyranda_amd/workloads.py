@@ -187,7 +187,8 @@ rhoM = (rho1-rho2)/2.0
 """
 
 
-def kh_setup(pr, nx, ny=None, vmean=0.0, precision="fp64", name="KelvinHelmholtz", device=None):
+def kh_setup(pr, nx, ny=None, vmean=0.0, precision="fp64", name="KelvinHelmholtz", device=None,
+             device_scalars=False):
     """A KH.py simulation on an nx x ny grid (2-D, doubly periodic, the
     script's unit box with spacing 1/n), returned after setIC. The script
     uses a square grid (KH.py:52-55); ny != nx gives the uneven grids of an
@@ -201,7 +202,7 @@ def kh_setup(pr, nx, ny=None, vmean=0.0, precision="fp64", name="KelvinHelmholtz
     ydom = (0.0, LY,  NY, periodic=True)
     """.replace("LX", str(1.0 * (nx - 1.0) / nx)).replace("LY", str(1.0 * (ny - 1.0) / ny)) \
        .replace("NX", str(nx)).replace("NY", str(ny))
-    kw = {"silent": True, "precision": precision}
+    kw = {"silent": True, "precision": precision, "device_scalars": device_scalars}
     if device is not None:
         kw["device"] = device
     ss = pr.pyrandaSim(name, mesh, **kw)
@@ -226,3 +227,41 @@ def kh_run(ss, t_final=1.5, dt_max=1.0, max_steps=None):
         if max_steps is not None and steps >= max_steps:
             break
     return time, steps
+
+
+def kh_advance(ss, st, dt_max=1.0):
+    """One step of the KH.py loop with time and dt as 0-d device tensors
+    (a simulation built with device_scalars=True): rk4, then dt grows by at
+    most 10% and stays below dt_max. The t_final clamp of the script is left
+    to the caller, which runs a fixed number of steps."""
+    dt = st["dt"]
+    st["time"] = ss.rk4(st["time"], dt)
+    nd = torch.minimum(ss.variables["dt"].data, 1.1 * dt)
+    st["dt"] = torch.clamp(nd, max=dt_max)
+
+
+def ensemble_sizes(n_members=64, lo=128, hi=512, seed=2024):
+    """Grid sizes (nx, ny) of an ensemble: each axis log-uniform in [lo, hi],
+    rounded to a multiple of 8, from a fixed seed."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_members):
+        nx, ny = (int(8 * round(math.exp(rng.uniform(math.log(lo), math.log(hi))) / 8)) for _ in range(2))
+        out.append((min(max(nx, lo), hi), min(max(ny, lo), hi)))
+    return out
+
+
+def kh_ensemble(pr, sizes, precision="fp32", device=None):
+    """KH members on the given grids, each with its own seeded-free IC (the
+    script's), plus their initial driver state {time, dt}."""
+    sims, state = [], []
+    for i, (nx, ny) in enumerate(sizes):
+        ss = kh_setup(pr, nx, ny, precision=precision, name="KH-member-%d" % i, device=device,
+                      device_scalars=True)
+        dt = ss.var("dt").data
+        dt = dt if torch.is_tensor(dt) else torch.tensor(float(dt), dtype=torch.float64,
+                                                         device=ss.PyMPI.device)
+        sims.append(ss)
+        state.append({"time": torch.zeros((), dtype=torch.float64, device=ss.PyMPI.device),
+                      "dt": dt.to(torch.float64).clone()})
+    return sims, state

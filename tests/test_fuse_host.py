@@ -181,3 +181,31 @@ def test_run_and_reduction_kernels_compile(amd):
         for t in ("T32", "T64"):
             name, code = red._source(tuple(t for _ in range(nf)))
             assert L.pyr_pw_check_source(code.encode(), name.encode()) == 0, L.pyr_pw_last_error().decode()
+
+
+def test_device_scalar_operands_compile(amd):
+    """0-d device values (reductions kept on the GPU) enter fused kernels as
+    pointers read once; they do not decide the result type (like Python
+    scalars), and the kernels compile for gfx950."""
+    import ast
+    L = amd.lib()
+    src = ("self.ring(self.variables['T'].data) * self.variables['rho'].data * 1.5"
+           " / (self.variables['T'].data * self.variables['dt'].data)")
+    node = ast.parse(src, mode="eval").body
+    sub = fuse.FusedSubtree(node, "kappa")
+    assert sub.nleaves == 4
+    sig = ("T32", "T32", "T32", "D64")
+    name, code = sub._source(sig, "f32")
+    assert "(double)*(const double*)a.p[3]" in code and "float* __restrict__ out = (float*)a.p[4]" in code
+    assert L.pyr_pw_check_source(code.encode(), name.encode()) == 0, L.pyr_pw_last_error().decode()
+    run = fuse.PointwiseRun([("a", "self.variables['x'].data * self.variables['dt'].data"),
+                             ("b", "self.variables['a'].data + self.variables['x'].data")])
+    sig = ("T32", "D64")
+    types_ = run._types(sig)
+    assert types_ == ["f32", "f32"]
+    name, code = run._source(sig, types_)
+    assert L.pyr_pw_check_source(code.encode(), name.encode()) == 0, L.pyr_pw_last_error().decode()
+    # the one-workgroup final pass of a device reduction
+    fin = fuse._final("max")
+    name, code = fin._source(("T64",))
+    assert L.pyr_pw_check_source(code.encode(), name.encode()) == 0, L.pyr_pw_last_error().decode()
