@@ -293,6 +293,95 @@ def ensemble_bench(args, dev):
     return 0
 
 
+def closure_bench(args, dev):
+    """Learned LAD closure on the 2-D Richtmyer-Meshkov case (examples/RM3D.py,
+    2-D branch): snapshots of the plain run train the BF16-MFMA MLP; then the
+    MLP's evaluation of beta and adiff is timed against the plain chain on the
+    same state, RK4 steps with either are timed, and the judge reports the
+    closure error and the solution drift after the same steps."""
+    import torch
+    import pyranda_amd as pr
+    from pyranda_amd.closure import MLPClosure, pyrandaLearnedLAD
+    from pyranda_amd.workloads import rm_run, rm_setup
+    n, prec = args.closure_n, args.closure_precision
+    plain = rm_setup(pr, n, precision=prec, device=dev)
+    nx, ny = plain.nx, plain.ny
+    print("[closure] RM 2-D %dx%d %s" % (nx, ny, prec), file=sys.stderr, flush=True)
+    helper = pyrandaLearnedLAD(plain, None)
+    lad_in = ("div", "vx", "uy", "rho")
+    samples, st = [], None
+    for _ in range(args.closure_snapshots):
+        st = rm_run(plain, args.closure_every, state=st)
+        feats = [f.clone() for f in helper.features(*[plain.var(k).data for k in lad_in])]
+        samples.append((feats, [plain.var("beta").data.clone(), plain.var("adiff").data.clone()]))
+    model = MLPClosure(dev)
+    model.set_normalisation([s_[0] for s_ in samples], [s_[1] for s_ in samples])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    losses = model.train(samples, (False, True), steps=args.closure_train_steps, lr=2e-3)
+    torch.cuda.synchronize()
+    t_train = time.perf_counter() - t0
+    print("[closure] trained %d steps in %.2f s, mse %.3e -> %.3e" % (
+        args.closure_train_steps, t_train, float(losses[:10].mean()), float(losses[-10:].mean())),
+        file=sys.stderr, flush=True)
+
+    def timed(fn, reps):
+        fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        torch.cuda.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    feats = samples[-1][0]
+    v = {k: plain.var(k).data for k in ("div", "vx", "uy", "rho", "Yh")}
+    ms_mlp = timed(lambda: model.infer(feats, (False, True)), 50)
+    ms_feat = timed(lambda: helper.features(v["div"], v["vx"], v["uy"], v["rho"]), 20)
+    ms_plain = timed(lambda: helper.plain(), 20)
+    npts = nx * ny
+    mfma_flops = 2.0 * (32 * 64 + 64 * 64 + 64 * 16) * npts      # as issued (padded tiles)
+    useful_flops = 2.0 * (27 * 64 + 64 * 64 + 64 * 2) * npts
+    # steps of the RM run with either evaluation, from the same initial state
+    runs = {}
+    for mode in ("plain", "closure"):
+        ss = rm_setup(pr, n, precision=prec, device=dev, closure=model if mode == "closure" else None)
+        stt = rm_run(ss, 1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        stt = rm_run(ss, args.steps, state=stt)
+        torch.cuda.synchronize()
+        runs[mode] = (ss, (time.perf_counter() - t0) / args.steps)
+    cl = runs["closure"][0]
+    lad = cl.packages["LearnedLAD"]
+    judge = lad.judge(*[cl.var(k).data for k in ("div", "vx", "uy", "rho", "Yh")], cl.var("dt").data)
+    rp, rc = runs["plain"][0].var("rho").data.double(), cl.var("rho").data.double()
+    drift = float((rc - rp).norm() / rp.norm())
+    line = {
+        "metric": "RM 2-D learned LAD closure: MLP evaluation points/s (BF16 MFMA, FP32 accumulate)",
+        "value": npts / (ms_mlp * 1e-3), "unit": "points/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": 1, "ms_per_step": 1e3 * runs["closure"][1], "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "bf16 weights/activations, f32 accumulate; %s fields" % prec,
+        "data": "synthetic (RM3D.py 2-D initial condition, seeded perturbation)",
+        "config": {"workload": "rm_2d_learned_closure", "grid": [nx, ny], "mlp": "27-64-64-2 relu",
+                   "train_snapshots": args.closure_snapshots, "train_steps": args.closure_train_steps},
+        "mlp_ms": ms_mlp, "features_ms": ms_feat, "plain_lad_ms": ms_plain,
+        "closure_eval_speedup": ms_plain / (ms_mlp + ms_feat),
+        "roofline": {"bound": "mfma", "achieved": mfma_flops / (ms_mlp * 1e-3) / 1e12, "peak": 2500.0,
+                     "unit": "TFLOP/s", "frac": mfma_flops / (ms_mlp * 1e-3) / 1e12 / 2500.0,
+                     "useful_tflops": useful_flops / (ms_mlp * 1e-3) / 1e12, "traffic": None},
+        "step_ms": {"plain": 1e3 * runs["plain"][1], "closure": 1e3 * runs["closure"][1]},
+        "closure_calls": lad.calls, "fallbacks": lad.fallbacks,
+        "judge": {k: {"rel_l2": a_, "rel_max": b_} for k, (a_, b_) in judge.items()},
+        "drift_rho_rel_l2": drift, "train_s": t_train,
+        "train_mse": [float(losses[:10].mean()), float(losses[-10:].mean())],
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -315,6 +404,13 @@ def main():
                     help="KH ensemble (independent 2-D layers on uneven grids, one HIP graph per step)")
     ap.add_argument("--members", type=int, default=64, help="--ensemble: number of members")
     ap.add_argument("--ens-precision", default="fp32", choices=["fp64", "fp32", "fp32-fp64solve"])
+    ap.add_argument("--closure", action="store_true",
+                    help="learned LAD closure on the 2-D RM case (train, time, judge)")
+    ap.add_argument("--closure-n", type=int, default=256, help="--closure: RM grid is 1.4n x n")
+    ap.add_argument("--closure-precision", default="fp64", choices=["fp64", "fp32"])
+    ap.add_argument("--closure-snapshots", type=int, default=16)
+    ap.add_argument("--closure-every", type=int, default=4)
+    ap.add_argument("--closure-train-steps", type=int, default=1500)
     ap.add_argument("--only", default=None,
                     help="--ops: comma-separated name prefixes to run (e.g. d1_x_f64,copy_f64)")
     args = ap.parse_args()
@@ -334,6 +430,8 @@ def main():
         return ops_bench(n, args.steps, args.warmup, dev, args.only, args.solver, args.r32_solve)
     if args.ensemble:
         return ensemble_bench(args, dev)
+    if args.closure:
+        return closure_bench(args, dev)
     gamma = 1.4
     p0 = 1.0 / (gamma * 0.1 ** 2)  # Mach 0.1 on the unit velocity
 

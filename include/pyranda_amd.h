@@ -158,6 +158,41 @@ int pyr_pw_launch_blocks(void* fn, void* stream, long long n, int blocks, int np
                          void* const* ptrs, int nscal, const double* scal);
 const char* pyr_pw_last_error(void);
 
+/* --- learned LAD closure (BF16 MFMA) -------------------------------------
+ * No counterpart in the reference: pyranda evaluates the LAD artificial
+ * properties with compact operators only (examples/RM3D.py:95-104). A
+ * pointwise MLP maps the 3x3 neighbourhoods of three feature fields of a 2-D
+ * (nx, ny) grid (x fastest; neighbours wrap on periodic axes and are clamped
+ * otherwise) to two outputs: 27 inputs (padded to 32) -> 64 -> 64 -> 2, ReLU
+ * after the hidden layers, BF16 weights and activations, FP32 accumulation.
+ * Input k = 9 f + 3 (dy + 1) + (dx + 1) of point (i, j) is
+ * (feature_f(i+dx, j+dy) - fshift[f]) * fscale[f].
+ *
+ * `params` is one device block of PYR_MLP_PARAM_BYTES bytes:
+ *   bf16 W1[32][64], W2[64][64], W3[64][16]   (row = input unit; rows 27..31
+ *                                                of W1 and columns 2..15 of W3 are 0)
+ *   at PYR_MLP_PARAM_FLOAT_OFFSET, float b1[64], b2[64], b3[16], fshift[4],
+ *                                     fscale[4], oscale[4]
+ * All fields of one call (features, targets, outputs) have the precision
+ * `prec` and the (nx, ny) layout.
+ * Inference: out_j = max(o_j, 0) * oscale[j].
+ * Training: the gradient of L = sum_points sum_j (o_j - t_j / oscale[j])^2 / n
+ * with respect to every parameter, in PYR_MLP_GRAD_FLOATS floats:
+ *   gW1[32][64], gW2[64][64], gW3[64][16], gb1[64], gb2[64], gb3[16], then
+ *   sum_points sum_j (o_j - t_j / oscale[j])^2 (not divided by n).
+ * `partial` is caller-provided workspace of npartial * PYR_MLP_GRAD_FLOATS
+ * floats (one block per workgroup, at most npartial workgroups; the sum over
+ * blocks is taken in a fixed order). Both calls enqueue on `stream` (a
+ * hipStream_t) and return without synchronising. */
+#define PYR_MLP_PARAM_FLOAT_OFFSET 14336
+#define PYR_MLP_PARAM_BYTES 14960
+#define PYR_MLP_GRAD_FLOATS 7313
+int pyr_mlp_infer(void* stream, int prec, int nx, int ny, int periodic_x, int periodic_y,
+                  const void* const* features, const void* params, void* const* out);
+int pyr_mlp_train_grad(void* stream, int prec, int nx, int ny, int periodic_x, int periodic_y,
+                       const void* const* features, const void* const* targets, const void* params,
+                       float* partial, int npartial, float* grad);
+
 /* --- host-side checks (no GPU needed) ----------------------------------- */
 /* Weight tables as the library restates them (stencils.f90), for comparison
  * with the oracle: ali[5], ari[9], alb1/alb2[3*4*5], arb1/arb2[3*4*9]. */

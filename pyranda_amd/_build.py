@@ -3,7 +3,8 @@ import os
 import subprocess
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-SRCS = [os.path.join(HERE, "csrc", "pyr_core.hip"), os.path.join(HERE, "csrc", "pyr_fuse.cpp")]
+SRCS = [os.path.join(HERE, "csrc", "pyr_core.hip"), os.path.join(HERE, "csrc", "pyr_mlp.hip"),
+        os.path.join(HERE, "csrc", "pyr_fuse.cpp")]
 OUT = os.path.join(HERE, "libpyranda_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 

@@ -39,6 +39,11 @@ SIGNATURES = {
                                      ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "pyr_rk_update_dt": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong, _vp, _vp, _vp, _vp, _vp,
                                         _vp, ctypes.c_double, ctypes.c_double]),
+    "pyr_mlp_infer": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                     ctypes.c_int, ctypes.POINTER(_vp), _vp, ctypes.POINTER(_vp)]),
+    "pyr_mlp_train_grad": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp,
+                                          _vp, ctypes.c_int, _vp]),
     "pyr_pw_compile": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.POINTER(_vp)]),
     "pyr_pw_check_source": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p]),
     "pyr_pw_launch": (ctypes.c_int, [_vp, _vp, ctypes.c_longlong, ctypes.c_int, ctypes.POINTER(_vp),
@@ -49,6 +54,7 @@ SIGNATURES = {
     "pyr_set_r32_solve": (ctypes.c_int, [_vp, ctypes.c_int]),
 }
 PW_MAX_PTRS, PW_MAX_SCALARS = 24, 16
+MLP_PARAM_FLOAT_OFFSET, MLP_PARAM_BYTES, MLP_GRAD_FLOATS = 14336, 14960, 7313
 for _prec in ("f64", "r32"):
     for _name in ("ddx", "ddy", "ddz", "plaplacian", "pring", "sfilter", "gfilter"):
         SIGNATURES["pyr_%s_%s" % (_name, _prec)] = (ctypes.c_int, [_vp, _vp, _vp])

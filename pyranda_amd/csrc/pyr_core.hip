@@ -35,6 +35,10 @@ static int fail(const std::string& m) {
   g_err = m;
   return 1;
 }
+namespace pyr {
+// errors of the other translation units (pyr_mlp.hip) go to pyr_last_error too
+int report_error(const std::string& m) { return fail(m); }
+}  // namespace pyr
 #define HIPCHECK(x)                                                              \
   do {                                                                           \
     hipError_t e_ = (x);                                                         \

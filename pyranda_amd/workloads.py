@@ -265,3 +265,176 @@ def kh_ensemble(pr, sizes, precision="fp32", device=None):
         state.append({"time": torch.zeros((), dtype=torch.float64, device=ss.PyMPI.device),
                       "dt": dt.to(torch.float64).clone()})
     return sims, state
+
+
+# Richtmyer-Meshkov, 2-D branch of examples/RM3D.py (EOM :55-131 and IC
+# :139-176 verbatim; the reference's EOM and IC strings are its API): Mach ~1.5
+# shock into a perturbed heavy/light interface (rho_h/rho_l = 3), bounded x
+# with BC lines and sponges, periodic y.
+RM_EOM = """
+# Primary Equations of motion here
+ddt(:rhoYh:)  =  -ddx(:rhoYh:*:u: - :Jx:)    - ddy(:rhoYh:*:v: - :Jy:)   - ddz(:rhoYh:*:w: - :Jz:)
+ddt(:rhoYl:)  =  -ddx(:rhoYl:*:u: + :Jx:)    - ddy(:rhoYl:*:v: + :Jy:)   - ddz(:rhoYl:*:w: + :Jz:)
+ddt(:rhou:)   =  -ddx(:rhou:*:u: - :tauxx:)  - ddy(:rhou:*:v: - :tauxy:) - ddz(:rhou:*:w: - :tauxz:)
+ddt(:rhov:)   =  -ddx(:rhov:*:u: - :tauxy:)  - ddy(:rhov:*:v: - :tauyy:) - ddz(:rhov:*:w: - :tauyz:)
+ddt(:rhow:)   =  -ddx(:rhow:*:u: - :tauxz:)  - ddy(:rhow:*:v: - :tauyz:) - ddz(:rhow:*:w: - :tauzz:)
+ddt(:Et:)     =  -ddx( (:Et: - :tauxx:)*:u: - :tauxy:*:v: - :tauxz:*:w: ) - ddy( (:Et: - :tauyy:)*:v: -:tauxy:*:u: - :tauyz:*:w:) - ddz( (:Et: - :tauzz:)*:w: - :tauxz:*:u: - :tauyz:*:v: )
+# Conservative filter of the EoM
+:rhoYh:     =  fbar( :rhoYh:  )
+:rhoYl:     =  fbar( :rhoYl:  )
+:rhou:      =  fbar( :rhou: )
+:rhov:      =  fbar( :rhov: )
+:rhow:      =  fbar( :rhow: )
+:Et:        =  fbar( :Et:   )
+# Update the primatives and enforce the EOS
+:rho:       = :rhoYh: + :rhoYl:
+:Yh:        =  :rhoYh: / :rho:
+:Yl:        =  :rhoYl: / :rho:
+:u:         =  :rhou: / :rho:
+:v:         =  :rhov: / :rho:
+:w:         =  :rhow: / :rho:
+:p:         =  ( :Et: - .5*:rho:*(:u:*:u: + :v:*:v: + :w:*:w:) ) * ( :gamma: - 1.0 )
+# Artificial bulk viscosity / shear viscosity
+:ux:        =  ddx(:u:)
+:vy:        =  ddy(:v:)
+:wz:        =  ddz(:w:)
+:div:       =  :ux: + :vy: + :wz:
+# Remaining cross derivatives
+:uy:        =  ddy(:u:)
+:uz:        =  ddz(:u:)
+:vx:        =  ddx(:v:)
+:vz:        =  ddz(:v:)
+:wy:        =  ddy(:w:)
+:wx:        =  ddx(:w:)
+:Yx:        =  ddx(:Yh:)
+:Yy:        =  ddy(:Yh:)
+:Yz:        =  ddz(:Yh:)
+:enst:      = sqrt( (:uy:-:vx:)**2 + (:uz: - :wx:)**2 + (:vz:-:wy:)**2 )
+:tke:       = :rho:*(:u:*:u: + :v:*:v: + :w:*:w:)
+:S:         = sqrt( :ux:*:ux: + :vy:*:vy: + :wz:*:wz: + .5*((:uy:+:vx:)**2 + (:uz: + :wx:)**2 + (:vz:+:wy:)**2) )
+:mu:        =  gbar( abs(ring(:S:  )) ) * :rho: * 1.0e-4
+:beta:      =  gbar( abs(ring(:div:)) * :rho: )  * 7.0e-3
+# Artificial species diffusivities
+:Dsgs:      =  ring(:Yh:) * 1.0e-4
+:Ysgs:      =  1.0e2*(abs(:Yh:) - 1.0 + abs(1.0-:Yh: ) )*gridLen**2
+:adiff:     =  gbar( :rho:*numpy.maximum(:Dsgs:,:Ysgs:) / :dt: )
+:Jx:        =  :adiff:*:Yx:
+:Jy:        =  :adiff:*:Yy:
+:Jz:        =  :adiff:*:Yz:
+:taudia:    =  (:beta:-2./3.*:mu:) *:div: - :p:
+:tauxx:     =  2.0*:mu:*:ux:   + :taudia:
+:tauyy:     =  2.0*:mu:*:vy:   + :taudia:
+:tauzz:     =  2.0*:mu:*:wz:   + :taudia:
+:tauxy:     = :mu:*(:uy:+:vx:)
+:tauxz:     = :mu:*(:uz:+:wx:)
+:tauyz:     = :mu:*(:vz:+:wz:)
+:cs:        = sqrt( :p: / :rho: * :gamma: )
+# Time step control routines
+:dt:        = dt.courant(:u:,:v:,:w:,:cs:)*1.0
+:dt:        = numpy.minimum(:dt:,0.2 * dt.diff(:beta:,:rho:))
+:dt:        = numpy.minimum(:dt:,0.2 * dt.diff(:mu:,:rho:))
+:dt:        = numpy.minimum(:dt:,0.2 * dt.diff(:adiff:,:rho:))
+# Add some BCs in the x-direction
+bc.const(['Yh'],['xn'],0.0)
+bc.const(['Yh'],['x1'],1.0)
+bc.const(['Yl'],['x1'],0.0)
+bc.const(['Yl'],['xn'],1.0)
+bc.extrap(['rho','Et'],['x1','xn'])
+bc.extrap(['u'],['x1','xn'])
+bc.const(['v','w'],['x1','xn'],0.0)
+# Sponge BCs
+:leftBC:  = 1.0 - 0.5 * (1.0 + tanh( (meshx-1.0) / .1 ) )
+:rightBC: = 0.5 * (1.0 + tanh( (meshx-8.0) / .1 ) )
+:BC: = numpy.maximum(:leftBC:,:rightBC:)
+:u: = gbar(:u:)*:BC: + :u:*(1.0 - :BC:)
+:rho: = gbar(:rho:)*:BC: + :rho:*(1.0 - :BC:)
+:Et: = gbar(:Et:)*:BC: + :Et:*(1.0 - :BC:)
+:rhoYh: = :rho:*:Yh:
+:rhoYl: = :rho:*:Yl:
+:rhou: = :rho:*:u:
+:rhov: = :rho:*:v:
+:rhow: = :rho:*:w:
+"""
+
+RM_IC = """
+:gamma:= 5./3.
+L      =  waveLength
+u0     = -291.575
+p0     = 100000.0
+rho0_h = 3.0
+rho0_l = 1.0
+rhoS   = 6.37497
+uS     = -61.48754
+pS     = 399995.9
+Amp = 3.5 + .05*(sin(meshy*L) + cos(meshz*L) ) + .25*gbar( random3D() - .5 )
+:Yh: = .5 * (1.0-tanh( sqrt(pi)*( meshx - Amp) / delta ))
+:Yl: = 1.0 - :Yh:
+:p:  += p0
+:v: *= 0.0
+:w: *= 0.0
+:u: = gbar( where( meshx < 3.0, uS, u0) )
+:rho: = gbar( where( meshx < 3.0, rhoS, rho0_h ) )
+:rho: = where( meshx < 3.2, :rho:, rho0_h * :Yh: + rho0_l * :Yl:)
+:p: = gbar( where( meshx < 3.0, pS, :p: ) )
+:rhoYh: = :rho:*:Yh:
+:rhoYl: = :rho:*:Yl:
+:rhou: = :rho:*:u:
+:rhov: = :rho:*:v:
+:rhow: = :rho:*:w:
+:Et:  = :p: / (:gamma:-1.0) + 0.5*:rho:*(:u:*:u: + :v:*:v: + :w:*:w:)
+:cs:  = sqrt( :p: / :rho: * :gamma: )
+:tke: = :rho:*(:u:*:u: + :v:*:v: + :w:*:w:)
+:dt: = dt.courant(:u:,:v:,:w:,:cs:)
+"""
+
+# the LAD lines the learned closure replaces (beta, then the adiff chain)
+RM_LAD_LINES = (":beta:      =  gbar( abs(ring(:div:)) * :rho: )  * 7.0e-3",
+                ":Dsgs:      =  ring(:Yh:) * 1.0e-4",
+                ":Ysgs:      =  1.0e2*(abs(:Yh:) - 1.0 + abs(1.0-:Yh: ) )*gridLen**2",
+                ":adiff:     =  gbar( :rho:*numpy.maximum(:Dsgs:,:Ysgs:) / :dt: )")
+RM_CLOSURE_LINE = "[:beta:,:adiff:] = lad.closure(:div:,:vx:,:uy:,:rho:,:Yh:,:dt:)"
+
+
+def rm_setup(pr, npts=128, precision="fp64", seed=1234, closure=None, name="RM_2D", device=None):
+    """examples/RM3D.py, 2-D branch, on an int(1.4 npts) x npts grid; the IC's
+    random perturbation draws from numpy's global generator seeded with
+    `seed`. With `closure` (an MLPClosure) the beta and adiff lines become
+    the learned closure (plain chain as fallback)."""
+    import numpy
+    from .closure import pyrandaLearnedLAD
+    imesh = """
+    xdom = (0.0, 2.8*pi , int(Npts*1.4), periodic=False)
+    ydom = (0.0, 2*pi*FF,  Npts, periodic=True)
+    zdom = (0.0, 2*pi*FF,  1, periodic=True)
+    """.replace("Npts", str(npts)).replace("pi", str(numpy.pi)).replace("FF", str(float(npts - 1) / npts))
+    kw = {"silent": True, "precision": precision}
+    if device is not None:
+        kw["device"] = device
+    ss = pr.pyrandaSim(name, imesh, **kw)
+    ss.addPackage(pr.pyrandaTimestep(ss))
+    ss.addPackage(pr.pyrandaBC(ss))
+    eom = RM_EOM
+    if closure is not None:
+        ss.addPackage(pyrandaLearnedLAD(ss, closure))
+        eom = eom.replace(RM_LAD_LINES[0], RM_CLOSURE_LINE)
+        for line in RM_LAD_LINES[1:]:
+            eom = eom.replace(line + "\n", "")
+    ss.EOM(eom)
+    numpy.random.seed(seed)
+    ss.setIC(RM_IC, {"waveLength": 4, "delta": 2.0 * numpy.pi / npts * 4})
+    return ss
+
+
+def rm_run(ss, nsteps, cfl=1.0, state=None):
+    """The RM3D.py loop (:196-201): dt from the EOM's dt line, growth at most
+    10% per step; returns the driver state {time, dt, dtmax} to continue."""
+    if state is None:
+        dt = float(ss.variables["dt"].data) * cfl
+        state = {"time": 0.0, "dt": dt, "dtmax": dt * .1}
+    time, dt, dtmax = state["time"], state["dt"], state["dtmax"]
+    for _ in range(nsteps):
+        time = ss.rk4(time, dt)
+        dt = float(ss.variables["dt"].data) * cfl
+        dt = min(dt, dtmax * 1.1)
+        dtmax = dt * 1.0
+    return {"time": time, "dt": dt, "dtmax": dtmax}

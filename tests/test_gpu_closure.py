@@ -1,0 +1,225 @@
+"""Learned LAD closure on the GPU: BF16-MFMA kernels against a PyTorch FP32
+reference of the same operation, training, and the closure's fallback.
+
+The reference (below) evaluates the MLP with torch FP32 matrix products on
+the same BF16-rounded weights, inputs and activations as the kernels; the
+kernels accumulate in a different order, so an activation can round to the
+neighbouring BF16 value now and then: outputs and gradients are compared
+at a few parts in a thousand of their largest magnitude.
+"""
+import math
+
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+
+NIN, NH, NO, NF = 32, 64, 16, 27
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def pr():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import pyranda_amd
+    return pyranda_amd
+
+
+def bf(t):
+    return t.to(torch.bfloat16).to(torch.float32)
+
+
+def ref_features(fields, fshift, fscale, periodic):
+    nx, ny = fields[0].shape[:2]
+    i = torch.arange(nx, device=fields[0].device)
+    j = torch.arange(ny, device=fields[0].device)
+    cols = []
+    for k in range(NIN):
+        if k >= NF:
+            cols.append(torch.zeros(nx * ny, device=fields[0].device))
+            continue
+        f, r = divmod(k, 9)
+        dx, dy = r % 3 - 1, r // 3 - 1
+        ii = (i + dx) % nx if periodic[0] else (i + dx).clamp(0, nx - 1)
+        jj = (j + dy) % ny if periodic[1] else (j + dy).clamp(0, ny - 1)
+        v = fields[f][:, :, 0].float()[ii][:, jj]
+        v = (v - fshift[f]) * fscale[f]
+        cols.append(v.t().reshape(-1))          # point p = j * nx + i
+    return bf(torch.stack(cols, 1))
+
+
+def ref_forward(m, X):
+    W1, W2, W3, b1, b2, b3 = [p.float() for p in m.params]
+    H1 = bf(torch.relu(X @ bf(W1) + b1))
+    H2 = bf(torch.relu(H1 @ bf(W2) + b2))
+    O = H2 @ bf(W3) + b3
+    return H1, H2, O
+
+
+def ref_grad(m, X, T):
+    W1, W2, W3, b1, b2, b3 = [p.float() for p in m.params]
+    H1, H2, O = ref_forward(m, X)
+    n = X.shape[0]
+    e = torch.zeros_like(O)
+    e[:, :2] = O[:, :2] - T / m.oscale[:2]
+    dO = bf(2.0 * e * (1.0 / n))
+    dZ2 = bf((dO @ bf(W3).t()) * (H2 > 0))
+    dZ1 = bf((dZ2 @ bf(W2).t()) * (H1 > 0))
+    return [X.t() @ dZ1, H1.t() @ dZ2, H2.t() @ dO, dZ1.sum(0), dZ2.sum(0), dO.sum(0)], (e * e).sum()
+
+
+def _model(pr, seed=3):
+    from pyranda_amd.closure import MLPClosure
+    m = MLPClosure("cuda", seed=seed)
+    g = torch.Generator().manual_seed(seed + 1)
+    m.params[3].copy_(0.1 * torch.randn(NH, generator=g))
+    m.params[4].copy_(0.1 * torch.randn(NH, generator=g))
+    m.params[5][:2].copy_(0.1 * torch.randn(2, generator=g))
+    m.fshift[:3] = torch.tensor([0.1, -0.2, 0.3])
+    m.fscale[:3] = torch.tensor([2.0, 0.5, 1.5])
+    m.oscale[:2] = torch.tensor([3.0, 0.25])
+    m.pack()
+    return m
+
+
+def _close(got, ref, frac=4e-3):
+    scale = float(ref.abs().max()) or 1.0
+    err = float((got.float() - ref.float()).abs().max()) / scale
+    return err <= frac, err
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+@pytest.mark.parametrize("shape,periodic", [((37, 29), (False, True)), ((64, 48), (True, True)),
+                                            ((5, 3), (False, False))])
+def test_mlp_infer_matches_torch(pr, dtype, shape, periodic):
+    _say("mlp infer", dtype, shape, periodic)
+    m = _model(pr)
+    nx, ny = shape
+    g = torch.Generator().manual_seed(nx * ny)
+    fields = [torch.randn(ny, nx, generator=g, dtype=torch.float64).t().reshape(nx, ny, 1).to("cuda", dtype)
+              for _ in range(3)]
+    out = m.infer(fields, periodic)
+    torch.cuda.synchronize()
+    X = ref_features(fields, m.fshift, m.fscale, periodic)
+    _, _, O = ref_forward(m, X)
+    for j in range(2):
+        want = (torch.relu(O[:, j]) * m.oscale[j]).reshape(ny, nx).t()
+        got = out[j][:, :, 0]
+        assert got.dtype == dtype
+        ok, err = _close(got, want)
+        _say("  output", j, "max err / max", err)
+        assert ok, err
+
+
+@pytest.mark.parametrize("shape,periodic", [((37, 29), (False, True)), ((128, 96), (True, True))])
+def test_mlp_gradient_matches_torch(pr, shape, periodic):
+    _say("mlp gradient", shape, periodic)
+    m = _model(pr)
+    nx, ny = shape
+    g = torch.Generator().manual_seed(7 + nx)
+    fields = [torch.randn(ny, nx, generator=g, dtype=torch.float64).t().reshape(nx, ny, 1).to("cuda", torch.float32)
+              for _ in range(3)]
+    targets = [torch.rand(ny, nx, generator=g, dtype=torch.float64).t().reshape(nx, ny, 1).to("cuda", torch.float32)
+               * s for s in (3.0, 0.25)]
+    grads, mse = m.gradient(fields, targets, periodic)
+    torch.cuda.synchronize()
+    X = ref_features(fields, m.fshift, m.fscale, periodic)
+    T = torch.stack([t[:, :, 0].t().reshape(-1) for t in targets], 1)
+    want, loss = ref_grad(m, X, T)
+    names = ["gW1", "gW2", "gW3", "gb1", "gb2", "gb3"]
+    for name, got, ref in zip(names, grads, want):
+        ok, err = _close(got, ref, 1e-2)
+        _say("  ", name, "max err / max", err)
+        assert ok, (name, err)
+    assert abs(float(mse) * nx * ny - float(loss)) <= 1e-3 * float(loss)
+    # padded inputs and outputs get no gradient
+    assert float(grads[0][NF:].abs().max()) == 0.0
+    assert float(grads[2][:, 2:].abs().max()) == 0.0
+
+
+def test_mlp_learns_a_smooth_map(pr):
+    """Training on the matrix cores reduces the error of a map the network
+    can represent (targets from another network of the same shape)."""
+    teacher = _model(pr, seed=11)
+    m = _model(pr, seed=5)
+    nx, ny = 96, 64
+    g = torch.Generator().manual_seed(2)
+    samples = []
+    for _ in range(4):
+        fields = [torch.randn(ny, nx, generator=g, dtype=torch.float64).t().reshape(nx, ny, 1).to("cuda", torch.float32)
+                  for _ in range(3)]
+        samples.append((fields, teacher.infer(fields, (True, True))))
+    before = float(m.gradient(*samples[0], (True, True))[1])
+    losses = m.train(samples, (True, True), steps=300, lr=3e-3)
+    after = float(m.gradient(*samples[0], (True, True))[1])
+    _say("mse before", before, "after", after, "last losses", losses[-4:].tolist())
+    assert math.isfinite(after) and after < 0.2 * before
+
+
+def test_closure_fallback_is_the_plain_chain(pr):
+    """An untrained closure flags every evaluation, and its fallback is the
+    plain LAD chain: the RM run is bit for bit the plain run (FP64)."""
+    from pyranda_amd.closure import MLPClosure
+    from pyranda_amd.workloads import rm_run, rm_setup
+    a = rm_setup(pr, 48)
+    model = MLPClosure("cuda")
+    b = rm_setup(pr, 48, closure=model)
+    sa = rm_run(a, 4)
+    sb = rm_run(b, 4)
+    lad = b.packages["LearnedLAD"]
+    assert lad.calls > 0 and lad.fallbacks == lad.calls
+    assert sa == sb
+    for k in ("rho", "Et", "beta", "adiff", "Yh"):
+        assert torch.equal(a.var(k).data, b.var(k).data), k
+
+
+def test_closure_trained_on_rm(pr):
+    """Train on snapshots of the plain RM run, then run the closure: it is
+    used (no fallback) and the judge reports its error against the plain
+    chain; the run stays finite."""
+    from pyranda_amd.closure import MLPClosure
+    from pyranda_amd.workloads import rm_run, rm_setup
+    plain = rm_setup(pr, 64)
+    lad_vars = ("div", "vx", "uy", "rho")
+    from pyranda_amd.closure import pyrandaLearnedLAD
+    helper = pyrandaLearnedLAD(plain, None)
+    samples, st = [], None
+    for _ in range(6):
+        st = rm_run(plain, 5, state=st)
+        feats = [f.float() for f in helper.features(*[plain.var(k).data for k in lad_vars])]
+        targets = [plain.var("beta").data.float(), plain.var("adiff").data.float()]
+        samples.append(([f.clone() for f in feats], [t.clone() for t in targets]))
+    model = MLPClosure("cuda")
+    model.set_normalisation([s[0] for s in samples], [s[1] for s in samples])
+    losses = model.train(samples, (False, True), steps=200, lr=2e-3)
+    _say("training mse first/last", float(losses[0]), float(losses[-1]))
+    assert float(losses[-8:].mean()) < float(losses[:8].mean())
+    run = rm_setup(pr, 64, closure=model)
+    rm_run(run, 5)
+    lad = run.packages["LearnedLAD"]
+    rep = lad.judge(*[run.var(k).data for k in ("div", "vx", "uy", "rho", "Yh")], run.var("dt").data)
+    _say("closure calls", lad.calls, "fallbacks", lad.fallbacks, "judge", rep)
+    assert lad.calls > lad.fallbacks
+    assert torch.isfinite(run.var("rho").data).all()
+    for name, (l2, mx) in rep.items():
+        assert math.isfinite(l2) and math.isfinite(mx), name
+
+
+def test_mlp_mixed_precision_fields(pr):
+    """Feature fields of mixed precision (an FP32 run whose EOM mixes in the
+    FP64 mesh coordinates) are evaluated in FP64, as if all were FP64."""
+    m = _model(pr)
+    nx, ny = 40, 24
+    g = torch.Generator().manual_seed(9)
+    f64 = [torch.randn(ny, nx, generator=g, dtype=torch.float64).t().reshape(nx, ny, 1).to("cuda")
+           for _ in range(3)]
+    mixed = [f64[0].float(), f64[1], f64[2].float()]
+    want = m.infer([f.float().double() if i != 1 else f for i, f in enumerate(mixed)], (False, True))
+    got = m.infer(mixed, (False, True))
+    torch.cuda.synchronize()
+    for a, b in zip(got, want):
+        assert a.dtype == torch.float64 and torch.equal(a, b)
