@@ -140,6 +140,33 @@ int pyr_rk_update_dt(pyr_ctx* ctx, int prec, long long n, const void* F, double*
                      const void* U_in, void* U_out, double* U64, const double* dt, double A,
                      double B);
 
+/* --- groups: one operator over several simulations' fields ---------------
+ * No counterpart in the reference (pyranda advances one simulation per MPI
+ * communicator, pyranda/pyranda.py:32-100). The members are contexts of any
+ * grid sizes (pyr_setup each) whose fields are stored in one allocation,
+ * member m's (nx,ny,nz) block at offsets[m] elements; offsets that are
+ * multiples of 16 elements keep the 16-byte row copies. pyr_group_apply is
+ * pyr_apply on every member at once: one launch for all members' lines of
+ * the direction (any lengths 8..512), plus a null-operator pass for members
+ * whose direction has fewer than 4 points. The epilogue constant of member m is
+ * epi_scale * d_m^mesh_power (d_m its mesh spacing d1..d3 along the
+ * direction: mesh_power 2 gives the ring scaling of operators.f90:615-643).
+ * Periodic lines and explicit filters only (the bounded LU is not grouped).
+ * The first call of each (op, dir, prec, epi, scale) builds and uploads the
+ * members' line arguments (synchronous copy); later calls only launch. */
+typedef struct pyr_group pyr_group;
+int pyr_group_create(pyr_group** group, int n_members, pyr_ctx* const* members,
+                     const long long* offsets);
+int pyr_group_destroy(pyr_group* group);
+int pyr_group_set_stream(pyr_group* group, void* stream);
+int pyr_group_apply(pyr_group* group, int op, int dir, int prec, const void* in, void* out,
+                    int epi, double epi_scale, int mesh_power);
+/* RK4 stage update of a group's fields (n points in all) with each member's
+ * own dt: dt_member[member_of_point[i]] (FP64, device memory). */
+int pyr_rk_update_member_dt(void* stream, int prec, long long n, const void* F, double* PHI,
+                            const void* U_in, void* U_out, double* U64, const double* dt_member,
+                            const int* member_of_point, double A, double B);
+
 /* --- fused elementwise kernels for EOM pointwise algebra ----------------
  * No counterpart in the reference (its pointwise algebra is numpy, pyrandaEq.py:39).
  * Source is generated by pyranda_amd/fuse.py: one kernel taking a by-value

@@ -11,5 +11,7 @@ from .pyrandaTimestep import pyrandaTimestep
 from .pyrandaBC import pyrandaBC
 from .pyrandaPackage import pyrandaPackage
 from .pyrandaMesh import defaultMeshOptions
+from .group import pyrandaGroupSim
 
-__all__ = ["pyrandaSim", "pyrandaTimestep", "pyrandaBC", "pyrandaPackage", "defaultMeshOptions"]
+__all__ = ["pyrandaSim", "pyrandaTimestep", "pyrandaBC", "pyrandaPackage", "defaultMeshOptions",
+           "pyrandaGroupSim"]

@@ -39,6 +39,14 @@ SIGNATURES = {
                                      ctypes.c_double, ctypes.c_double, ctypes.c_double]),
     "pyr_rk_update_dt": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong, _vp, _vp, _vp, _vp, _vp,
                                         _vp, ctypes.c_double, ctypes.c_double]),
+    "pyr_group_create": (ctypes.c_int, [ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(_vp),
+                                        ctypes.POINTER(ctypes.c_longlong)]),
+    "pyr_group_destroy": (ctypes.c_int, [_vp]),
+    "pyr_group_set_stream": (ctypes.c_int, [_vp, _vp]),
+    "pyr_group_apply": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _vp, _vp, ctypes.c_int,
+                                       ctypes.c_double, ctypes.c_int]),
+    "pyr_rk_update_member_dt": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_longlong, _vp, _vp, _vp, _vp, _vp,
+                                               _vp, _vp, ctypes.c_double, ctypes.c_double]),
     "pyr_mlp_infer": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, ctypes.POINTER(_vp), _vp, ctypes.POINTER(_vp)]),
     "pyr_mlp_train_grad": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,

@@ -16,6 +16,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -151,23 +152,32 @@ __device__ __forceinline__ void solve_chunk_lu(const LineArgs& a, const Tabs& t,
 
 // One circulant solve A y = r of the chunk (without the 1/c): forward
 // recurrence with its carry through Xa, backward recurrence with its carry
-// through Xb; the rows of the result go to sink(k, y_k). Two barriers.
+// through Xb; the rows of the result go to sink(k, y_k). Two barriers. A
+// thread whose chunk index p is not below P (a workgroup shaped for a longer
+// line of another member of a grouped launch) only takes the barriers.
 template <typename C, typename Sink>
 __device__ __forceinline__ void circ_solve(const C* kf, const C* kb, C p1, C p2, int p, int P,
                                            int m, C* Xa, C* Xb, int ly, int col, C* x, Sink sink) {
-  C e0, e1;
-  circ_fwd_exit(x, m, p1, p2, e0, e1);
-  Xa[(2 * p) * ly + col] = e0;
-  Xa[(2 * p + 1) * ly + col] = e1;
+  const bool act = p < P;
+  C e0 = 0, e1 = 0;
+  if (act) {
+    circ_fwd_exit(x, m, p1, p2, e0, e1);
+    Xa[(2 * p) * ly + col] = e0;
+    Xa[(2 * p + 1) * ly + col] = e1;
+  }
   __syncthreads();
-  circ_carry(kf, p, P, [&](int c) { return Xa[c * ly + col]; }, e0, e1);
-  circ_fwd(x, m, p1, p2, e0, e1);
-  circ_bwd_exit(x, m, p1, p2, e0, e1);
-  Xb[(2 * p) * ly + col] = e0;
-  Xb[(2 * p + 1) * ly + col] = e1;
+  if (act) {
+    circ_carry(kf, p, P, [&](int c) { return Xa[c * ly + col]; }, e0, e1);
+    circ_fwd(x, m, p1, p2, e0, e1);
+    circ_bwd_exit(x, m, p1, p2, e0, e1);
+    Xb[(2 * p) * ly + col] = e0;
+    Xb[(2 * p + 1) * ly + col] = e1;
+  }
   __syncthreads();
-  circ_carry(kb, p, P, [&](int c) { return Xb[c * ly + col]; }, e0, e1);
-  circ_bwd_sink(x, m, p1, p2, e0, e1, sink);
+  if (act) {
+    circ_carry(kb, p, P, [&](int c) { return Xb[c * ly + col]; }, e0, e1);
+    circ_bwd_sink(x, m, p1, p2, e0, e1, sink);
+  }
 }
 
 // Circulant solve of one chunk; X0 and X1 hold 4P x ly entries each. Every
@@ -196,16 +206,21 @@ __device__ __forceinline__ void solve_chunk_circ(const LineArgs& a, const CS* kf
     circ_solve<float>(kf, kb, a.p1f, a.p2f, p, P, m, X0, X1, ly, col, y,
                       [&](int k, float v) { y[k] = v * a.invcf; });
     // neighbour values of the FP32 solution for the residual
-    float e4[4];
-    edges(y, m, e4);
+    const bool act = p < P;
+    if (act) {
+      float e4[4];
+      edges(y, m, e4);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) X0[(4 * p + q) * ly + col] = e4[q];
+      for (int q = 0; q < 4; ++q) X0[(4 * p + q) * ly + col] = e4[q];
+    }
     __syncthreads();
-    const int pm = p == 0 ? P - 1 : p - 1, pp = p == P - 1 ? 0 : p + 1;
-    const float z[4] = {X0[(4 * pm + 2) * ly + col], X0[(4 * pm + 3) * ly + col],
-                        X0[(4 * pp) * ly + col], X0[(4 * pp + 1) * ly + col]};
-    const double row[5] = {a.bd[0], a.bd[1], a.bd[2], a.bd[1], a.bd[0]};
-    residual64([&](int) { return row; }, m, rhs, y, z, rhs);
+    if (act) {
+      const int pm = p == 0 ? P - 1 : p - 1, pp = p == P - 1 ? 0 : p + 1;
+      const float z[4] = {X0[(4 * pm + 2) * ly + col], X0[(4 * pm + 3) * ly + col],
+                          X0[(4 * pp) * ly + col], X0[(4 * pp + 1) * ly + col]};
+      const double row[5] = {a.bd[0], a.bd[1], a.bd[2], a.bd[1], a.bd[0]};
+      residual64([&](int) { return row; }, m, rhs, y, z, rhs);
+    }
     float d[CHUNK];
 #pragma unroll
     for (int k = 0; k < CHUNK; ++k) d[k] = (float)rhs[k];
@@ -279,9 +294,11 @@ struct YzWaves {
 
 // FULL: every chunk has CHUNK rows (N a multiple of CHUNK), so the row
 // predicates vanish at compile time. The circulant solve implies periodic lines.
+// The body takes the workgroup's line-group index `blk` (blockIdx.x for one
+// field, the member-local index for a grouped launch over several fields).
 template <typename TS, int KIND, int SOLVE, int MODE, int LY, bool FULL, int PRE>
-__global__ void __launch_bounds__(WG_MAX) __attribute__((amdgpu_waves_per_eu(YzWaves<TS, SOLVE, MODE, PRE>::v, 8)))
-k_line_yz(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
+__device__ __forceinline__ void line_yz_body(const LineArgs& a, long long blk, const TS* __restrict__ in,
+                                             TS* __restrict__ out) {
   constexpr int R = Halo<KIND>::R;
   constexpr bool PER = SOLVE == SOLVE_CIRC;
   using CS = typename SolveT<MODE>::type;
@@ -294,12 +311,15 @@ k_line_yz(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
   CS* Y = reinterpret_cast<CS*>(smem);
   Tabs t;
   if (SOLVE == SOLVE_LU) t = stage_tables<MODE>(a, smem + align16((size_t)4 * a.P * LY * sizeof(CS)), tid, nthr);
-  const long long L = (long long)blockIdx.x * LY + lane;
+  const long long L = blk * LY + lane;
   const bool valid = L < a.nlines;
   const long long Lc = valid ? L : (long long)a.nlines - 1;
   const long long base = (Lc % a.A) + a.B * (Lc / a.A);
-  const int s = FULL ? p * CHUNK : a.cstart[p];
-  const int m = FULL ? CHUNK : a.cstart[p + 1] - s;
+  // act: this thread's chunk exists (a grouped launch shapes its workgroups
+  // for the member with the most chunks)
+  const bool act = FULL || p < a.P;
+  const int s = FULL ? p * CHUNK : (act ? a.cstart[p] : 0);
+  const int m = FULL ? CHUNK : (act ? a.cstart[p + 1] - s : 0);
   const __amdgpu_buffer_rsrc_t rin = buf_rsrc(in, a.nbytes), rout = buf_rsrc(out, a.nbytes);
   const int lane_off = (int)(base * (long long)sizeof(TS));
   const int row_bytes = (int)(a.stride * (long long)sizeof(TS));
@@ -323,7 +343,7 @@ k_line_yz(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
   for (int w = 0; w < CHUNK + 2 * R; ++w) {
     int row = s - R + w;
     TS v = (TS)0;
-    if (w < m + 2 * R) {
+    if (act && w < m + 2 * R) {
       if (PER || a.periodic) {
         row = row < 0 ? row + a.N : (row >= a.N ? row - a.N : row);
         v = ld(rin, row);
@@ -339,7 +359,7 @@ k_line_yz(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
   }
   double rhs[CHUNK];
   rhs_interior<KIND>(a, win, rhs, m);
-  if (!PER) rhs_boundary(a, s, m, rhs, [&](int i) { return ld(rin, i); });
+  if (!PER && act) rhs_boundary(a, s, m, rhs, [&](int i) { return ld(rin, i); });
   // row k of the result: metric scale, filters add v back, fused epilogue, store
   auto emit = [&](int k, double y) {
     if (!valid) return;
@@ -367,6 +387,12 @@ k_line_yz(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
     for (int k = 0; k < CHUNK; ++k)
       if (k < m) emit(k, rhs[k]);
   }
+}
+
+template <typename TS, int KIND, int SOLVE, int MODE, int LY, bool FULL, int PRE>
+__global__ void __launch_bounds__(WG_MAX) __attribute__((amdgpu_waves_per_eu(YzWaves<TS, SOLVE, MODE, PRE>::v, 8)))
+k_line_yz(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
+  line_yz_body<TS, KIND, SOLVE, MODE, LY, FULL, PRE>(a, blockIdx.x, in, out);
 }
 
 // 16-byte copy from HBM straight into LDS (global_load_lds_dwordx4): lane i's
@@ -514,8 +540,8 @@ __device__ __forceinline__ void vzip(float4& r, const float4& o, F f) {
 // windows are in registers. Circulant solve: the carry exchange sits after
 // the tile.
 template <typename TS, int KIND, int SOLVE, int MODE, bool FULL>
-__global__ void __launch_bounds__(WG_MAX) __attribute__((amdgpu_waves_per_eu(PYR_X_WAVES_PER_EU, 8)))
-k_line_x(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
+__device__ __forceinline__ void line_x_body(const LineArgs& a, long long blk, const TS* __restrict__ in,
+                                            TS* __restrict__ out) {
   constexpr int R = Halo<KIND>::R;
   constexpr bool PER = SOLVE == SOLVE_CIRC;
   using CS = typename SolveT<MODE>::type;
@@ -528,7 +554,7 @@ k_line_x(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
   const int nthr = blockDim.x;
   const int l = tid % LX;
   const int p = tid / LX;
-  const long long L0 = (long long)blockIdx.x * LX;
+  const long long L0 = blk * LX;
   const long long rem = (long long)a.nlines - L0;
   const int nval = (int)(rem < LX ? rem : LX);
   const TS* src = in + L0 * N;
@@ -552,14 +578,17 @@ k_line_x(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
   __syncthreads();
   const int lr = l < nval ? l : 0;
   const TS* row0 = tile + lr * pitch;
-  const int s = FULL ? p * CHUNK : a.cstart[p];
-  const int m = FULL ? CHUNK : a.cstart[p + 1] - s;
+  // act: this thread's chunk exists (a grouped launch sizes its workgroups
+  // for the member with the most threads per workgroup)
+  const bool act = p < a.P;
+  const int s = FULL ? p * CHUNK : (act ? a.cstart[p] : 0);
+  const int m = FULL ? CHUNK : (act ? a.cstart[p + 1] - s : 0);
   TS win[CHUNK + 2 * R];
 #pragma unroll
   for (int w = 0; w < CHUNK + 2 * R; ++w) {
     int row = s - R + w;
     TS v = (TS)0;
-    if (w < m + 2 * R) {
+    if (act && w < m + 2 * R) {
       if (PER || a.periodic) {
         row = row < 0 ? row + N : (row >= N ? row - N : row);
         v = row0[row];
@@ -571,7 +600,7 @@ k_line_x(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
   }
   double rhs[CHUNK];
   rhs_interior<KIND>(a, win, rhs, m);
-  if (!PER) rhs_boundary(a, s, m, rhs, [&](int i) { return row0[i]; });
+  if (!PER && act) rhs_boundary(a, s, m, rhs, [&](int i) { return row0[i]; });
   if (SOLVE == SOLVE_LU) {
     __syncthreads();  // the tile is consumed; reuse its LDS
     CS* Y = reinterpret_cast<CS*>(smem);
@@ -584,7 +613,7 @@ k_line_x(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
   }
   TS* rowo = tile + l * pitch;
   auto emit = [&](int k, double y) {
-    if (l < nval) rowo[s + k] = (TS)(y * a.scale);
+    if (act && l < nval) rowo[s + k] = (TS)(y * a.scale);
   };
   if (SOLVE == SOLVE_CIRC) {
     // the first carry barrier also retires every read of the tile
@@ -628,6 +657,55 @@ k_line_x(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
   }
 }
 
+template <typename TS, int KIND, int SOLVE, int MODE, bool FULL>
+__global__ void __launch_bounds__(WG_MAX) __attribute__((amdgpu_waves_per_eu(PYR_X_WAVES_PER_EU, 8)))
+k_line_x(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
+  line_x_body<TS, KIND, SOLVE, MODE, FULL>(a, blockIdx.x, in, out);
+}
+
+// Grouped launches: one launch applies the same operator to the fields of
+// several simulations (an ensemble's members, each at `off` elements into
+// one allocation), with lines of any lengths. Entry j covers workgroups
+// [blk0_j, blk0_{j+1}); a workgroup finds its member by binary search
+// (wave-uniform) and runs the single-field body with that member's line
+// arguments. Workgroups are shaped for the member with the most chunks (or
+// threads); the bodies leave threads without a chunk idle at the barriers.
+struct GrpEntry {
+  LineArgs a;
+  long long off;
+  long long blk0;
+};
+
+__device__ __forceinline__ int grp_member(const GrpEntry* __restrict__ g, int M, long long b) {
+  int lo = 0, hi = M - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (g[mid].blk0 <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  return lo;
+}
+
+template <typename TS, int KIND, int SOLVE, int MODE>
+__global__ void __launch_bounds__(WG_MAX) __attribute__((amdgpu_waves_per_eu(PYR_X_WAVES_PER_EU, 8)))
+k_line_x_grp(const GrpEntry* __restrict__ g, int M, const TS* __restrict__ in, TS* __restrict__ out) {
+  const int j = grp_member(g, M, blockIdx.x);
+  // a private copy: its fields load once (wave-uniform) and no store of the
+  // body can alias them
+  const LineArgs a = g[j].a;
+  const long long off = g[j].off, b0 = g[j].blk0;
+  line_x_body<TS, KIND, SOLVE, MODE, false>(a, (long long)blockIdx.x - b0, in + off, out + off);
+}
+
+template <typename TS, int KIND, int SOLVE, int MODE, int LY, int PRE>
+__global__ void __launch_bounds__(WG_MAX) __attribute__((amdgpu_waves_per_eu(YzWaves<TS, SOLVE, MODE, PRE>::v, 8)))
+k_line_yz_grp(const GrpEntry* __restrict__ g, int M, const TS* __restrict__ in, TS* __restrict__ out) {
+  const int j = grp_member(g, M, blockIdx.x);
+  const LineArgs a = g[j].a;
+  const long long off = g[j].off, b0 = g[j].blk0;
+  line_yz_body<TS, KIND, SOLVE, MODE, LY, false, PRE>(a, (long long)blockIdx.x - b0, in + off, out + off);
+}
+
 // null operator of a direction with fewer than 4 points (zero or copy)
 template <typename TS>
 __global__ void k_null(long long n, int copy, int epi, double c, const TS* __restrict__ in,
@@ -641,17 +719,20 @@ __global__ void k_null(long long n, int copy, int epi, double c, const TS* __res
 
 // RK4 stage update (pyranda.py:800-804), one pass over F, PHI and U. With
 // A == 0 (the first stage) A*PHI is zero and PHI is not read, so it needs no
-// clearing before the step. dtp (if not null) holds dt in device memory.
+// clearing before the step. dtp (if not null) holds dt in device memory: one
+// value, or with mid (the member index of every point of an ensemble's
+// fields) one value per member.
 template <typename TS>
 __global__ void k_rk_update(long long n, const TS* __restrict__ F, double* __restrict__ PHI,
                             const TS* __restrict__ Uin, TS* __restrict__ Uout,
-                            double* __restrict__ U64, const double* __restrict__ dtp, double dt,
-                            double A, double B) {
-  if (dtp) dt = *dtp;
+                            double* __restrict__ U64, const double* __restrict__ dtp,
+                            const int* __restrict__ mid, double dt, double A, double B) {
+  if (dtp && !mid) dt = *dtp;
   for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x) {
+    const double dti = mid ? dtp[mid[i]] : dt;
     const double tmp1 = A == 0.0 ? 0.0 : A * PHI[i];
-    const double phi = dt * (double)F[i] + tmp1;
+    const double phi = dti * (double)F[i] + tmp1;
     PHI[i] = phi;
     const double tmp2 = B * phi;
     if (U64) {
@@ -1303,6 +1384,199 @@ static int emulate(pyr_ctx* c, int op, int dir, const TS* in, TS* out) {
 }
 
 // ---------------------------------------------------------------------------
+// groups: one operator over the fields of several simulations
+// ---------------------------------------------------------------------------
+// Members of a group are contexts of any grid sizes whose fields live in one
+// allocation (member m at off[m] elements). An operator along one direction
+// runs as ONE grouped launch over every member (workgroups shaped for the
+// member with the most chunks per line; threads without a chunk only take
+// the barriers), plus a k_null launch for a member whose direction is a null
+// operator. The per-member line arguments of a
+// launch are built once per (operator, direction, precision, epilogue) and
+// kept on the device.
+struct GrpLaunch {
+  GrpEntry* dev = nullptr;
+  int M = 0;
+  long long nblocks = 0;
+  dim3 block;
+  size_t shm = 0;
+  int kind = 0, solve = 0, ly = 0, pre = 0;
+  bool x = false;
+};
+struct GrpNull {
+  int member;
+  int copy;
+  double ec;
+};
+struct GrpPlan {
+  std::vector<GrpLaunch> launches;
+  std::vector<GrpNull> nulls;
+};
+struct GrpKey {
+  int op, dir, prec, epi, power, aligned;
+  double ec;
+  bool operator<(const GrpKey& o) const {
+    if (op != o.op) return op < o.op;
+    if (dir != o.dir) return dir < o.dir;
+    if (prec != o.prec) return prec < o.prec;
+    if (epi != o.epi) return epi < o.epi;
+    if (power != o.power) return power < o.power;
+    if (aligned != o.aligned) return aligned < o.aligned;
+    return ec < o.ec;
+  }
+};
+struct pyr_group {
+  std::vector<pyr_ctx*> mem;
+  std::vector<long long> off;
+  hipStream_t stream = nullptr;
+  std::map<GrpKey, GrpPlan> plans;
+  ~pyr_group() {
+    for (auto& kv : plans)
+      for (auto& l : kv.second.launches)
+        if (l.dev) (void)hipFree(l.dev);
+  }
+};
+
+template <typename TS, int MODE>
+static int grp_build(pyr_group* g, const GrpKey& key, GrpPlan& gp) {
+  using CS = typename SolveT<MODE>::type;
+  constexpr int VW = Vec16<TS>::W;
+  const int M = (int)g->mem.size();
+  std::vector<int> ms;
+  for (int m = 0; m < M; ++m) {
+    pyr_ctx* c = g->mem[m];
+    const OpPlan& pl = c->plan[key.op][key.dir];
+    const double ec = key.ec * std::pow(c->dmesh[key.dir], (double)key.power);
+    if (pl.null_op) gp.nulls.push_back({m, pl.null_option == 1 ? 1 : 0, ec});
+    else ms.push_back(m);
+  }
+  if (ms.empty()) return 0;
+  const OpPlan& pl0 = g->mem[ms[0]]->plan[key.op][key.dir];
+  const int sv = plan_solve(g->mem[ms[0]], pl0);
+  if (sv == SOLVE_LU) return fail("grouped operators need periodic lines (circulant solve) or explicit filters");
+  GrpLaunch L;
+  L.kind = pl0.kind;
+  L.solve = sv;
+  L.x = key.dir == 0;
+  std::vector<GrpEntry> host(ms.size());
+  int Pmax = 1;
+  for (size_t j = 0; j < ms.size(); ++j) {
+    pyr_ctx* c = g->mem[ms[j]];
+    const OpPlan& pl = c->plan[key.op][key.dir];
+    if (plan_solve(c, pl) != sv || pl.kind != L.kind)
+      return fail("group members differ in boundary conditions along a direction");
+    const double ec = key.ec * std::pow(c->dmesh[key.dir], (double)key.power);
+    host[j].a = make_args(c, pl, c->dplan[key.op][key.dir], key.dir, key.epi, ec, key.prec);
+    host[j].off = g->off[ms[j]];
+    Pmax = std::max(Pmax, pl.P);
+  }
+  long long blk = 0;
+  if (L.x) {
+    // each member keeps its own lines per workgroup (its LDS within the
+    // budget); the launch takes the most threads and LDS of any member
+    int nthr = 0;
+    size_t shm = 0;
+    for (auto& e : host) {
+      const int P = e.a.P;
+      e.a.vec = key.aligned && (e.a.N % VW == 0) && ((e.off * (long long)sizeof(TS)) % 16 == 0);
+      e.a.pitch = e.a.vec ? e.a.N + VW : e.a.N + 1;
+      auto lds = [&](int lx) {
+        size_t t = align16((size_t)lx * e.a.pitch * sizeof(TS));
+        if (sv == SOLVE_CIRC) t += 2 * align16((size_t)4 * P * lx * sizeof(CS));
+        return t;
+      };
+      int lx = WG_MAX / P;
+      if (lx > 64) lx = 64;
+      while (lx > 1 && lds(lx) > (size_t)g->mem[0]->x_lds_kb * 1024) lx /= 2;
+      e.a.lx = lx;
+      e.blk0 = blk;
+      blk += (e.a.nlines + lx - 1) / lx;
+      nthr = std::max(nthr, lx * P);
+      shm = std::max(shm, lds(lx));
+    }
+    L.block = dim3(nthr);
+    L.shm = shm;
+  } else {
+    const int ly = Pmax <= 8 ? 64 : 32;
+    for (auto& e : host) {
+      e.blk0 = blk;
+      blk += (e.a.nlines + ly - 1) / ly;
+    }
+    L.ly = ly;
+    L.block = dim3(ly, Pmax);
+    L.shm = sv == SOLVE_CIRC ? 2 * align16((size_t)4 * Pmax * ly * sizeof(CS)) : 0;
+    L.pre = host[0].a.addv ? PRE_V : ((key.epi == EPI_ADD || key.epi == EPI_MAXABS) ? PRE_OLD : PRE_NONE);
+  }
+  L.nblocks = blk;
+  L.M = (int)host.size();
+  HIPCHECK(hipMalloc(&L.dev, host.size() * sizeof(GrpEntry)));
+  HIPCHECK(hipMemcpy(L.dev, host.data(), host.size() * sizeof(GrpEntry), hipMemcpyHostToDevice));
+  gp.launches.push_back(L);
+  return 0;
+}
+
+template <typename TS, int KIND, int SOLVE, int MODE>
+static int grp_launch_typed(pyr_group* g, const GrpLaunch& L, const TS* in, TS* out) {
+  if (L.nblocks <= 0) return 0;
+  const dim3 grid((unsigned)L.nblocks);
+  if (L.x) {
+    auto k = k_line_x_grp<TS, KIND, SOLVE, MODE>;
+    if (set_lds(k, L.shm)) return 1;
+    hipLaunchKernelGGL(k, grid, L.block, L.shm, g->stream, L.dev, L.M, in, out);
+  } else {
+    auto pick = [&](auto k64, auto k32) {
+      return L.ly == 64 ? k64 : k32;
+    };
+    auto k = pick(k_line_yz_grp<TS, KIND, SOLVE, MODE, 64, PRE_NONE>, k_line_yz_grp<TS, KIND, SOLVE, MODE, 32, PRE_NONE>);
+    if (L.pre == PRE_OLD)
+      k = pick(k_line_yz_grp<TS, KIND, SOLVE, MODE, 64, PRE_OLD>, k_line_yz_grp<TS, KIND, SOLVE, MODE, 32, PRE_OLD>);
+    if (L.pre == PRE_V) {
+      if constexpr (KIND == K_R4)
+        k = pick(k_line_yz_grp<TS, KIND, SOLVE, MODE, 64, PRE_V>, k_line_yz_grp<TS, KIND, SOLVE, MODE, 32, PRE_V>);
+      else
+        return fail("filter epilogue on a non-filter kernel");
+    }
+    if (set_lds(k, L.shm)) return 1;
+    hipLaunchKernelGGL(k, grid, L.block, L.shm, g->stream, L.dev, L.M, in, out);
+  }
+  HIPCHECK(hipGetLastError());
+  return 0;
+}
+
+template <typename TS, int MODE>
+static int grp_apply(pyr_group* g, const GrpKey& key, const TS* in, TS* out) {
+  auto it = g->plans.find(key);
+  if (it == g->plans.end()) {
+    GrpPlan gp;
+    if (grp_build<TS, MODE>(g, key, gp)) {
+      for (auto& l : gp.launches)
+        if (l.dev) (void)hipFree(l.dev);
+      return 1;
+    }
+    it = g->plans.emplace(key, std::move(gp)).first;
+  }
+  for (const GrpLaunch& L : it->second.launches) {
+    int r;
+    switch (L.kind) {
+      case K_D1: r = grp_launch_typed<TS, K_D1, SOLVE_CIRC, MODE>(g, L, in, out); break;
+      case K_R3: r = grp_launch_typed<TS, K_R3, SOLVE_CIRC, MODE>(g, L, in, out); break;
+      default:
+        r = L.solve == SOLVE_CIRC ? grp_launch_typed<TS, K_R4, SOLVE_CIRC, MODE>(g, L, in, out)
+                                  : grp_launch_typed<TS, K_R4, SOLVE_NONE, MODE>(g, L, in, out);
+    }
+    if (r) return 1;
+  }
+  for (const GrpNull& z : it->second.nulls) {
+    const long long n = (long long)g->mem[z.member]->npts();
+    const long long o = g->off[z.member];
+    hipLaunchKernelGGL((k_null<TS>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, g->stream, n, z.copy,
+                       key.epi, z.ec, in + o, out + o);
+    HIPCHECK(hipGetLastError());
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
 // C-ABI
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -1442,19 +1716,19 @@ int pyr_divergence_r32(pyr_ctx* c, const float* fx, const float* fy, const float
   return sum3(c, OP_D1, PYR_R32, fx, fy, fz, o);
 }
 
-static int rk_update(pyr_ctx* c, int prec, long long n, const void* F, double* PHI, const void* Uin,
-                     void* Uout, double* U64, const double* dtp, double dt, double A, double B) {
-  if (check_ctx(c)) return 1;
+static int rk_update(hipStream_t stream, int prec, long long n, const void* F, double* PHI, const void* Uin,
+                     void* Uout, double* U64, const double* dtp, const int* mid, double dt, double A,
+                     double B) {
   if (!F || !PHI || !Uout || (!Uin && !U64)) return fail("null pointer to pyr_rk_update");
   if (n <= 0) return 0;
   long long blocks = (n + 1023) / 1024;
   if (blocks > 8192) blocks = 8192;
   if (prec == PYR_F64)
-    hipLaunchKernelGGL((k_rk_update<double>), dim3((unsigned)blocks), dim3(256), 0, c->stream, n,
-                       (const double*)F, PHI, (const double*)Uin, (double*)Uout, U64, dtp, dt, A, B);
+    hipLaunchKernelGGL((k_rk_update<double>), dim3((unsigned)blocks), dim3(256), 0, stream, n,
+                       (const double*)F, PHI, (const double*)Uin, (double*)Uout, U64, dtp, mid, dt, A, B);
   else if (prec == PYR_R32)
-    hipLaunchKernelGGL((k_rk_update<float>), dim3((unsigned)blocks), dim3(256), 0, c->stream, n,
-                       (const float*)F, PHI, (const float*)Uin, (float*)Uout, U64, dtp, dt, A, B);
+    hipLaunchKernelGGL((k_rk_update<float>), dim3((unsigned)blocks), dim3(256), 0, stream, n,
+                       (const float*)F, PHI, (const float*)Uin, (float*)Uout, U64, dtp, mid, dt, A, B);
   else
     return fail("bad precision");
   HIPCHECK(hipGetLastError());
@@ -1463,13 +1737,72 @@ static int rk_update(pyr_ctx* c, int prec, long long n, const void* F, double* P
 
 int pyr_rk_update(pyr_ctx* c, int prec, long long n, const void* F, double* PHI, const void* Uin,
                   void* Uout, double* U64, double dt, double A, double B) {
-  return rk_update(c, prec, n, F, PHI, Uin, Uout, U64, nullptr, dt, A, B);
+  if (check_ctx(c)) return 1;
+  return rk_update(c->stream, prec, n, F, PHI, Uin, Uout, U64, nullptr, nullptr, dt, A, B);
 }
 
 int pyr_rk_update_dt(pyr_ctx* c, int prec, long long n, const void* F, double* PHI, const void* Uin,
                      void* Uout, double* U64, const double* dt, double A, double B) {
+  if (check_ctx(c)) return 1;
   if (!dt) return fail("null dt pointer to pyr_rk_update_dt");
-  return rk_update(c, prec, n, F, PHI, Uin, Uout, U64, dt, 0.0, A, B);
+  return rk_update(c->stream, prec, n, F, PHI, Uin, Uout, U64, dt, nullptr, 0.0, A, B);
+}
+
+int pyr_rk_update_member_dt(void* stream, int prec, long long n, const void* F, double* PHI,
+                            const void* Uin, void* Uout, double* U64, const double* dt_member,
+                            const int* member_of_point, double A, double B) {
+  if (!dt_member || !member_of_point) return fail("null pointer to pyr_rk_update_member_dt");
+  return rk_update((hipStream_t)stream, prec, n, F, PHI, Uin, Uout, U64, dt_member, member_of_point, 0.0, A, B);
+}
+
+int pyr_group_create(pyr_group** out, int M, pyr_ctx* const* members, const long long* offsets) {
+  if (!out || M < 1 || !members || !offsets) return fail("bad arguments to pyr_group_create");
+  std::unique_ptr<pyr_group> g(new pyr_group());
+  for (int m = 0; m < M; ++m) {
+    if (check_ctx(members[m])) return 1;
+    if (offsets[m] < 0) return fail("negative member offset");
+    g->mem.push_back(members[m]);
+    g->off.push_back(offsets[m]);
+  }
+  *out = g.release();
+  return 0;
+}
+
+int pyr_group_destroy(pyr_group* g) {
+  delete g;
+  return 0;
+}
+
+int pyr_group_set_stream(pyr_group* g, void* stream) {
+  if (!g) return fail("null group");
+  g->stream = (hipStream_t)stream;
+  return 0;
+}
+
+int pyr_group_apply(pyr_group* g, int op, int dir, int prec, const void* in, void* out, int epi, double ec,
+                    int mesh_power) {
+  if (!g) return fail("null group");
+  if (op < 0 || op >= NOPS || dir < 0 || dir > 2) return fail("bad operator or direction");
+  if (!in || !out) return fail("null field pointer");
+  if (in == out) return fail("input and output must not alias");
+  if (mesh_power < 0 || mesh_power > 2) return fail("mesh_power must be 0, 1 or 2");
+  GrpKey key;
+  key.op = op;
+  key.dir = dir;
+  key.prec = prec;
+  key.epi = epi;
+  key.power = mesh_power;
+  key.aligned = ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 16 == 0);
+  key.ec = ec;
+  if (prec == PYR_F64) return grp_apply<double, MODE_F64>(g, key, (const double*)in, (double*)out);
+  if (prec == PYR_R32) {
+    const int r = g->mem[0]->r32_solve;
+    for (pyr_ctx* c : g->mem)
+      if (c->r32_solve != r) return fail("group members differ in their FP32 solve mode");
+    if (r == PYR_R32_SOLVE_FP64) return grp_apply<float, MODE_S32>(g, key, (const float*)in, (float*)out);
+    return grp_apply<float, MODE_R32>(g, key, (const float*)in, (float*)out);
+  }
+  return fail("bad precision");
 }
 
 int pyr_host_weights(int op, double* ali, double* ari, double* alb1, double* alb2, double* arb1,

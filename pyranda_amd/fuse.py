@@ -134,6 +134,31 @@ def _device_scalar(v):
             and v.dtype in (torch.float64, torch.float32))
 
 
+class GroupShape(tuple):
+    """Shape (n, 1, 1) of the fields of a group of simulations stored in one
+    allocation, carrying `mid` (the member index of every point, int32 on the
+    device) and `nmem`: a 1-D tensor of `nmem` values is then one scalar per
+    member (dt, say), read per point as value[mid[i]]."""
+
+    def __new__(cls, n, mid, nmem):
+        t = super().__new__(cls, (n, 1, 1))
+        t.mid = mid
+        t.nmem = nmem
+        return t
+
+
+def _member_scalar(v, shape):
+    return (isinstance(shape, GroupShape) and torch.is_tensor(v) and v.dim() == 1 and v.is_cuda
+            and v.shape[0] == shape.nmem and v.dtype in (torch.float64, torch.float32))
+
+
+def _expand_members(vals, shape):
+    """Per-member values expanded to fields (for the torch path)."""
+    if not isinstance(shape, GroupShape):
+        return vals
+    return [v[shape.mid.long()].reshape(shape) if _member_scalar(v, shape) else v for v in vals]
+
+
 def _field_ok(t, shape):
     if not torch.is_tensor(t) or t.dtype not in (torch.float64, torch.float32) or not t.is_cuda:
         return False
@@ -176,9 +201,12 @@ class FusedSubtree:
                 decl.append("  const double x%d = (double)*(const %s*)a.p[%d];" % (i, ct, ip))
                 ip += 1
             else:
-                ct = "double" if s == "T64" else "float"
+                ct = "double" if s in ("T64", "G64") else "float"
                 decl.append("  const %s* __restrict__ p%d = (const %s*)a.p[%d];" % (ct, i, ct, ip))
                 ip += 1
+        if any(s[0] == "G" for s in sig):
+            decl.append("  const int* __restrict__ pmid = (const int*)a.p[%d];" % ip)
+            ip += 1
         ot = "double" if out_dt == "f64" else "float"
         lines += decl
         lines.append("  %s* __restrict__ out = (%s*)a.p[%d];" % (ot, ot, ip))
@@ -187,6 +215,8 @@ class FusedSubtree:
         for i, s in enumerate(sig):
             if s in ("T64", "T32"):
                 lines.append("    const double x%d = (double)p%d[i];" % (i, i))
+            elif s in ("G64", "G32"):
+                lines.append("    const double x%d = (double)p%d[pmid[i]];" % (i, i))
         lines.append("    out[i] = (%s)(%s);" % (ot, self.body))
         lines.append("  }")
         lines.append("}")
@@ -225,6 +255,10 @@ class FusedSubtree:
                 # scalar it does not decide the result type
                 sig.append("D64" if v.dtype == torch.float64 else "D32")
                 fields.append(v)
+            elif _member_scalar(v, shape):
+                # one value per member of a group (read per point by member)
+                sig.append("G64" if v.dtype == torch.float64 else "G32")
+                fields.append(v)
             elif _field_ok(v, shape):
                 sig.append("T64" if v.dtype == torch.float64 else "T32")
                 any64 |= v.dtype == torch.float64
@@ -232,11 +266,13 @@ class FusedSubtree:
                 nfield += 1
                 dev = v.device
             else:
-                return self._py(*vals)
+                return self._py(*_expand_members(vals, shape))
         if not nfield or self.is_bool:
-            return self._py(*vals)
+            return self._py(*_expand_members(vals, shape) if nfield else vals)
+        if any(k[0] == "G" for k in sig):
+            fields.append(shape.mid)
         if len(fields) + 1 > _lib.PW_MAX_PTRS or len(scalars) > _lib.PW_MAX_SCALARS:
-            return self._py(*vals)
+            return self._py(*_expand_members(vals, shape))
         out_dt = "f64" if any64 else "f32"
         fn = self.kernel(tuple(sig), out_dt)
         nx, ny, nz = shape
@@ -327,7 +363,8 @@ class FusedRHS:
         self._fn = eval(compile(lam, "<eom %s>" % src[:40], "eval"), globals_)
 
     def __call__(self, sim):
-        return self._fn(sim, tuple(sim.mesh.shape), self.subtrees)
+        shape = sim.mesh.shape
+        return self._fn(sim, shape if isinstance(shape, GroupShape) else tuple(shape), self.subtrees)
 
 
 _RED = {
@@ -412,6 +449,154 @@ class FusedReduce:
         import numpy as _np
         vals = part.cpu().numpy()
         return float(_np.max(vals) if self.op == "max" else _np.min(vals))
+
+
+class SegmentedReduce:
+    """max/min of one pointwise expression over each member of a group of
+    simulations stored in one allocation (member m: points seg[m] ..
+    seg[m+1]), in one launch: each workgroup reduces a share of one member's
+    points (workgroups blk[m] .. blk[m+1] belong to member m, found by binary
+    search) to a partial; a second launch (one workgroup per member) reduces
+    each member's partials. Scalars are per member: x<nfields + j> reads
+    sc[m * nscalars + j]. Returns one FP64 value per member (device)."""
+
+    def __init__(self, op, body, nfields, nscalars, tag=""):
+        self.op, self.body, self.nf, self.ns, self.tag = op, body, nfields, nscalars, tag
+        self._kernels = {}
+
+    def _source(self, sig):
+        ident, comb = _RED[self.op]
+        name = "pyr_segred_" + hashlib.sha1((self.op + self.body + repr(sig)).encode()).hexdigest()[:16]
+        nf = len(sig)
+        L = [PRELUDE, 'extern "C" __global__ void __launch_bounds__(256) %s(PyrPwArgs a) {' % name]
+        for i, t in enumerate(sig):
+            ct = "double" if t == "T64" else "float"
+            L.append("  const %s* __restrict__ p%d = (const %s*)a.p[%d];" % (ct, i, ct, i))
+        L.append("  double* __restrict__ part = (double*)a.p[%d];" % nf)
+        L.append("  const long long* __restrict__ seg = (const long long*)a.p[%d];" % (nf + 1))
+        L.append("  const int* __restrict__ blk = (const int*)a.p[%d];" % (nf + 2))
+        L.append("  const double* __restrict__ sc = (const double*)a.p[%d];" % (nf + 3))
+        L.append("  const int M = (int)a.n;")
+        L.append("  int lo = 0, hi = M - 1;")
+        L.append("  while (lo < hi) { const int md = (lo + hi + 1) >> 1; if (blk[md] <= (int)blockIdx.x) lo = md; else hi = md - 1; }")
+        L.append("  const int m = lo;")
+        for j in range(self.ns):
+            L.append("  const double x%d = sc[m * %d + %d];" % (self.nf + j, max(self.ns, 1), j))
+        L.append("  const long long s1 = seg[2 * m + 1];")
+        L.append("  const long long nb = blk[m + 1] - blk[m];")
+        L.append("  double acc = %s;" % ident)
+        L.append("  for (long long i = seg[2 * m] + (long long)(blockIdx.x - blk[m]) * blockDim.x + threadIdx.x;"
+                 " i < s1; i += nb * blockDim.x) {")
+        for i in range(nf):
+            L.append("    const double x%d = (double)p%d[i];" % (i, i))
+        L.append("    const double v = (%s);" % self.body)
+        L.append("    acc = %s;" % comb)
+        L.append("  }")
+        L.append("  for (int off = 32; off > 0; off >>= 1) {")
+        L.append("    const double v = __shfl_xor(acc, off, 64);")
+        L.append("    acc = %s;" % comb)
+        L.append("  }")
+        L.append("  __shared__ double wsum[4];")
+        L.append("  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;")
+        L.append("  __syncthreads();")
+        L.append("  if (threadIdx.x == 0) {")
+        L.append("    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { const double v = wsum[w]; acc = %s; }" % comb)
+        L.append("    part[blockIdx.x] = acc;")
+        L.append("  }")
+        L.append("}")
+        return name, "\n".join(L)
+
+    def _fn(self, key, src_fn):
+        fn = self._kernels.get(key)
+        if fn is None:
+            name, src = src_fn()
+            h = ctypes.c_void_p()
+            L = _lib.lib()
+            if L.pyr_pw_compile(src.encode(), name.encode(), ctypes.byref(h)) != 0:
+                raise _lib.PyrandaAMDError(L.pyr_pw_last_error().decode())
+            fn = self._kernels[key] = h
+        return fn
+
+    def __call__(self, seg, fields, member_scalars):
+        """seg: SegmentTable of the group; member_scalars: FP64 device tensor
+        (M, nscalars) (or None without scalars)."""
+        sig = tuple("T64" if f.dtype == torch.float64 else "T32" for f in fields)
+        fn = self._fn(sig, lambda: self._source(sig))
+        dev = fields[0].device
+        part = torch.empty(seg.nblocks, dtype=torch.float64, device=dev)
+        sc = member_scalars if member_scalars is not None else seg.zero
+        ptrs = (ctypes.c_void_p * (len(fields) + 4))(*[f.data_ptr() for f in fields], part.data_ptr(),
+                                                     seg.seg.data_ptr(), seg.blk.data_ptr(), sc.data_ptr())
+        one = (ctypes.c_double * 1)(0.0)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        L = _lib.lib()
+        _lib.check_pw(L.pyr_pw_launch_blocks(fn, stream, seg.nmem, seg.nblocks, len(fields) + 4, ptrs, 0, one))
+        # second pass: member m reduces its partials blk[m] .. blk[m+1]
+        fin = _segfinal(self.op)
+        out = torch.empty(seg.nmem, dtype=torch.float64, device=dev)
+        ptrs2 = (ctypes.c_void_p * 4)(part.data_ptr(), seg.pseg.data_ptr(), seg.blk.data_ptr(), out.data_ptr())
+        _lib.check_pw(L.pyr_pw_launch_blocks(fin, stream, seg.nmem, seg.nmem, 4, ptrs2, 0, one))
+        return out
+
+
+_SEGFIN = {}
+
+
+def _segfinal(op):
+    """Per-member reduction of the partials: workgroup m reduces part[pblk[m]
+    .. pblk[m+1]) (pseg/pblk: the partial ranges, one workgroup per member)."""
+    if op not in _SEGFIN:
+        ident, comb = _RED[op]
+        name = "pyr_segfin_" + op
+        L = [PRELUDE, 'extern "C" __global__ void __launch_bounds__(256) %s(PyrPwArgs a) {' % name,
+             "  const double* __restrict__ part = (const double*)a.p[0];",
+             "  const long long* __restrict__ pseg = (const long long*)a.p[1];",
+             "  double* __restrict__ out = (double*)a.p[3];",
+             "  const int m = blockIdx.x;",
+             "  double acc = %s;" % ident,
+             "  for (long long i = pseg[m] + threadIdx.x; i < pseg[m + 1]; i += blockDim.x) {",
+             "    const double v = part[i];",
+             "    acc = %s;" % comb,
+             "  }",
+             "  for (int off = 32; off > 0; off >>= 1) {",
+             "    const double v = __shfl_xor(acc, off, 64);",
+             "    acc = %s;" % comb,
+             "  }",
+             "  __shared__ double wsum[4];",
+             "  if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = acc;",
+             "  __syncthreads();",
+             "  if (threadIdx.x == 0) {",
+             "    for (int w = 1; w < (int)(blockDim.x >> 6); ++w) { const double v = wsum[w]; acc = %s; }" % comb,
+             "    out[m] = acc;",
+             "  }",
+             "}"]
+        h = ctypes.c_void_p()
+        Lb = _lib.lib()
+        if Lb.pyr_pw_compile("\n".join(L).encode(), name.encode(), ctypes.byref(h)) != 0:
+            raise _lib.PyrandaAMDError(Lb.pyr_pw_last_error().decode())
+        _SEGFIN[op] = h
+    return _SEGFIN[op]
+
+
+class SegmentTable:
+    """Member segments of a group's fields and the workgroup partition of a
+    segmented reduction: member m owns points seg[2m] .. seg[2m+1] (members
+    may be separated by alignment padding) and workgroups blk[m] .. blk[m+1]
+    (about four points per thread, at most 64 workgroups per member)."""
+
+    def __init__(self, sizes, offsets, device):
+        import numpy as _np
+        self.nmem = len(sizes)
+        blocks = [max(1, min(64, (n + 1023) // 1024)) for n in sizes]
+        blk = _np.concatenate([[0], _np.cumsum(blocks)]).astype(_np.int32)
+        self.nblocks = int(blk[-1])
+        pairs = []
+        for o, n in zip(offsets, sizes):
+            pairs += [int(o), int(o) + int(n)]
+        self.seg = torch.tensor(pairs, dtype=torch.int64, device=device)
+        self.blk = torch.tensor(blk, dtype=torch.int32, device=device)
+        self.pseg = torch.tensor(blk.astype(_np.int64), dtype=torch.int64, device=device)
+        self.zero = torch.zeros(1, dtype=torch.float64, device=device)
 
 
 _FINALS = {}

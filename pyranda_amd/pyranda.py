@@ -250,9 +250,11 @@ class pyrandaSim:
             if name.startswith("mesh"):
                 continue
             d = var.data
-            # fields take the run's dtype; 0-d device scalars (reductions kept
-            # on the GPU) stay FP64 like the host floats they stand for
-            if torch.is_tensor(d) and d.dim() > 0 and d.is_floating_point() and d.dtype != self.dtype:
+            # fields take the run's dtype; device scalars (reductions kept on
+            # the GPU, one value or one per ensemble member) stay FP64 like
+            # the host floats they stand for
+            if (torch.is_tensor(d) and tuple(d.shape) == tuple(self.mesh.shape) and d.is_floating_point()
+                    and d.dtype != self.dtype):
                 var.data = d.to(self.dtype)
 
     def emptyScalar(self, val=0.0):

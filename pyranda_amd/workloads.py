@@ -438,3 +438,30 @@ def rm_run(ss, nsteps, cfl=1.0, state=None):
         dt = min(dt, dtmax * 1.1)
         dtmax = dt * 1.0
     return {"time": time, "dt": dt, "dtmax": dtmax}
+
+
+def kh_group(pr, sizes, precision="fp32", device=None):
+    """The KH members of `sizes` as ONE pyrandaGroupSim (shared allocation,
+    one launch per operator class for all members) plus its driver state
+    {time, dt}: (M,) FP64 device tensors, one value per member."""
+    from .group import pyrandaGroupSim
+    gamma, R = 5. / 3., 1.0
+    cp = R / (1.0 - 1.0 / gamma)
+    cv = cp - R
+    meshes = []
+    for nx, ny in sizes:
+        meshes.append("""
+        xdom = (0.0, LX,  NX, periodic=True)
+        ydom = (0.0, LY,  NY, periodic=True)
+        """.replace("LX", str(1.0 * (nx - 1.0) / nx)).replace("LY", str(1.0 * (ny - 1.0) / ny))
+                      .replace("NX", str(nx)).replace("NY", str(ny)))
+    kw = {"precision": precision}
+    if device is not None:
+        kw["device"] = device
+    g = pr.pyrandaGroupSim("KH-group", meshes, **kw)
+    g.addPackage(pr.pyrandaTimestep(g))
+    g.EOM(KH_EOM, {"gamma": gamma, "R0": R, "cv": cv})
+    g.setIC(KH_IC, {"gamma": gamma, "u1": 0.5, "u2": -0.5, "p0": 2.5, "rho1": 1.0,
+                    "rho2": 2.0, "L": 0.025, "Vmean": 0.0})
+    dt = g.var("dt").data.to(torch.float64).clone()
+    return g, {"time": torch.zeros_like(dt), "dt": dt}

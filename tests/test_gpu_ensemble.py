@@ -90,3 +90,28 @@ def test_ensemble_graph_matches_members_alone(pr, precision, concurrent):
     torch.cuda.synchronize()
     for i, (m, s) in enumerate(zip(ens.members, solo)):
         assert _same(m.var("rho").data, s.var("rho").data), i
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_group_sim_matches_members_alone(pr, precision):
+    """The ensemble as ONE pyrandaGroupSim (shared allocation, grouped
+    operator launches, per-member dt through segmented reductions) is bit for
+    bit each member run alone, step after step."""
+    from pyranda_amd.workloads import kh_advance, kh_ensemble, kh_group
+    sizes = [(64, 48), (40, 72), (96, 64), (48, 48), (72, 40), (100, 36)]
+    _say("group sim vs members alone", precision, sizes)
+    g, gst = kh_group(pr, sizes, precision=precision)
+    solo, solo_state = kh_ensemble(pr, sizes, precision=precision)
+    for m in range(len(sizes)):
+        assert _same(g.var("dt").data[m], solo[m].var("dt").data), m
+    for step in range(4):
+        kh_advance(g, gst)
+        for s, st in zip(solo, solo_state):
+            kh_advance(s, st)
+        torch.cuda.synchronize()
+        for m, s in enumerate(solo):
+            for k in ("rho", "rhou", "rhov", "Et", "kappa", "mu", "beta"):
+                assert _same(g.member_field(k, m), s.var(k).data), (step, m, k)
+            assert _same(gst["dt"][m], solo_state[m]["dt"]), (step, m, "dt")
+            assert _same(gst["time"][m], solo_state[m]["time"]), (step, m, "time")
+        _say("step", step, "ok")
