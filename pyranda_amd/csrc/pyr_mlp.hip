@@ -284,13 +284,26 @@ __global__ void __launch_bounds__(64 * WAVES) k_mlp(MlpArgs<TS> a) {
   }
 }
 
-// grad[i] = sum over workgroups b (in order) of partial[b][i]
-__global__ void k_mlp_sum(const float* __restrict__ partial, int nb, float* __restrict__ grad) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= GRAD) return;
+// grad[i] = sum over workgroups b of partial[b][i], in a fixed order: thread
+// (i, q) of a 16 x 16 workgroup sums b = q, q + 16, ... and the 16 sums are
+// added in order of q (many short independent load chains instead of one
+// chain of nb loads per output)
+constexpr int SUM_I = 16, SUM_Q = 16;
+__global__ void __launch_bounds__(SUM_I * SUM_Q) k_mlp_sum(const float* __restrict__ partial, int nb,
+                                                          float* __restrict__ grad) {
+  __shared__ float red[SUM_Q][SUM_I];
+  const int il = threadIdx.x % SUM_I, q = threadIdx.x / SUM_I;
+  const int i = blockIdx.x * SUM_I + il;
   float s = 0.f;
-  for (int b = 0; b < nb; ++b) s += partial[(size_t)b * GRAD + i];
-  grad[i] = s;
+  if (i < GRAD)
+    for (int b = q; b < nb; b += SUM_Q) s += partial[(size_t)b * GRAD + i];
+  red[q][il] = s;
+  __syncthreads();
+  if (q == 0 && i < GRAD) {
+    float t = 0.f;
+    for (int k = 0; k < SUM_Q; ++k) t += red[k][il];
+    grad[i] = t;
+  }
 }
 
 template <typename TS>
@@ -319,8 +332,8 @@ int mlp_launch(void* stream, int nx, int ny, int px, int py, const void* const* 
   } else {
     if (blocks > npartial) blocks = npartial;
     hipLaunchKernelGGL((k_mlp<TS, true>), dim3((unsigned)blocks), dim3(64 * WAVES), 0, s, a);
-    hipLaunchKernelGGL(k_mlp_sum, dim3((GRAD + 255) / 256), dim3(256), 0, s, (const float*)partial,
-                       (int)blocks, grad);
+    hipLaunchKernelGGL(k_mlp_sum, dim3((GRAD + SUM_I - 1) / SUM_I), dim3(SUM_I * SUM_Q), 0, s,
+                       (const float*)partial, (int)blocks, grad);
   }
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return pyr::report_error(std::string("MLP launch: ") + hipGetErrorString(e));
