@@ -155,10 +155,10 @@ __device__ __forceinline__ void solve_chunk_lu(const LineArgs& a, const Tabs& t,
 // through Xb; the rows of the result go to sink(k, y_k). Two barriers. A
 // thread whose chunk index p is not below P (a workgroup shaped for a longer
 // line of another member of a grouped launch) only takes the barriers.
-template <typename C, typename Sink>
+template <bool GRP, typename C, typename Sink>
 __device__ __forceinline__ void circ_solve(const C* kf, const C* kb, C p1, C p2, int p, int P,
                                            int m, C* Xa, C* Xb, int ly, int col, C* x, Sink sink) {
-  const bool act = p < P;
+  const bool act = !GRP || p < P;
   C e0 = 0, e1 = 0;
   if (act) {
     circ_fwd_exit(x, m, p1, p2, e0, e1);
@@ -191,22 +191,22 @@ __device__ __forceinline__ const typename SolveT<MODE>::type* kmat(const LineArg
 }
 
 // kf/kb: the carry rows in the solve type (global memory or an LDS copy)
-template <int MODE, typename CS, typename Emit>
+template <int MODE, typename CS, bool GRP = false, typename Emit>
 __device__ __forceinline__ void solve_chunk_circ(const LineArgs& a, const CS* kf, const CS* kb, int p,
                                                  int m, CS* X0, CS* X1, int ly, int col, double* rhs,
                                                  Emit emit) {
   const int P = a.P;
   if constexpr (MODE != MODE_R32) {
-    circ_solve<double>(kf, kb, a.p1, a.p2, p, P, m, X0, X1, ly, col, rhs,
+    circ_solve<GRP, double>(kf, kb, a.p1, a.p2, p, P, m, X0, X1, ly, col, rhs,
                        [&](int k, double v) { emit(k, v * a.invc); });
   } else {
     float y[CHUNK];
 #pragma unroll
     for (int k = 0; k < CHUNK; ++k) y[k] = (float)rhs[k];
-    circ_solve<float>(kf, kb, a.p1f, a.p2f, p, P, m, X0, X1, ly, col, y,
+    circ_solve<GRP, float>(kf, kb, a.p1f, a.p2f, p, P, m, X0, X1, ly, col, y,
                       [&](int k, float v) { y[k] = v * a.invcf; });
     // neighbour values of the FP32 solution for the residual
-    const bool act = p < P;
+    const bool act = !GRP || p < P;
     if (act) {
       float e4[4];
       edges(y, m, e4);
@@ -224,7 +224,7 @@ __device__ __forceinline__ void solve_chunk_circ(const LineArgs& a, const CS* kf
     float d[CHUNK];
 #pragma unroll
     for (int k = 0; k < CHUNK; ++k) d[k] = (float)rhs[k];
-    circ_solve<float>(kf, kb, a.p1f, a.p2f, p, P, m, X1, X0, ly, col, d,
+    circ_solve<GRP, float>(kf, kb, a.p1f, a.p2f, p, P, m, X1, X0, ly, col, d,
                       [&](int k, float v) { emit(k, (double)y[k] + (double)(v * a.invcf)); });
   }
 }
@@ -296,7 +296,7 @@ struct YzWaves {
 // predicates vanish at compile time. The circulant solve implies periodic lines.
 // The body takes the workgroup's line-group index `blk` (blockIdx.x for one
 // field, the member-local index for a grouped launch over several fields).
-template <typename TS, int KIND, int SOLVE, int MODE, int LY, bool FULL, int PRE>
+template <typename TS, int KIND, int SOLVE, int MODE, int LY, bool FULL, int PRE, bool GRP = false>
 __device__ __forceinline__ void line_yz_body(const LineArgs& a, long long blk, const TS* __restrict__ in,
                                              TS* __restrict__ out) {
   constexpr int R = Halo<KIND>::R;
@@ -317,7 +317,7 @@ __device__ __forceinline__ void line_yz_body(const LineArgs& a, long long blk, c
   const long long base = (Lc % a.A) + a.B * (Lc / a.A);
   // act: this thread's chunk exists (a grouped launch shapes its workgroups
   // for the member with the most chunks)
-  const bool act = FULL || p < a.P;
+  const bool act = FULL || !GRP || p < a.P;
   const int s = FULL ? p * CHUNK : (act ? a.cstart[p] : 0);
   const int m = FULL ? CHUNK : (act ? a.cstart[p + 1] - s : 0);
   const __amdgpu_buffer_rsrc_t rin = buf_rsrc(in, a.nbytes), rout = buf_rsrc(out, a.nbytes);
@@ -376,8 +376,8 @@ __device__ __forceinline__ void line_yz_body(const LineArgs& a, long long blk, c
     st(epilogue<TS>(a.epi, a.epi_c, val, old), s + k);
   };
   if (SOLVE == SOLVE_CIRC) {
-    solve_chunk_circ<MODE, CS>(a, kmat<MODE>(a, 0), kmat<MODE>(a, 1), p, m, Y, Y + 4 * a.P * LY, LY, lane,
-                               rhs, emit);
+    solve_chunk_circ<MODE, CS, GRP>(a, kmat<MODE>(a, 0), kmat<MODE>(a, 1), p, m, Y, Y + 4 * a.P * LY, LY, lane,
+                                    rhs, emit);
   } else {
     if (SOLVE == SOLVE_LU) {
       __syncthreads();  // staged tables visible
@@ -539,7 +539,7 @@ __device__ __forceinline__ void vzip(float4& r, const float4& o, F f) {
 // LU solve: the tile region also holds the tables and edge exchange once the
 // windows are in registers. Circulant solve: the carry exchange sits after
 // the tile.
-template <typename TS, int KIND, int SOLVE, int MODE, bool FULL>
+template <typename TS, int KIND, int SOLVE, int MODE, bool FULL, bool GRP = false>
 __device__ __forceinline__ void line_x_body(const LineArgs& a, long long blk, const TS* __restrict__ in,
                                             TS* __restrict__ out) {
   constexpr int R = Halo<KIND>::R;
@@ -580,7 +580,7 @@ __device__ __forceinline__ void line_x_body(const LineArgs& a, long long blk, co
   const TS* row0 = tile + lr * pitch;
   // act: this thread's chunk exists (a grouped launch sizes its workgroups
   // for the member with the most threads per workgroup)
-  const bool act = p < a.P;
+  const bool act = !GRP || p < a.P;
   const int s = FULL ? p * CHUNK : (act ? a.cstart[p] : 0);
   const int m = FULL ? CHUNK : (act ? a.cstart[p + 1] - s : 0);
   TS win[CHUNK + 2 * R];
@@ -618,8 +618,8 @@ __device__ __forceinline__ void line_x_body(const LineArgs& a, long long blk, co
   if (SOLVE == SOLVE_CIRC) {
     // the first carry barrier also retires every read of the tile
     CS* X0 = reinterpret_cast<CS*>(smem + align16((size_t)LX * pitch * sizeof(TS)));
-    solve_chunk_circ<MODE, CS>(a, kmat<MODE>(a, 0), kmat<MODE>(a, 1), p, m, X0, X0 + 4 * a.P * LX, LX, l,
-                               rhs, emit);
+    solve_chunk_circ<MODE, CS, GRP>(a, kmat<MODE>(a, 0), kmat<MODE>(a, 1), p, m, X0, X0 + 4 * a.P * LX, LX, l,
+                                    rhs, emit);
   } else {
 #pragma unroll
     for (int k = 0; k < CHUNK; ++k)
@@ -694,7 +694,7 @@ k_line_x_grp(const GrpEntry* __restrict__ g, int M, const TS* __restrict__ in, T
   // body can alias them
   const LineArgs a = g[j].a;
   const long long off = g[j].off, b0 = g[j].blk0;
-  line_x_body<TS, KIND, SOLVE, MODE, false>(a, (long long)blockIdx.x - b0, in + off, out + off);
+  line_x_body<TS, KIND, SOLVE, MODE, false, true>(a, (long long)blockIdx.x - b0, in + off, out + off);
 }
 
 template <typename TS, int KIND, int SOLVE, int MODE, int LY, int PRE>
@@ -703,7 +703,7 @@ k_line_yz_grp(const GrpEntry* __restrict__ g, int M, const TS* __restrict__ in, 
   const int j = grp_member(g, M, blockIdx.x);
   const LineArgs a = g[j].a;
   const long long off = g[j].off, b0 = g[j].blk0;
-  line_yz_body<TS, KIND, SOLVE, MODE, LY, false, PRE>(a, (long long)blockIdx.x - b0, in + off, out + off);
+  line_yz_body<TS, KIND, SOLVE, MODE, LY, false, PRE, true>(a, (long long)blockIdx.x - b0, in + off, out + off);
 }
 
 // null operator of a direction with fewer than 4 points (zero or copy)
@@ -1391,7 +1391,9 @@ static int emulate(pyr_ctx* c, int op, int dir, const TS* in, TS* out) {
 // runs as ONE grouped launch over every member (workgroups shaped for the
 // member with the most chunks per line; threads without a chunk only take
 // the barriers), plus a k_null launch for a member whose direction is a null
-// operator. The per-member line arguments of a
+// operator. (Splitting the members by chunk count into separate launches
+// idles fewer threads but measured slower: the launches are latency-bound
+// and run one after another.) The per-member line arguments of a
 // launch are built once per (operator, direction, precision, epilogue) and
 // kept on the device.
 struct GrpLaunch {

@@ -209,3 +209,31 @@ def test_device_scalar_operands_compile(amd):
     fin = fuse._final("max")
     name, code = fin._source(("T64",))
     assert L.pyr_pw_check_source(code.encode(), name.encode()) == 0, L.pyr_pw_last_error().decode()
+
+
+def test_group_operands_and_segmented_reductions_compile(amd):
+    """Per-member values of a grouped simulation enter fused kernels through
+    the member-index field; the segmented dt reductions (one value per
+    member) compile for gfx950, with member segments and workgroup ranges
+    laid out as the kernels read them."""
+    import ast
+    import numpy as np
+    L = amd.lib()
+    src = "self.variables['T'].data * self.variables['dt'].data"
+    sub = fuse.FusedSubtree(ast.parse(src, mode="eval").body, "kappa")
+    name, code = sub._source(("T32", "G64"), "f32")
+    assert "pmid[i]" in code and "const int* __restrict__ pmid = (const int*)a.p[2]" in code
+    assert L.pyr_pw_check_source(code.encode(), name.encode()) == 0, L.pyr_pw_last_error().decode()
+    import importlib
+    T = importlib.import_module("pyranda_amd.pyrandaTimestep")
+    red = fuse.SegmentedReduce("max", T._COURANT.body, 4, 4, "courant")
+    name, code = red._source(("T32",) * 4)
+    assert L.pyr_pw_check_source(code.encode(), name.encode()) == 0, L.pyr_pw_last_error().decode()
+    fin = fuse._segfinal  # compiled at first use on a GPU; its source is built inline
+    assert callable(fin)
+    seg = fuse.SegmentTable([100, 5000, 70000], [0, 112, 5120], "cpu")
+    assert seg.seg.tolist() == [0, 100, 112, 5112, 5120, 75120]
+    blk = seg.blk.tolist()
+    assert blk[0] == 0 and seg.nblocks == blk[-1] and all(b > a for a, b in zip(blk, blk[1:]))
+    shape = fuse.GroupShape(75136, torch.zeros(75136, dtype=torch.int32), 3)
+    assert tuple(shape) == (75136, 1, 1) and shape.nmem == 3
