@@ -289,7 +289,7 @@ struct YzWaves {
   static constexpr int v =
       PYR_YZ_WAVES_PER_EU > 1 ? PYR_YZ_WAVES_PER_EU
       : (((MODE == MODE_S32 && SOLVE == SOLVE_CIRC && PRE == PRE_NONE) ||
-          (SOLVE == SOLVE_NONE && sizeof(TS) == 4)) ? 4 : 1);
+          (SOLVE == SOLVE_NONE && sizeof(TS) == 4 && PRE == PRE_NONE)) ? 4 : 1);
 };
 
 // FULL: every chunk has CHUNK rows (N a multiple of CHUNK), so the row
@@ -488,15 +488,14 @@ __global__ void __launch_bounds__(WG_MAX) k_line_yz_pipe(LineArgs a, const TS* _
     TS win[CHUNK + 2 * R];
 #pragma unroll
     for (int w = 0; w < CHUNK + 2 * R; ++w) win[w] = col0[w * LG];
-    if constexpr (PRE == PRE_V) {
-#pragma unroll
-      for (int k = 0; k < CHUNK; ++k) pre[k] = win[k + R];
-    }
     double rhs[CHUNK];
     rhs_interior<KIND>(a, win, rhs, m);
     auto emit = [&](int k, double y) {
       double val = y * a.scale;
-      if constexpr (PRE == PRE_V) val += (double)pre[k];
+      // filters add v back: re-read from the tile (intact until the copies
+      // of the group after next) rather than holding CHUNK values across the
+      // solve, which would spill
+      if constexpr (PRE == PRE_V) val += (double)col0[(k + R) * LG];
       TS old = (TS)0;
       if constexpr (PRE == PRE_OLD) old = pre[k];
       buf_store(epilogue<TS>(a.epi, a.epi_c, val, old), rout, lane_off + (s + k) * row_bytes, 0);
