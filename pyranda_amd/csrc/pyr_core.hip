@@ -353,7 +353,10 @@ __device__ __forceinline__ void line_yz_body(const LineArgs& a, long long blk, c
     }
     win[w] = v;
   }
-  if constexpr (PRE == PRE_V) {
+  // grouped launches (runtime row counts, longer register live ranges) re-read
+  // v when it is added back instead of holding the chunk across the solve
+  constexpr bool HOLDV = PRE == PRE_V && !GRP;
+  if constexpr (HOLDV) {
 #pragma unroll
     for (int k = 0; k < CHUNK; ++k) pre[k] = win[k + R];
   }
@@ -364,7 +367,8 @@ __device__ __forceinline__ void line_yz_body(const LineArgs& a, long long blk, c
   auto emit = [&](int k, double y) {
     if (!valid) return;
     double val = y * a.scale;
-    if constexpr (PRE == PRE_V) val += (double)pre[k];
+    if constexpr (HOLDV) val += (double)pre[k];
+    if constexpr (PRE == PRE_V && GRP) val += (double)ld(rin, s + k);
     if constexpr (PRE == PRE_LATE)
       if (a.addv) val += (double)ld(rin, s + k);
     TS old = (TS)0;
