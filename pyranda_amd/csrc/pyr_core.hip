@@ -615,8 +615,16 @@ __device__ __forceinline__ void line_x_body(const LineArgs& a, long long blk, co
     __syncthreads();  // the tile is consumed
   }
   TS* rowo = tile + l * pitch;
+  // filters add v back here, read from the tile row before the result takes
+  // its place (the windows are in registers by now; with the LU solve the
+  // tile holds the tables instead and v is added on the way out)
+  constexpr bool VTILE = SOLVE != SOLVE_LU;
   auto emit = [&](int k, double y) {
-    if (act && l < nval) rowo[s + k] = (TS)(y * a.scale);
+    if (act && l < nval) {
+      double val = y * a.scale;
+      if (VTILE && a.addv) val += (double)rowo[s + k];
+      rowo[s + k] = (TS)val;
+    }
   };
   if (SOLVE == SOLVE_CIRC) {
     // the first carry barrier also retires every read of the tile
@@ -639,8 +647,7 @@ __device__ __forceinline__ void line_x_body(const LineArgs& a, long long blk, co
     for (int e = tid; e < nvec; e += nthr) {
       const int ll = e / vpl, i = e - ll * vpl;
       V r = *reinterpret_cast<const V*>(tile + ll * pitch + i * VW);
-      // filters: the sum is formed in FP64 and rounded once to the storage type
-      if (a.addv) vzip(r, sv[e], [](TS y, TS v) { return (TS)((double)y + (double)v); });
+      if (!VTILE && a.addv) vzip(r, sv[e], [](TS y, TS v) { return (TS)((double)y + (double)v); });
       if (a.epi != EPI_STORE) {
         const V o = need_old ? dv[e] : V{};
         vzip(r, o, [&](TS y, TS old) { return epilogue<TS>(a.epi, a.epi_c, (double)y, old); });
@@ -653,7 +660,7 @@ __device__ __forceinline__ void line_x_body(const LineArgs& a, long long blk, co
     for (int e = tid; e < ne; e += nthr) {
       const int ll = e / N, i = e - ll * N;
       double val = (double)tile[ll * pitch + i];
-      if (a.addv) val = (double)(TS)(val + (double)src[e]);
+      if (!VTILE && a.addv) val = (double)(TS)(val + (double)src[e]);
       const TS old = need_old ? dst[e] : (TS)0;
       dst[e] = epilogue<TS>(a.epi, a.epi_c, val, old);
     }
