@@ -404,6 +404,8 @@ def main():
                     help="--ops: arithmetic of the solves on FP32 fields")
     ap.add_argument("--precision", default="fp64,fp32,fp32-fp64solve",
                     help="comma-separated pyrandaSim precisions to time (the headline is fp32)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch shapes (no per-operator autotuning before timing)")
     ap.add_argument("--ensemble", action="store_true",
                     help="KH ensemble (independent 2-D layers on uneven grids, one HIP graph per step)")
     ap.add_argument("--members", type=int, default=64, help="--ensemble: number of members")
@@ -444,6 +446,7 @@ def main():
     print("[bench] setting up TG %d^3" % n, file=sys.stderr, flush=True)
     results = {}
     states = {}
+    tuned = {}
     dt = None
     precs = [p for p in args.precision.split(",") if p and not (args.no_fp64 and p == "fp64")]
     if "fp32" not in precs:
@@ -453,6 +456,11 @@ def main():
         if dt is None:
             dt = float(ss.variables["dt"].data) * 0.5  # CFL 0.5 from the initial state
         ctx = ss.PyMPI.ctx
+        if not args.no_autotune:
+            rep = ss.autotune()
+            tuned[precision] = {"%s_%s" % (OPS[op], DIRS[d]): "%s=%d" % best
+                                for (op, d), (best, _) in sorted(rep.items())}
+            print("[bench] %s autotune %s" % (precision, tuned[precision]), file=sys.stderr, flush=True)
         print("[bench] %s warmup %d" % (precision, args.warmup), file=sys.stderr, flush=True)
         t = run_steps(ss, dt, args.warmup)
         torch.cuda.synchronize()
@@ -515,7 +523,7 @@ def main():
         "data": "synthetic (Taylor-Green initial condition)",
         "config": {"workload": "taylor_green_euler_lad_%d^3" % n, "grid": [n, n, n], "gamma": gamma,
                    "mach": 0.1, "cfl": 0.5, "dt": dt, "rk_stages": 5, "lad": True,
-                   "periodic_solver": args.solver},
+                   "periodic_solver": args.solver, "autotune": tuned.get("fp32")},
         "fp64": ({"value": results["fp64"]["pups"], "ms_per_step": results["fp64"]["ms_per_step"]}
                  if "fp64" in results else None),
         "speedup_fp32r_over_fp64": (head["pups"] / results["fp64"]["pups"]) if "fp64" in results else None,

@@ -786,6 +786,15 @@ struct pyr_ctx {
   int r32_solve = PYR_R32_SOLVE_REFINED;
   int x_lds_kb = 40;                        // LDS budget of one x-line workgroup
   int yz_pipe = 1;                           // pipelined y/z kernels where the shape allows
+  // per operator, direction and solve mode (MODE_*): tuned values of the two
+  // knobs above, -1 = the context default (pyr_set_tuning)
+  int tune_xlds[NOPS][3][3];
+  int tune_pipe[NOPS][3][3];
+  pyr_ctx() {
+    for (int o = 0; o < NOPS; ++o)
+      for (int d = 0; d < 3; ++d)
+        for (int m = 0; m < 3; ++m) tune_xlds[o][d][m] = tune_pipe[o][d][m] = -1;
+  }
   int ncu = 256;                             // compute units of the device
   size_t npts() const { return (size_t)n[0] * n[1] * n[2]; }
 };
@@ -1103,9 +1112,15 @@ static int launch_op(pyr_ctx* c, int op, int dir, const TS* in, TS* out, int epi
   // the balanced partition gives every chunk CHUNK rows exactly when CHUNK | N;
   // the LU solve keeps runtime row counts (with compile-time counts the
   // compiler hoists every table read of the chunk and runs out of registers)
-  if (PYR_FULL_CHUNKS && pl.N % CHUNK == 0 && sv != SOLVE_LU)
-    return launch_kind<TS, MODE, true>(c, dir, pl.kind, sv, a, in, out);
-  return launch_kind<TS, MODE, false>(c, dir, pl.kind, sv, a, in, out);
+  const int sx = c->x_lds_kb, sp = c->yz_pipe;
+  if (c->tune_xlds[op][dir][MODE] >= 0) c->x_lds_kb = c->tune_xlds[op][dir][MODE];
+  if (c->tune_pipe[op][dir][MODE] >= 0) c->yz_pipe = c->tune_pipe[op][dir][MODE];
+  const int r = (PYR_FULL_CHUNKS && pl.N % CHUNK == 0 && sv != SOLVE_LU)
+                    ? launch_kind<TS, MODE, true>(c, dir, pl.kind, sv, a, in, out)
+                    : launch_kind<TS, MODE, false>(c, dir, pl.kind, sv, a, in, out);
+  c->x_lds_kb = sx;
+  c->yz_pipe = sp;
+  return r;
 }
 
 static int check_ctx(pyr_ctx* c) {
@@ -1628,6 +1643,20 @@ int pyr_set_r32_solve(pyr_ctx* c, int mode) {
   if (!c) return fail("null context");
   if (mode != PYR_R32_SOLVE_REFINED && mode != PYR_R32_SOLVE_FP64) return fail("bad r32 solve mode");
   c->r32_solve = mode;
+  return 0;
+}
+
+int pyr_set_tuning(pyr_ctx* c, int op, int dir, int prec, int x_lds_kb, int yz_pipe) {
+  if (!c) return fail("null context");
+  if (op < 0 || op >= NOPS || dir < 0 || dir > 2) return fail("bad operator or direction");
+  if (x_lds_kb >= 0 && (x_lds_kb < 8 || x_lds_kb > 160)) return fail("x_lds_kb must be 8..160 (or -1)");
+  if (yz_pipe > 1) return fail("yz_pipe must be 0, 1 (or -1)");
+  int mode;
+  if (prec == PYR_F64) mode = MODE_F64;
+  else if (prec == PYR_R32) mode = c->r32_solve == PYR_R32_SOLVE_FP64 ? MODE_S32 : MODE_R32;
+  else return fail("bad precision");
+  c->tune_xlds[op][dir][mode] = x_lds_kb < 0 ? -1 : x_lds_kb;
+  c->tune_pipe[op][dir][mode] = yz_pipe < 0 ? -1 : yz_pipe;
   return 0;
 }
 

@@ -108,6 +108,44 @@ class ParcopContext:
         """'circulant' (default) or 'lu': solver of periodic implicit lines."""
         _lib.check(self._L.pyr_set_periodic_solver(self._h, _lib.PSOLVE[name]))
 
+    def set_tuning(self, op, direction, dtype, x_lds_kb=-1, yz_pipe=-1):
+        """Per-operator knobs (pyr_set_tuning): LDS budget of x-line
+        workgroups (KB) and the pipelined y/z kernel; -1 = default."""
+        prec = _lib.F64 if dtype == torch.float64 else _lib.R32
+        _lib.check(self._L.pyr_set_tuning(self._h, op, direction, prec, int(x_lds_kb), int(yz_pipe)))
+
+    def autotune(self, dtype, ops=None, reps=5, x_lds_kb=(8, 12, 16, 24, 40, 64), seed=7):
+        """Time each operator and direction at this precision with every
+        candidate (x: LDS budget, hence lines per workgroup; y/z: pipelined
+        kernel or not) on a random field and keep the fastest. Returns
+        {(op, dir): (choice, {choice: mean ms})}."""
+        ops = ops if ops is not None else (_lib.OP_D1, _lib.OP_D2, _lib.OP_D8, _lib.OP_SFILTER, _lib.OP_GFILTER)
+        g = torch.Generator(device=self.device).manual_seed(seed)
+        v = torch.randn(tuple(reversed(self.nn)), generator=g, device=self.device, dtype=dtype).permute(2, 1, 0)
+        out = self._out(v)
+        report = {}
+        for op in ops:
+            for d in range(3):
+                if self.nn[d] <= 3:
+                    continue
+                cands = [("x_lds_kb", kb) for kb in x_lds_kb] if d == 0 else [("yz_pipe", 1), ("yz_pipe", 0)]
+                times = {}
+                for knob, val in cands:
+                    self.set_tuning(op, d, dtype, **{knob: val})
+                    self.apply(op, d, v, out=out)
+                    ev0 = torch.cuda.Event(enable_timing=True)
+                    ev1 = torch.cuda.Event(enable_timing=True)
+                    ev0.record()
+                    for _ in range(reps):
+                        self.apply(op, d, v, out=out)
+                    ev1.record()
+                    torch.cuda.synchronize(self.device)
+                    times[(knob, val)] = ev0.elapsed_time(ev1) / reps
+                best = min(times, key=times.get)
+                self.set_tuning(op, d, dtype, **{best[0]: best[1]})
+                report[(op, d)] = (best, times)
+        return report
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             self._L.pyr_destroy(self._h)

@@ -100,6 +100,11 @@ class pyrandaSim:
         self.deltat = 0.0
         self._shadow = {}  # FP64 RK state of conserved variables (fp32 mode)
 
+    def autotune(self, **kw):
+        """Tune the line-operator launches of this grid and precision on the
+        device (ParcopContext.autotune); returns the timing report."""
+        return self.PyMPI.ctx.autotune(self.dtype, **kw)
+
     # -- bookkeeping -------------------------------------------------------------
     def iprint(self, sprnt, wpm=0):
         if self.PyMPI.master and not self.silent:

@@ -179,3 +179,28 @@ def test_fourier_field_256(dev):
         _say("fbar r32 dir", d, err32)
         assert err32 <= 1e-6
     ctx.close()
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_autotune_keeps_results(dev, dtype):
+    """The autotuner only changes launch shapes (lines per x workgroup, the
+    pipelined y/z kernel): every operator's result is bit for bit the same
+    before and after tuning, and the chosen knobs are among the candidates."""
+    from pyranda_amd import _lib
+    from pyranda_amd.parcop import ParcopContext
+    nn = (64, 96, 64)
+    L = 2 * math.pi
+    ctx = ParcopContext(nn, (0.0, 0.0, 0.0), tuple(L * (n - 1) / n for n in nn), ["PERI"] * 6, device=dev)
+    g = torch.Generator(device=dev).manual_seed(3)
+    v = torch.randn(tuple(reversed(nn)), generator=g, device=dev, dtype=dtype).permute(2, 1, 0)
+    ops = (_lib.OP_D1, _lib.OP_D2, _lib.OP_D8, _lib.OP_SFILTER, _lib.OP_GFILTER)
+    before = {(op, d): ctx.apply(op, d, v).clone() for op in ops for d in range(3)}
+    rep = ctx.autotune(dtype, reps=2)
+    assert set(rep) == set(before)
+    for (op, d), (best, times) in rep.items():
+        assert best in times
+        assert best[0] == ("x_lds_kb" if d == 0 else "yz_pipe")
+        after = ctx.apply(op, d, v)
+        assert torch.equal(after, before[(op, d)]), (op, d, best)
+    _say("autotune", dtype, {k: v_[0] for k, v_ in rep.items()})
+    ctx.close()
