@@ -458,7 +458,7 @@ def main():
         ctx = ss.PyMPI.ctx
         if not args.no_autotune:
             rep = ss.autotune()
-            tuned[precision] = {"%s_%s" % (OPS[op], DIRS[d]): "%s=%d" % best
+            tuned[precision] = {"%s_%s" % (OPS[op], DIRS[d]): ",".join("%s=%d" % kv for kv in best)
                                 for (op, d), (best, _) in sorted(rep.items())}
             print("[bench] %s autotune %s" % (precision, tuned[precision]), file=sys.stderr, flush=True)
         print("[bench] %s warmup %d" % (precision, args.warmup), file=sys.stderr, flush=True)

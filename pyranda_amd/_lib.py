@@ -29,7 +29,7 @@ SIGNATURES = {
     "pyr_destroy": (ctypes.c_int, [_vp]),
     "pyr_set_stream": (ctypes.c_int, [_vp, _vp]),
     "pyr_set_periodic_solver": (ctypes.c_int, [_vp, ctypes.c_int]),
-    "pyr_set_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
+    "pyr_set_tuning": (ctypes.c_int, [_vp] + [ctypes.c_int] * 6),
     "pyr_last_error": (ctypes.c_char_p, []),
     "pyr_axis_coords": (ctypes.c_int, [_vp, ctypes.c_int, _dp]),
     "pyr_mesh_scalars": (ctypes.c_int, [_vp, _dp, _dp, _dp]),

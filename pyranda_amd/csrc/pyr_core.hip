@@ -788,12 +788,14 @@ struct pyr_ctx {
   int yz_pipe = 1;                           // pipelined y/z kernels where the shape allows
   // per operator, direction and solve mode (MODE_*): tuned values of the two
   // knobs above, -1 = the context default (pyr_set_tuning)
+  int yz_lines = 0;                          // lines per plain y/z workgroup (0: 64, or 32 past 8 chunks)
   int tune_xlds[NOPS][3][3];
   int tune_pipe[NOPS][3][3];
+  int tune_ly[NOPS][3][3];
   pyr_ctx() {
     for (int o = 0; o < NOPS; ++o)
       for (int d = 0; d < 3; ++d)
-        for (int m = 0; m < 3; ++m) tune_xlds[o][d][m] = tune_pipe[o][d][m] = -1;
+        for (int m = 0; m < 3; ++m) tune_xlds[o][d][m] = tune_pipe[o][d][m] = tune_ly[o][d][m] = -1;
   }
   int ncu = 256;                             // compute units of the device
   size_t npts() const { return (size_t)n[0] * n[1] * n[2]; }
@@ -1044,7 +1046,7 @@ static int launch_typed(pyr_ctx* c, int dir, LineArgs a, const TS* in, TS* out) 
     if (set_lds(k, shm)) return 1;
     hipLaunchKernelGGL(k, grid, block, shm, c->stream, a, in, out);
   } else {
-    const int ly = P <= 8 ? 64 : 32;
+    const int ly = (P <= 8 && c->yz_lines != 32) ? 64 : 32;
     const size_t ex = align16((size_t)4 * P * ly * sizeof(CS));
     const size_t shm = SOLVE == SOLVE_CIRC ? 2 * ex : (SOLVE == SOLVE_LU ? ex + stage : 0);
     const dim3 grid((a.nlines + ly - 1) / ly), block(ly, P);
@@ -1112,14 +1114,16 @@ static int launch_op(pyr_ctx* c, int op, int dir, const TS* in, TS* out, int epi
   // the balanced partition gives every chunk CHUNK rows exactly when CHUNK | N;
   // the LU solve keeps runtime row counts (with compile-time counts the
   // compiler hoists every table read of the chunk and runs out of registers)
-  const int sx = c->x_lds_kb, sp = c->yz_pipe;
+  const int sx = c->x_lds_kb, sp = c->yz_pipe, sl = c->yz_lines;
   if (c->tune_xlds[op][dir][MODE] >= 0) c->x_lds_kb = c->tune_xlds[op][dir][MODE];
   if (c->tune_pipe[op][dir][MODE] >= 0) c->yz_pipe = c->tune_pipe[op][dir][MODE];
+  if (c->tune_ly[op][dir][MODE] >= 0) c->yz_lines = c->tune_ly[op][dir][MODE];
   const int r = (PYR_FULL_CHUNKS && pl.N % CHUNK == 0 && sv != SOLVE_LU)
                     ? launch_kind<TS, MODE, true>(c, dir, pl.kind, sv, a, in, out)
                     : launch_kind<TS, MODE, false>(c, dir, pl.kind, sv, a, in, out);
   c->x_lds_kb = sx;
   c->yz_pipe = sp;
+  c->yz_lines = sl;
   return r;
 }
 
@@ -1646,17 +1650,19 @@ int pyr_set_r32_solve(pyr_ctx* c, int mode) {
   return 0;
 }
 
-int pyr_set_tuning(pyr_ctx* c, int op, int dir, int prec, int x_lds_kb, int yz_pipe) {
+int pyr_set_tuning(pyr_ctx* c, int op, int dir, int prec, int x_lds_kb, int yz_pipe, int yz_lines) {
   if (!c) return fail("null context");
   if (op < 0 || op >= NOPS || dir < 0 || dir > 2) return fail("bad operator or direction");
   if (x_lds_kb >= 0 && (x_lds_kb < 8 || x_lds_kb > 160)) return fail("x_lds_kb must be 8..160 (or -1)");
   if (yz_pipe > 1) return fail("yz_pipe must be 0, 1 (or -1)");
+  if (yz_lines >= 0 && yz_lines != 32 && yz_lines != 64) return fail("yz_lines must be 32 or 64 (or -1)");
   int mode;
   if (prec == PYR_F64) mode = MODE_F64;
   else if (prec == PYR_R32) mode = c->r32_solve == PYR_R32_SOLVE_FP64 ? MODE_S32 : MODE_R32;
   else return fail("bad precision");
   c->tune_xlds[op][dir][mode] = x_lds_kb < 0 ? -1 : x_lds_kb;
   c->tune_pipe[op][dir][mode] = yz_pipe < 0 ? -1 : yz_pipe;
+  c->tune_ly[op][dir][mode] = yz_lines < 0 ? -1 : yz_lines;
   return 0;
 }
 

@@ -108,11 +108,13 @@ class ParcopContext:
         """'circulant' (default) or 'lu': solver of periodic implicit lines."""
         _lib.check(self._L.pyr_set_periodic_solver(self._h, _lib.PSOLVE[name]))
 
-    def set_tuning(self, op, direction, dtype, x_lds_kb=-1, yz_pipe=-1):
+    def set_tuning(self, op, direction, dtype, x_lds_kb=-1, yz_pipe=-1, yz_lines=-1):
         """Per-operator knobs (pyr_set_tuning): LDS budget of x-line
-        workgroups (KB) and the pipelined y/z kernel; -1 = default."""
+        workgroups (KB), the pipelined y/z kernel, lines per plain y/z
+        workgroup; -1 = default."""
         prec = _lib.F64 if dtype == torch.float64 else _lib.R32
-        _lib.check(self._L.pyr_set_tuning(self._h, op, direction, prec, int(x_lds_kb), int(yz_pipe)))
+        _lib.check(self._L.pyr_set_tuning(self._h, op, direction, prec, int(x_lds_kb), int(yz_pipe),
+                                          int(yz_lines)))
 
     def autotune(self, dtype, ops=None, reps=5, x_lds_kb=(8, 12, 16, 24, 40, 64), seed=7):
         """Time each operator and direction at this precision with every
@@ -128,10 +130,14 @@ class ParcopContext:
             for d in range(3):
                 if self.nn[d] <= 3:
                     continue
-                cands = [("x_lds_kb", kb) for kb in x_lds_kb] if d == 0 else [("yz_pipe", 1), ("yz_pipe", 0)]
+                if d == 0:
+                    cands = [(("x_lds_kb", kb),) for kb in x_lds_kb]
+                else:
+                    cands = [(("yz_pipe", 1),), (("yz_pipe", 0), ("yz_lines", 64)),
+                             (("yz_pipe", 0), ("yz_lines", 32))]
                 times = {}
-                for knob, val in cands:
-                    self.set_tuning(op, d, dtype, **{knob: val})
+                for cand in cands:
+                    self.set_tuning(op, d, dtype, **dict(cand))
                     self.apply(op, d, v, out=out)
                     ev0 = torch.cuda.Event(enable_timing=True)
                     ev1 = torch.cuda.Event(enable_timing=True)
@@ -140,9 +146,9 @@ class ParcopContext:
                         self.apply(op, d, v, out=out)
                     ev1.record()
                     torch.cuda.synchronize(self.device)
-                    times[(knob, val)] = ev0.elapsed_time(ev1) / reps
+                    times[cand] = ev0.elapsed_time(ev1) / reps
                 best = min(times, key=times.get)
-                self.set_tuning(op, d, dtype, **{best[0]: best[1]})
+                self.set_tuning(op, d, dtype, **dict(best))
                 report[(op, d)] = (best, times)
         return report
 

@@ -199,7 +199,7 @@ def test_autotune_keeps_results(dev, dtype):
     assert set(rep) == set(before)
     for (op, d), (best, times) in rep.items():
         assert best in times
-        assert best[0] == ("x_lds_kb" if d == 0 else "yz_pipe")
+        assert best[0][0] == ("x_lds_kb" if d == 0 else "yz_pipe")
         after = ctx.apply(op, d, v)
         assert torch.equal(after, before[(op, d)]), (op, d, best)
     _say("autotune", dtype, {k: v_[0] for k, v_ in rep.items()})
