@@ -13,8 +13,9 @@ hardware overlap the members' kernels.
 
 State across replays: the graph reads the tensors that hold each member's
 state when it was captured and writes new tensors from its private pool, so
-the captured step ends by copying every variable's new value (and the
-driver's time/dt) back into those state tensors.
+the captured step ends by copying every variable's new value (the driver's
+time/dt and the FP64 states that FP32 members keep for their conservatively
+filtered variables included) back into those state tensors.
 
 No counterpart in the reference (pyranda runs one simulation per MPI
 communicator, pyranda/pyranda.py:32-100).
@@ -55,13 +56,20 @@ class pyrandaEnsemble:
 
     @staticmethod
     def _write_back(sim, st, static, rebind):
-        names, svars, sst = static
+        names, svars, sst, s64 = static
         for n in names:
             d = sim.variables[n].data
             if d is not svars[n]:
                 svars[n].copy_(d)
                 if rebind:
                     sim.variables[n].data = svars[n]
+        for U, t in s64.items():
+            d = sim._state64.get(U)
+            if d is not None and d is not t:
+                t.copy_(d)
+            if rebind:
+                sim._state64[U] = t
+                sim._mark_state(U)
         for k, v in list(st.items()):
             if v is not sst[k]:
                 sst[k].copy_(v)
@@ -74,6 +82,9 @@ class pyrandaEnsemble:
         own (aliases are split) that the graph will read and write back."""
         seen = set()
         names = []
+        # FP64 states of conserved variables (FP32 runs through the
+        # conservative filter) that still match their fields
+        fresh = [U for U in sim._state64 if sim._state_current(U)]
         for name, var in sim.variables.items():
             d = var.data
             if name.startswith("mesh") or not torch.is_tensor(d):
@@ -86,7 +97,14 @@ class pyrandaEnsemble:
             if not torch.is_tensor(v) or id(v) in seen:
                 st[k] = torch.as_tensor(v, dtype=torch.float64, device=sim.PyMPI.device).clone()
             seen.add(id(st[k]))
-        return names, {n: sim.variables[n].data for n in names}, dict(st)
+        s64 = {}
+        for U in fresh:
+            t = sim._state64[U]
+            s64[U] = t.clone() if id(t) in seen else t
+            seen.add(id(s64[U]))
+            sim._state64[U] = s64[U]
+            sim._mark_state(U)
+        return names, {n: sim.variables[n].data for n in names}, dict(st), s64
 
     def capture(self, concurrent=True):
         """Record one time step of every member as one HIP graph. The members
@@ -100,7 +118,6 @@ class pyrandaEnsemble:
         with torch.cuda.graph(g):
             cap = torch.cuda.current_stream(dev)
             for i, (sim, st) in enumerate(zip(self.members, self.state)):
-                names, svars, sst = statics[i]
                 s = streams[i] if concurrent else cap
                 s.wait_stream(cap)
                 with torch.cuda.stream(s):
@@ -109,9 +126,12 @@ class pyrandaEnsemble:
                 if concurrent:
                     cap.wait_stream(s)
         # point every member back at its state tensors
-        for (sim, st), (names, svars, sst) in zip(zip(self.members, self.state), statics):
+        for (sim, st), (names, svars, sst, s64) in zip(zip(self.members, self.state), statics):
             for n in names:
                 sim.variables[n].data = svars[n]
+            for U, t in s64.items():
+                sim._state64[U] = t
+                sim._mark_state(U)
             st.clear()
             st.update(sst)
         self.graph = g

@@ -387,6 +387,8 @@ class pyrandaGroupSim(pyrandaSim):
         self.cycle = 0
         self.deltat = 0.0
         self._shadow = {}
+        self._state64 = {}
+        self._state_src = {}
 
     def checkForNan(self, names=None):
         """NaN check over the members' points (not the alignment padding)."""

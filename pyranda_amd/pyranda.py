@@ -99,6 +99,8 @@ class pyrandaSim:
         self.cycle = 0
         self.deltat = 0.0
         self._shadow = {}  # FP64 RK state of conserved variables (fp32 mode)
+        self._state64 = {}  # persistent FP64 state of self-filtered conserved variables
+        self._state_src = {}  # the FP32 field last derived from it
 
     def autotune(self, **kw):
         """Tune the line-operator launches of this grid and precision on the
@@ -241,7 +243,17 @@ class pyrandaSim:
                 self.iprint(ic)
                 self.iprint("Error message: %s" % e)
                 raise
+        if self.dtype != torch.float64:
+            # the FP64 state of the conserved variables starts from the FP64
+            # initial conditions
+            self._state64 = {}
+            for U in self._self_filtered():
+                d = self.variables[U].data
+                if torch.is_tensor(d) and tuple(d.shape) == tuple(self.mesh.shape):
+                    self._state64[U] = d.to(torch.float64)
         self._cast_fields()
+        for U in self._state64:
+            self._mark_state(U)
         snans = self.checkForNan(sorted(set(names)))
         if snans:
             raise FloatingPointError("Found some nans in inits: %s" % snans)
@@ -287,6 +299,17 @@ class pyrandaSim:
     def _eval_alg(self, eq, verbose):
         if verbose:
             self.iprint(eq.eqstr)
+        U = eq.LHS[0] if eq.LHS and len(eq.LHS) == 1 else None
+        if U is not None and U in self._state64 and U in self._self_filtered():
+            # the conservative filter of an FP32 run acts on the FP64 state
+            # (FP64 kernels) and the field is its FP32 rounding
+            if not self._state_current(U):
+                self._state64[U] = self.variables[U].data.to(torch.float64)
+            out = self.filter(self._state64[U])
+            self._state64[U] = out
+            self.variables[U].data = out.to(self.dtype)
+            self._mark_state(U)
+            return
         rhs = eq.RHS(self)
         if not eq.LHS:
             return
@@ -315,15 +338,23 @@ class pyrandaSim:
         fp32 = self.dtype != torch.float64
         mpi = self.PyMPI
         PHI = {}
-        # conserved variables that an ALG line overwrites after every stage (the
-        # conservative filter): their FP64 shadow would be re-derived from the
-        # FP32 field each stage, which is what the update does without one
+        # conserved variables whose only rewrite is the conservative filter keep
+        # a persistent FP64 state (the filter acts on it, see _eval_alg); other
+        # conserved variables that an ALG line overwrites after every stage
+        # would have their FP64 shadow re-derived from the FP32 field each
+        # stage, which is what the update does without one
+        sf = self._self_filtered() if fp32 else {}
         rewritten = self._rewritten_conserved() if fp32 else set()
         for U in self.conserved:
             # the first stage has A = 0 and only writes PHI
             PHI[U] = torch.empty((self.nz, self.ny, self.nx), dtype=torch.float64,
                                  device=mpi.device).permute(2, 1, 0)
-            if fp32:
+            if fp32 and U in sf:
+                if not self._state_current(U):
+                    self._state64[U] = self.variables[U].data.to(torch.float64)
+                    self._mark_state(U)
+                self._shadow[U] = self._state64[U]
+            elif fp32:
                 self._shadow[U] = None if U in rewritten else self.variables[U].data.to(torch.float64)
         time_i = time
         self.deltat = dt
@@ -331,6 +362,10 @@ class pyrandaSim:
             FLUX = self.updateFlux()
             for U in self.conserved:
                 self.variables[U].data = self._rk_update(U, FLUX[U], PHI, dt, RK_A[ii], RK_B[ii], fp32)
+                if U in sf:
+                    # the update advanced the FP64 state in place and wrote its rounding
+                    self._state64[U] = self._shadow[U]
+                    self._mark_state(U)
             time = time_i + RK_ETA[ii] * dt
             self.time = time
             if fp32:
@@ -338,10 +373,51 @@ class pyrandaSim:
             self.updateVars()
             if fp32:
                 for U in self.conserved:
-                    if U not in rewritten and self.variables[U].data is not before[U]:
+                    if U in sf:
+                        self._shadow[U] = self._state64[U]
+                    elif U not in rewritten and self.variables[U].data is not before[U]:
                         self._shadow[U] = self.variables[U].data.to(torch.float64)
         self.cycle += 1
         return time
+
+    def _mark_state(self, U):
+        d = self.variables[U].data
+        self._state_src[U] = (d, d._version)
+
+    def _state_current(self, U):
+        """The FP64 state of U still matches its field: the field is the
+        tensor derived from it and nothing wrote into it since."""
+        src = self._state_src.get(U)
+        d = self.variables[U].data
+        return U in self._state64 and src is not None and src[0] is d and d._version == src[1]
+
+    def _self_filtered(self):
+        """Conserved variables whose only ALG writer is the conservative
+        filter `:U: = fbar(:U:)`, in an EOM without LHS-less ALG lines (BC
+        calls and the like may change fields in place): their FP32 runs keep
+        an FP64 state through the filter."""
+        key = tuple((id(eq), eq.active) for eq in self.equations)
+        if getattr(self, "_sf_key", None) == key:
+            return self._sf
+        writers = {}
+        bare = False
+        for eq in self.equations:
+            if not eq.active or eq.kind != "ALG":
+                continue
+            if not eq.LHS:
+                bare = True
+                continue
+            for U in eq.LHS:
+                writers.setdefault(U, []).append(eq)
+        out = {}
+        if not bare:
+            for U in self.conserved:
+                eqs = writers.get(U, [])
+                if len(eqs) == 1 and eqs[0].rank == 1 and eqs[0].sRHS.replace(" ", "").replace("'", '"') == \
+                        'self.filter(self.variables["%s"].data)' % U:
+                    out[U] = eqs[0]
+        self._sf, self._sf_key = out, key
+        return out
 
     def _rewritten_conserved(self):
         out = set()
