@@ -47,6 +47,8 @@ MEASURED_TRAFFIC = {
     ("fbar", "y", "torch.float32"): (65952 + 65536) * 1024,
     ("gbar", "y", "torch.float64"): (140248 + 131072) * 1024,
     ("gbar", "y", "torch.float32"): (65844 + 65536) * 1024,
+    # FP64 fbar z sweep that also stores the FP32 rounding (20 B/pt)
+    ("fbar", "z", "torch.float64+float32"): (131502 + 196608) * 1024,
 }
 OPS = ["d1", "d2", "d8", "fbar", "gbar"]
 DIRS = "xyz"
@@ -61,9 +63,13 @@ def tg_mesh(n):
 
 
 def make_tg(n, precision, p0, solver="circulant"):
+    """precision: a pyrandaSim precision, or "fp32-rounded-state" (fp32 with
+    fp64_state=False: the conservative filter rounds the state to FP32)."""
     import pyranda_amd as pr
     from pyranda_amd.workloads import TG_EOM, TG_IC
-    ss = pr.pyrandaSim("TGvortex", tg_mesh(n), silent=True, precision=precision)
+    rounded = precision == "fp32-rounded-state"
+    ss = pr.pyrandaSim("TGvortex", tg_mesh(n), silent=True, precision="fp32" if rounded else precision,
+                       fp64_state=not rounded)
     ss.PyMPI.ctx.set_periodic_solver(solver)
     ss.addPackage(pr.pyrandaTimestep(ss))
     ss.EOM(TG_EOM)
@@ -117,7 +123,7 @@ def ops_bench(n, steps, warmup, dev, only=None, solver="circulant", r32_solve="r
     timed with HIP events, error against the exact per-mode result."""
     import numpy as np
     import torch
-    from pyranda_amd.parcop import ParcopContext
+    from pyranda_amd.parcop import ParcopContext, empty_field
     from pyranda_amd.workloads import fourier_modes, fourier_field
     L = 2 * math.pi * (n - 1) / n
     ctx = ParcopContext((n, n, n), (0.0, 0.0, 0.0), (L, L, L), ["PERI"] * 6, device=dev)
@@ -165,18 +171,24 @@ def ops_bench(n, steps, warmup, dev, only=None, solver="circulant", r32_solve="r
                 exact = fourier_field(n, dev, (ks, amps, phis), weights=sym / h ** 2)
             else:
                 exact = fourier_field(n, dev, (ks, amps, phis), weights=sym)
-            for name, f, bpp in (("f64", f64, 16), ("r32", f32, 8)):
+            variants = [("f64", f64, 16), ("r32", f32, 8)]
+            if op == 3 and d > 0:
+                # FP64 fbar y/z sweep that also stores the FP32 rounding (the
+                # last sweep of the filter of an FP32 run's FP64 state)
+                variants.append(("f64r", f64, 20))
+            for name, f, bpp in variants:
                 if not wanted("%s_%s_%s" % (OPS[op], DIRS[d], name)):
                     continue
-                o = ctx.apply(op, d, f)
+                kw = {"out32": empty_field((n, n, n), torch.float32, dev)} if name == "f64r" else {}
+                o = ctx.apply(op, d, f, **kw)
                 for _ in range(warmup):
-                    ctx.apply(op, d, f, out=o)
+                    ctx.apply(op, d, f, out=o, **kw)
                 torch.cuda.synchronize()
                 e0 = torch.cuda.Event(enable_timing=True)
                 e1 = torch.cuda.Event(enable_timing=True)
                 e0.record()
                 for _ in range(reps):
-                    ctx.apply(op, d, f, out=o)
+                    ctx.apply(op, d, f, out=o, **kw)
                 e1.record()
                 torch.cuda.synchronize()
                 us = e0.elapsed_time(e1) * 1e3 / reps
@@ -402,8 +414,9 @@ def main():
                     help="solver of periodic implicit lines")
     ap.add_argument("--r32-solve", default="refined", choices=["refined", "fp64"],
                     help="--ops: arithmetic of the solves on FP32 fields")
-    ap.add_argument("--precision", default="fp64,fp32,fp32-fp64solve",
-                    help="comma-separated pyrandaSim precisions to time (the headline is fp32)")
+    ap.add_argument("--precision", default="fp64,fp32,fp32-fp64solve,fp32-rounded-state",
+                    help="comma-separated pyrandaSim precisions to time (the headline is fp32; "
+                         "fp32-rounded-state is fp32 without the FP64 state through the filter)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (no per-operator autotuning before timing)")
     ap.add_argument("--ensemble", action="store_true",
@@ -458,8 +471,9 @@ def main():
         ctx = ss.PyMPI.ctx
         if not args.no_autotune:
             rep = ss.autotune()
-            tuned[precision] = {"%s_%s" % (OPS[op], DIRS[d]): ",".join("%s=%d" % kv for kv in best)
-                                for (op, d), (best, _) in sorted(rep.items())}
+            tuned[precision] = {"_".join([OPS[key[0]], DIRS[key[1]]] + list(key[2:])):
+                                ",".join("%s=%d" % kv for kv in best)
+                                for key, (best, _) in sorted(rep.items(), key=lambda kv: str(kv[0]))}
             print("[bench] %s autotune %s" % (precision, tuned[precision]), file=sys.stderr, flush=True)
         print("[bench] %s warmup %d" % (precision, args.warmup), file=sys.stderr, flush=True)
         t = run_steps(ss, dt, args.warmup)
@@ -486,7 +500,9 @@ def main():
     tot = {k: c * ms for k, (c, ms) in kern.items()}
     dom = max(tot, key=tot.get)
     count, mean_ms = kern[dom]
-    bytes_per_pt = 8 if "float32" in dom[2] else 16
+    # one read and one write at the storage width; an FP64 sweep that also
+    # stores the FP32 rounding writes 4 B/pt more
+    bytes_per_pt = 20 if dom[2].endswith("+float32") else (8 if "float32" in dom[2] else 16)
     alg_bytes = n ** 3 * bytes_per_pt
     achieved = alg_bytes / (mean_ms * 1e-3) / 1e9
     op_share = sum(tot.values()) / (head["elapsed_s"] * 1e3)
@@ -497,6 +513,8 @@ def main():
             acc[k] = rel_err(states["fp32"][k], states["fp64"][k])
             if "fp32-fp64solve" in states:
                 acc_s[k] = rel_err(states["fp32-fp64solve"][k], states["fp64"][k])
+    acc_r = {k: rel_err(states["fp32-rounded-state"][k], states["fp64"][k])
+             for k in states["fp32"]} if "fp64" in states and "fp32-rounded-state" in states else {}
 
     cpu = None
     if not args.no_cpu:
@@ -535,11 +553,18 @@ def main():
             if "fp64" in results else None,
             "accuracy_vs_fp64": {k: {"rel_max": a, "rel_l2": b} for k, (a, b) in acc_s.items()},
         } if "fp32-fp64solve" in results else None,
+        # fp32 whose conservative filter rounds the RK state to FP32 each stage
+        "fp32_rounded_state": {
+            "value": results["fp32-rounded-state"]["pups"],
+            "ms_per_step": results["fp32-rounded-state"]["ms_per_step"],
+            "accuracy_vs_fp64": {k: {"rel_max": a, "rel_l2": b} for k, (a, b) in acc_r.items()},
+        } if "fp32-rounded-state" in results else None,
         "accuracy_fp32r_vs_fp64": {k: {"rel_max": a, "rel_l2": b} for k, (a, b) in acc.items()},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (MEASURED_TRAFFIC.get((OPS[dom[0]], DIRS[dom[1]], dom[2])) if n == 256 else None),
-                     "kernel": "%s_%s (%s)" % (OPS[dom[0]], DIRS[dom[1]], dom[2].replace("torch.", "")),
+                     "kernel": "%s_%s (%s)" % (OPS[dom[0]], DIRS[dom[1]],
+                                              dom[2].replace("torch.", "").replace("+float32", ", FP32 copy")),
                      "launches": count, "mean_us": mean_ms * 1e3, "bytes_per_launch": alg_bytes,
                      "operator_time_share": op_share},
         "cpu_baseline": cpu,

@@ -124,12 +124,25 @@ int pyr_divergence_r32(pyr_ctx* ctx, const float* fx, const float* fy, const flo
 int pyr_apply(pyr_ctx* ctx, int op, int dir, int prec, const void* in, void* out, int epi,
               double epi_scale);
 
+/* --- FP64 operators that also store the FP32 rounding of their result ----
+ * For FP32 runs that keep an FP64 state through the conservative filter
+ * (the state is filtered in FP64, the field is its rounding). No counterpart
+ * in the reference (FP64 only); the operators are those of
+ *   pyr_sfilter_f64_rounded: parcop.f90:337 sFilter (fbar), out32[i] = (float)dval[i]
+ *   pyr_apply_rounded:       compact_operators.f90:13-497, one y (dir 1) or
+ *                            z (dir 2) sweep of FP64 fields with PYR_EPI_STORE
+ * out32 holds as many floats as the field has points. */
+int pyr_sfilter_f64_rounded(pyr_ctx* ctx, const double* val, double* dval, float* out32);
+int pyr_apply_rounded(pyr_ctx* ctx, int op, int dir, const double* in, double* out, float* out32);
+
 /* --- one RK4 stage update of a conserved variable (pyranda.py:800-804) ----
  * PHI <- dt*F + A*PHI (in place, FP64; with A == 0 PHI is only written), then
  *   PYR_F64: U_out <- U_in + B*PHI                (double fields)
  *   PYR_R32: U64 <- U64 + B*PHI (FP64 state), U_out <- (float)U64; with
  *            U64 == NULL: U_out <- (float)((double)U_in + B*PHI)
- * F, U_in, U_out have the field precision; n = number of points. */
+ * F, U_in, U_out have the field precision; n = number of points. With U64
+ * given, U_out may be NULL: only the FP64 state advances (for a caller whose
+ * next read of U is the FP64 state, e.g. the filter of the state). */
 int pyr_rk_update(pyr_ctx* ctx, int prec, long long n, const void* F, double* PHI,
                   const void* U_in, void* U_out, double* U64, double dt, double A, double B);
 /* The same update with dt read from device memory (one FP64 value) when the

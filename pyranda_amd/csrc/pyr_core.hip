@@ -321,6 +321,8 @@ __device__ __forceinline__ void line_yz_body(const LineArgs& a, long long blk, c
   const int s = FULL ? p * CHUNK : (act ? a.cstart[p] : 0);
   const int m = FULL ? CHUNK : (act ? a.cstart[p + 1] - s : 0);
   const __amdgpu_buffer_rsrc_t rin = buf_rsrc(in, a.nbytes), rout = buf_rsrc(out, a.nbytes);
+  // the FP32 copy of an FP64 result (LineArgs::out32), half the bytes per element
+  [[maybe_unused]] const __amdgpu_buffer_rsrc_t rout32 = buf_rsrc(a.out32, a.nbytes / 2);
   const int lane_off = (int)(base * (long long)sizeof(TS));
   const int row_bytes = (int)(a.stride * (long long)sizeof(TS));
   // row `row` of this line; the row offset is wave-uniform when LY == 64
@@ -377,7 +379,14 @@ __device__ __forceinline__ void line_yz_body(const LineArgs& a, long long blk, c
     // a filter with an accumulating epilogue (only through pyr_apply) reads
     // the output in place
     if constexpr (PRE == PRE_V) old = (a.epi == EPI_ADD || a.epi == EPI_MAXABS) ? ld(rout, s + k) : (TS)0;
-    st(epilogue<TS>(a.epi, a.epi_c, val, old), s + k);
+    const TS r = epilogue<TS>(a.epi, a.epi_c, val, old);
+    st(r, s + k);
+    // (store epilogues only: the accumulating variants keep their registers)
+    if constexpr (sizeof(TS) == 8 && PRE != PRE_OLD && !GRP)
+      if (a.out32) {
+        if constexpr (LY == 64) buf_store((float)r, rout32, lane_off / 2, (s + k) * (row_bytes / 2));
+        else buf_store((float)r, rout32, lane_off / 2 + (s + k) * (row_bytes / 2), 0);
+      }
   };
   if (SOLVE == SOLVE_CIRC) {
     solve_chunk_circ<MODE, CS, GRP>(a, kmat<MODE>(a, 0), kmat<MODE>(a, 1), p, m, Y, Y + 4 * a.P * LY, LY, lane,
@@ -719,12 +728,14 @@ k_line_yz_grp(const GrpEntry* __restrict__ g, int M, const TS* __restrict__ in, 
 // null operator of a direction with fewer than 4 points (zero or copy)
 template <typename TS>
 __global__ void k_null(long long n, int copy, int epi, double c, const TS* __restrict__ in,
-                       TS* __restrict__ out) {
+                       TS* __restrict__ out, float* __restrict__ out32) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const double val = copy ? (double)in[i] : 0.0;
   const TS old = (epi == EPI_ADD || epi == EPI_MAXABS) ? out[i] : (TS)0;
-  out[i] = epilogue<TS>(epi, c, val, old);
+  const TS r = epilogue<TS>(epi, c, val, old);
+  out[i] = r;
+  if (out32) out32[i] = (float)r;
 }
 
 // RK4 stage update (pyranda.py:800-804), one pass over F, PHI and U. With
@@ -748,7 +759,7 @@ __global__ void k_rk_update(long long n, const TS* __restrict__ F, double* __res
     if (U64) {
       const double u = U64[i] + tmp2;
       U64[i] = u;
-      Uout[i] = (TS)u;
+      if (Uout) Uout[i] = (TS)u;
     } else {
       Uout[i] = (TS)((double)Uin[i] + tmp2);
     }
@@ -798,6 +809,7 @@ struct pyr_ctx {
         for (int m = 0; m < 3; ++m) tune_xlds[o][d][m] = tune_pipe[o][d][m] = tune_ly[o][d][m] = -1;
   }
   int ncu = 256;                             // compute units of the device
+  float* out32 = nullptr;                    // FP32 copy of the next FP64 y/z sweep's output
   size_t npts() const { return (size_t)n[0] * n[1] * n[2]; }
 };
 
@@ -1105,11 +1117,16 @@ static int launch_op(pyr_ctx* c, int op, int dir, const TS* in, TS* out, int epi
   if (pl.null_op) {
     const long long n = (long long)c->npts();
     hipLaunchKernelGGL((k_null<TS>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n,
-                       pl.null_option == 1 ? 1 : 0, epi, ec, in, out);
+                       pl.null_option == 1 ? 1 : 0, epi, ec, in, out, sizeof(TS) == 8 ? c->out32 : nullptr);
     HIPCHECK(hipGetLastError());
     return 0;
   }
-  const LineArgs a = make_args(c, pl, c->dplan[op][dir], dir, epi, ec, MODE == MODE_F64 ? PYR_F64 : PYR_R32);
+  LineArgs a = make_args(c, pl, c->dplan[op][dir], dir, epi, ec, MODE == MODE_F64 ? PYR_F64 : PYR_R32);
+  if (c->out32) {
+    // only the y/z kernels of FP64 fields write the rounded copy
+    if (sizeof(TS) != 8 || dir == 0) return fail("an FP32 copy of the output needs an FP64 y or z sweep");
+    a.out32 = c->out32;
+  }
   const int sv = plan_solve(c, pl);
   // the balanced partition gives every chunk CHUNK rows exactly when CHUNK | N;
   // the LU solve keeps runtime row counts (with compile-time counts the
@@ -1601,7 +1618,7 @@ static int grp_apply(pyr_group* g, const GrpKey& key, const TS* in, TS* out) {
     const long long n = (long long)g->mem[z.member]->npts();
     const long long o = g->off[z.member];
     hipLaunchKernelGGL((k_null<TS>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, g->stream, n, z.copy,
-                       key.epi, z.ec, in + o, out + o);
+                       key.epi, z.ec, in + o, out + o, nullptr);
     HIPCHECK(hipGetLastError());
   }
   return 0;
@@ -1735,6 +1752,28 @@ int pyr_pringv_f64(pyr_ctx* c, const double* vx, const double* vy, const double*
 }
 
 int pyr_sfilter_f64(pyr_ctx* c, const double* v, double* o) { return filter3(c, OP_SF, PYR_F64, v, o); }
+int pyr_apply_rounded(pyr_ctx* c, int op, int dir, const double* in, double* out, float* o32) {
+  if (check_ctx(c)) return 1;
+  if (!o32) return fail("null field pointer");
+  if (dir != 1 && dir != 2) return fail("the FP32 copy is written by y and z sweeps only");
+  c->out32 = o32;
+  const int r = apply_any(c, op, dir, PYR_F64, in, out, EPI_STORE, 0.0);
+  c->out32 = nullptr;
+  return r;
+}
+int pyr_sfilter_f64_rounded(pyr_ctx* c, const double* v, double* o, float* o32) {
+  if (check_ctx(c)) return 1;
+  if (!o32) return fail("null field pointer");
+  void* tmp;
+  if (scratch(c, PYR_F64, &tmp)) return 1;
+  if (apply_any(c, OP_SF, 0, PYR_F64, v, o, EPI_STORE, 0.0)) return 1;
+  if (apply_any(c, OP_SF, 1, PYR_F64, o, tmp, EPI_STORE, 0.0)) return 1;
+  // the z sweep also stores the FP32 rounding of the result
+  c->out32 = o32;
+  const int r = apply_any(c, OP_SF, 2, PYR_F64, tmp, o, EPI_STORE, 0.0);
+  c->out32 = nullptr;
+  return r;
+}
 int pyr_sfilter_r32(pyr_ctx* c, const float* v, float* o) { return filter3(c, OP_SF, PYR_R32, v, o); }
 int pyr_gfilter_f64(pyr_ctx* c, const double* v, double* o) { return filter3(c, OP_GF, PYR_F64, v, o); }
 int pyr_gfilter_r32(pyr_ctx* c, const float* v, float* o) { return filter3(c, OP_GF, PYR_R32, v, o); }
@@ -1766,7 +1805,8 @@ int pyr_divergence_r32(pyr_ctx* c, const float* fx, const float* fy, const float
 static int rk_update(hipStream_t stream, int prec, long long n, const void* F, double* PHI, const void* Uin,
                      void* Uout, double* U64, const double* dtp, const int* mid, double dt, double A,
                      double B) {
-  if (!F || !PHI || !Uout || (!Uin && !U64)) return fail("null pointer to pyr_rk_update");
+  // with an FP64 state the rounded field may be left unwritten (Uout null)
+  if (!F || !PHI || (!U64 && (!Uin || !Uout))) return fail("null pointer to pyr_rk_update");
   if (n <= 0) return 0;
   long long blocks = (n + 1023) / 1024;
   if (blocks > 8192) blocks = 8192;

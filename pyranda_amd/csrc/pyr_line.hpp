@@ -64,6 +64,9 @@ struct LineArgs {
   const double* kb;
   const float* kf32;
   const float* kb32;
+  // FP64 y/z kernels: when set, every stored value is also written here
+  // rounded to FP32 (same element index as the output)
+  float* out32;
 };
 
 template <int KIND> struct Halo { static constexpr int R = (KIND == K_R4) ? 4 : 3; };

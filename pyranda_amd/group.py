@@ -294,7 +294,7 @@ class pyrandaGroupSim(pyrandaSim):
     (dt, time) are (M,) FP64 device tensors; `member_field(name, m)` gives
     member m's (nx, ny, nz) view of a field."""
 
-    def __init__(self, name, meshes, silent=True, precision="fp64", device="cuda"):
+    def __init__(self, name, meshes, silent=True, precision="fp64", device="cuda", fp64_state=True):
         if precision not in ("fp64", "fp32", "fp32-fp64solve"):
             raise ValueError("precision must be 'fp64', 'fp32' or 'fp32-fp64solve'")
         if not torch.cuda.is_available():
@@ -387,6 +387,7 @@ class pyrandaGroupSim(pyrandaSim):
         self.cycle = 0
         self.deltat = 0.0
         self._shadow = {}
+        self.fp64_state = fp64_state
         self._state64 = {}
         self._state_src = {}
 
@@ -451,13 +452,13 @@ class pyrandaGroupSim(pyrandaSim):
         return self.PyMPI.der.div(f1, f2, f3)
 
     # -- time integration ---------------------------------------------------------
-    def _rk_update(self, U, F, PHI, dt, A, B, fp32):
+    def _rk_update(self, U, F, PHI, dt, A, B, fp32, store=True):
         u = self.variables[U].data
         if not (torch.is_tensor(dt) and dt.dim() == 1 and dt.shape[0] == len(self.members)
                 and torch.is_tensor(F) and tuple(F.shape) == tuple(self.mesh.shape) and F.dtype == self.dtype):
             dtv = dt[self.mesh.shape.mid.long()].reshape(self.mesh.shape) if (
                 torch.is_tensor(dt) and dt.dim() == 1) else dt
-            return pyrandaSim._rk_update(self, U, F, PHI, dtv, A, B, fp32)
+            return pyrandaSim._rk_update(self, U, F, PHI, dtv, A, B, fp32, store)
         out = torch.empty(self.mesh.shape, dtype=self.dtype, device=u.device)
         shadow = self._shadow[U].data_ptr() if fp32 and self._shadow[U] is not None else None
         dt64 = dt.to(torch.float64).contiguous()

@@ -193,22 +193,33 @@ class ParcopContext:
     def _suffix(t):
         return "f64" if t.dtype == torch.float64 else "r32"
 
-    def apply(self, op, direction, v, out=None, epi=_lib.EPI_STORE, epi_scale=0.0):
-        """One directional operator (compact_operators.f90:13-497)."""
+    def apply(self, op, direction, v, out=None, epi=_lib.EPI_STORE, epi_scale=0.0, out32=None):
+        """One directional operator (compact_operators.f90:13-497). out32
+        (an FP32 field; FP64 y/z sweeps with the store epilogue) also
+        receives the rounding of the result."""
         v = self._prep(v)
         if out is None:
             out = self._out(v)
         prec = _lib.F64 if v.dtype == torch.float64 else _lib.R32
+        if out32 is not None and (v.dtype != torch.float64 or epi != _lib.EPI_STORE
+                                  or out32.dtype != torch.float32 or not _contig_equiv(out32)
+                                  or tuple(out32.shape) != self.nn):
+            raise ValueError("out32: an FP32 field for the store epilogue of an FP64 sweep")
         self._stream()
         if self.timing is not None:
             ev0 = torch.cuda.Event(enable_timing=True)
             ev1 = torch.cuda.Event(enable_timing=True)
             ev0.record()
-        _lib.check(self._L.pyr_apply(self._h, op, direction, prec, self._ptr(v), self._ptr(out), epi,
-                                     float(epi_scale)))
+        if out32 is not None:
+            _lib.check(self._L.pyr_apply_rounded(self._h, op, direction, self._ptr(v), self._ptr(out),
+                                                 self._ptr(out32)))
+        else:
+            _lib.check(self._L.pyr_apply(self._h, op, direction, prec, self._ptr(v), self._ptr(out), epi,
+                                         float(epi_scale)))
         if self.timing is not None:
             ev1.record()
-            self.timing.setdefault((op, direction, str(v.dtype)), []).append((ev0, ev1))
+            key = str(v.dtype) + ("+float32" if out32 is not None else "")
+            self.timing.setdefault((op, direction, key), []).append((ev0, ev1))
         return out
 
     def timing_summary(self):
@@ -239,6 +250,23 @@ class ParcopContext:
     def plaplacian(self, v): return self._unary("plaplacian", v)
     def pring(self, v): return self._unary("pring", v)
     def sfilter(self, v): return self._unary("sfilter", v)
+
+    def sfilter_rounded(self, v):
+        """fbar of an FP64 field and the FP32 rounding of the result, both
+        stored by the last sweep (FP32 runs that filter an FP64 state)."""
+        v = self._prep(v)
+        if v.dtype != torch.float64:
+            raise ValueError("sfilter_rounded filters FP64 fields")
+        out32 = empty_field(self.nn, torch.float32, self.device)
+        if self.timing is not None:
+            a = self.apply(_lib.OP_SFILTER, 0, v)
+            b = self.apply(_lib.OP_SFILTER, 1, a)
+            out = self.apply(_lib.OP_SFILTER, 2, b, out32=out32)
+            return out, out32
+        out = self._out(v)
+        self._stream()
+        _lib.check(self._L.pyr_sfilter_f64_rounded(self._h, self._ptr(v), self._ptr(out), self._ptr(out32)))
+        return out, out32
     def gfilter(self, v): return self._unary("gfilter", v)
 
     def gfilterdir(self, v, direction):

@@ -49,8 +49,11 @@ RK_ETA = [494393426753. / 4806282396855., 4702696611523. / 9636871101405.,
 class pyrandaSim:
 
     def __init__(self, name, meshOptions, silent=False, precision="fp64", device="cuda",
-                 device_scalars=False):
+                 device_scalars=False, fp64_state=True):
         self.name = name
+        # fp32 modes: conserved variables rewritten only by the conservative
+        # filter keep an FP64 state through it (False: the FP32 field is the state)
+        self.fp64_state = fp64_state
         self.silent = silent
         self.mesh = pyrandaMesh()
         if isinstance(meshOptions, str):
@@ -104,8 +107,18 @@ class pyrandaSim:
 
     def autotune(self, **kw):
         """Tune the line-operator launches of this grid and precision on the
-        device (ParcopContext.autotune); returns the timing report."""
-        return self.PyMPI.ctx.autotune(self.dtype, **kw)
+        device (ParcopContext.autotune); returns the timing report
+        {(op, dir): (choice, times)}. FP32 runs that filter an FP64 state
+        also tune the FP64 filter, reported under (op, dir, "f64")."""
+        from . import _lib
+        ctx = self.PyMPI.ctx
+        rep = ctx.autotune(self.dtype, **kw)
+        if self.dtype != torch.float64 and self._self_filtered():
+            kw64 = dict(kw)
+            kw64["ops"] = (_lib.OP_SFILTER,)
+            for (op, d), r in ctx.autotune(torch.float64, **kw64).items():
+                rep[(op, d, "f64")] = r
+        return rep
 
     # -- bookkeeping -------------------------------------------------------------
     def iprint(self, sprnt, wpm=0):
@@ -305,9 +318,15 @@ class pyrandaSim:
             # (FP64 kernels) and the field is its FP32 rounding
             if not self._state_current(U):
                 self._state64[U] = self.variables[U].data.to(torch.float64)
-            out = self.filter(self._state64[U])
+            rounded = getattr(self.PyMPI.fil, "filter_rounded", None)
+            if rounded is not None and self.dtype == torch.float32:
+                # the last sweep stores the rounding as well
+                out, out32 = rounded(self._state64[U])
+            else:
+                out = self.filter(self._state64[U])
+                out32 = out.to(self.dtype)
             self._state64[U] = out
-            self.variables[U].data = out.to(self.dtype)
+            self.variables[U].data = out32
             self._mark_state(U)
             return
         rhs = eq.RHS(self)
@@ -344,6 +363,9 @@ class pyrandaSim:
         # would have their FP64 shadow re-derived from the FP32 field each
         # stage, which is what the update does without one
         sf = self._self_filtered() if fp32 else {}
+        # of those, the ones whose filter is the next reader of the field: the
+        # update need not store their rounding
+        skip = self._filtered_first(sf) if sf else set()
         rewritten = self._rewritten_conserved() if fp32 else set()
         for U in self.conserved:
             # the first stage has A = 0 and only writes PHI
@@ -361,7 +383,8 @@ class pyrandaSim:
         for ii in range(5):
             FLUX = self.updateFlux()
             for U in self.conserved:
-                self.variables[U].data = self._rk_update(U, FLUX[U], PHI, dt, RK_A[ii], RK_B[ii], fp32)
+                self.variables[U].data = self._rk_update(U, FLUX[U], PHI, dt, RK_A[ii], RK_B[ii], fp32,
+                                                         store=U not in skip)
                 if U in sf:
                     # the update advanced the FP64 state in place and wrote its rounding
                     self._state64[U] = self._shadow[U]
@@ -396,6 +419,8 @@ class pyrandaSim:
         filter `:U: = fbar(:U:)`, in an EOM without LHS-less ALG lines (BC
         calls and the like may change fields in place): their FP32 runs keep
         an FP64 state through the filter."""
+        if not getattr(self, "fp64_state", True):
+            return {}
         key = tuple((id(eq), eq.active) for eq in self.equations)
         if getattr(self, "_sf_key", None) == key:
             return self._sf
@@ -419,6 +444,20 @@ class pyrandaSim:
         self._sf, self._sf_key = out, key
         return out
 
+    def _filtered_first(self, sf):
+        """Self-filtered variables whose filter line is among the ALG lines
+        that run first (before any other line): after the RK update the
+        filter of the FP64 state is the next reader of the field."""
+        line = {id(eq): U for U, eq in sf.items()}
+        out = set()
+        for item in self._alg_plan():
+            for eq in ([item[1]] if item[0] == "eq" else list(item[2])):
+                U = line.get(id(eq))
+                if U is None:
+                    return out
+                out.add(U)
+        return out
+
     def _rewritten_conserved(self):
         out = set()
         for eq in self.equations:
@@ -426,7 +465,10 @@ class pyrandaSim:
                 out.update(U for U in eq.LHS if U in self.conserved)
         return out
 
-    def _rk_update(self, U, F, PHI, dt, A, B, fp32):
+    def _rk_update(self, U, F, PHI, dt, A, B, fp32, store=True):
+        """One stage update of U; returns the new field. store=False (fp32
+        with an FP64 state): only the state advances and the field returned
+        is the current one, which the caller replaces before reading it."""
         from . import _lib
         from .parcop import _contig_equiv
         u = self.variables[U].data
@@ -442,20 +484,24 @@ class pyrandaSim:
                 self._shadow[U] = base + B * PHI[U]
                 return self._shadow[U].to(self.dtype)
             return u + B * PHI[U]
-        out = torch.empty((self.nz, self.ny, self.nx), dtype=self.dtype, device=u.device).permute(2, 1, 0)
+        shadow = self._shadow[U].data_ptr() if fp32 and self._shadow[U] is not None else None
+        if shadow is not None and not store:
+            out, optr = u, None
+        else:
+            out = torch.empty((self.nz, self.ny, self.nx), dtype=self.dtype, device=u.device).permute(2, 1, 0)
+            optr = out.data_ptr()
         ctx = self.PyMPI.ctx
         ctx._stream()
-        shadow = self._shadow[U].data_ptr() if fp32 and self._shadow[U] is not None else None
         prec = _lib.R32 if fp32 else _lib.F64
         if torch.is_tensor(dt):
             # dt in device memory (device_scalars): read by the kernel
             dt64 = dt.to(device=u.device, dtype=torch.float64).reshape(())
             _lib.check(ctx._L.pyr_rk_update_dt(ctx._h, prec, self.npts, F.data_ptr(), PHI[U].data_ptr(),
-                                               u.data_ptr(), out.data_ptr(), shadow, dt64.data_ptr(),
+                                               u.data_ptr(), optr, shadow, dt64.data_ptr(),
                                                float(A), float(B)))
         else:
             _lib.check(ctx._L.pyr_rk_update(ctx._h, prec, self.npts, F.data_ptr(), PHI[U].data_ptr(),
-                                            u.data_ptr(), out.data_ptr(), shadow, float(dt), float(A),
+                                            u.data_ptr(), optr, shadow, float(dt), float(A),
                                             float(B)))
         return out
 

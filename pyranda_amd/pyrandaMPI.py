@@ -60,6 +60,7 @@ class _SFil:
         self.ctx = ctx
 
     def filter(self, v): return self.ctx.sfilter(v)
+    def filter_rounded(self, v): return self.ctx.sfilter_rounded(v)
 
 
 class pyrandaMPI:

@@ -76,3 +76,53 @@ def test_device_numpy_namespace_on_host_values():
     assert numpy.minimum(1.0, 2.0) == 1.0
     assert numpy.sqrt(4.0) == 2.0
     assert numpy.pi == pytest.approx(3.141592653589793)
+
+
+def _sim_with_eom(eom, fp64_state=True):
+    """A pyrandaSim shell holding only what the FP64-state rule reads: its
+    equations (parsed like EOM() parses them) and conserved variables."""
+    from pyranda_amd.pyranda import pyrandaSim
+    from pyranda_amd.pyrandaEq import pyrandaEq
+    sm = dict(base_map())
+    sm.update({"fbar(": "self.filter(", "bc.extrap(": "self.packages['BC'].extrap("})
+    s = object.__new__(pyrandaSim)
+    s.fp64_state = fp64_state
+    s.equations = [pyrandaEq(l, sm, None) for l in splitLines(eom)]
+    s.conserved = [eq.LHS[0] for eq in s.equations if eq.kind == "PDE"]
+    return s
+
+
+FILTERED_EOM = """
+ddt(:rho:)  = -ddx(:rho:*:u:)
+ddt(:Et:)   = -ddx(:Et:*:u:)
+:rho:       =  fbar( :rho: )
+:Et:        =  fbar( :Et:  )
+:u:         =  :Et: / :rho:
+"""
+
+
+def test_fp64_state_rule():
+    """FP32 runs keep an FP64 state for conserved variables whose only ALG
+    writer is their own conservative filter, in EOMs without LHS-less lines."""
+    assert sorted(_sim_with_eom(FILTERED_EOM)._self_filtered()) == ["Et", "rho"]
+    assert _sim_with_eom(FILTERED_EOM, fp64_state=False)._self_filtered() == {}
+    # a second writer of rho, or a filter of another field, takes rho out
+    two = FILTERED_EOM + ":rho: = :rho: * 1.0\n"
+    assert sorted(_sim_with_eom(two)._self_filtered()) == ["Et"]
+    other = FILTERED_EOM.replace("fbar( :rho: )", "fbar( :Et: )")
+    assert sorted(_sim_with_eom(other)._self_filtered()) == ["Et"]
+    # an LHS-less line (a boundary condition call) may write any field in place
+    bc = FILTERED_EOM + "bc.extrap(['rho'],['x1'])\n"
+    assert _sim_with_eom(bc)._self_filtered() == {}
+    # inactive lines do not count
+    s = _sim_with_eom(bc)
+    s.equations[-1].active = False
+    assert sorted(s._self_filtered()) == ["Et", "rho"]
+    # the RK update skips storing the rounding only where the filter lines
+    # run before any other ALG line (they are then the field's next reader)
+    s = _sim_with_eom(FILTERED_EOM)
+    assert sorted(s._filtered_first(s._self_filtered())) == ["Et", "rho"]
+    late = FILTERED_EOM.replace(":Et:        =  fbar( :Et:  )\n", "") + ":Et: = fbar( :Et: )\n"
+    s = _sim_with_eom(late)
+    assert sorted(s._self_filtered()) == ["Et", "rho"]
+    assert sorted(s._filtered_first(s._self_filtered())) == ["rho"]

@@ -262,3 +262,38 @@ def test_kh_2d_curve(pr):
     assert steps == 249
     # the reference harness accepts 1e-4 (tests/testObj.py:52-58)
     assert err <= 1e-9
+
+
+def test_fp32_fp64_state_through_filter(pr):
+    """FP32 KH run: the conservatively filtered variables keep an FP64 state
+    (the field is its rounding after every step) and end closer to the FP64
+    run than with the state rounded to FP32 at every filter; a field written
+    in place is taken as the new state."""
+    import torch
+    from pyranda_amd.workloads import kh_setup
+    runs = {"fp64": kh_setup(pr, 64, 48, precision="fp64"),
+            "state": kh_setup(pr, 64, 48, precision="fp32"),
+            "rounded": kh_setup(pr, 64, 48, precision="fp32")}
+    runs["rounded"].fp64_state = False
+    st = runs["state"]
+    assert sorted(st._self_filtered()) == ["Et", "rho", "rhou", "rhov"]
+    dt = float(runs["fp64"].var("dt").data) * 0.5
+    for name, s in runs.items():
+        t = 0.0
+        for _ in range(16):
+            t = s.rk4(t, dt)
+    torch.cuda.synchronize()
+    for k in ("rho", "rhou", "rhov", "Et"):
+        assert st._state64[k].dtype == torch.float64
+        assert torch.equal(st.var(k).data, st._state64[k].float()), k
+    ref = runs["fp64"].var("rhou").data
+    err = {n: float((runs[n].var("rhou").data.double() - ref).norm() / ref.norm()) for n in ("state", "rounded")}
+    _say("rhou rel l2 vs fp64", err)
+    assert err["state"] < err["rounded"]
+    # an in-place write to the field replaces the FP64 state
+    assert st._state_current("rho")
+    st.var("rho").data.mul_(1.0)
+    assert not st._state_current("rho")
+    st.rk4(t, dt)
+    assert st._state_current("rho")
+    assert torch.equal(st.var("rho").data, st._state64["rho"].float())

@@ -204,3 +204,35 @@ def test_autotune_keeps_results(dev, dtype):
         assert torch.equal(after, before[(op, d)]), (op, d, best)
     _say("autotune", dtype, {k: v_[0] for k, v_ in rep.items()})
     ctx.close()
+
+
+@pytest.mark.parametrize("nn,kinds", [((64, 96, 64), ("PERI", "PERI", "PERI")),
+                                      ((33, 40, 20), ("NONE", "NONE", "NONE")),
+                                      ((64, 48, 1), ("PERI", "PERI", "PERI"))])
+def test_filter_with_rounded_copy(dev, nn, kinds):
+    """fbar of an FP64 field with the FP32 rounding stored by the last sweep
+    (pyr_sfilter_f64_rounded, and sweep by sweep through pyr_apply_rounded,
+    the path the timed runs take): the FP64 result is bit for bit
+    pyr_sfilter_f64's, the copy is its rounding; bounded lines and a null z
+    direction (the copy kernel) included."""
+    from pyranda_amd import _lib
+    from pyranda_amd.parcop import ParcopContext
+    _say("fbar with rounded copy", nn, kinds)
+    x1 = (0.0, 0.0, 0.0)
+    xn = tuple(2 * math.pi * (n - 1) / n if k == "PERI" and n > 1 else 1.0 for n, k in zip(nn, kinds))
+    ctx = ParcopContext(nn, x1, xn, [k for k in kinds for _ in range(2)], device=dev)
+    g = torch.Generator(device=dev).manual_seed(11)
+    v = torch.randn(tuple(reversed(nn)), generator=g, device=dev, dtype=torch.float64).permute(2, 1, 0)
+    ref = ctx.sfilter(v)
+    out, out32 = ctx.sfilter_rounded(v)
+    assert out.dtype == torch.float64 and out32.dtype == torch.float32
+    assert torch.equal(out, ref) and torch.equal(out32, ref.float())
+    ctx.timing = {}
+    out_t, out32_t = ctx.sfilter_rounded(v)
+    ctx.timing = None
+    assert torch.equal(out_t, ref) and torch.equal(out32_t, ref.float())
+    # only FP64 y/z sweeps with the store epilogue write the copy
+    with pytest.raises(_lib.PyrandaAMDError):
+        _lib.check(ctx._L.pyr_apply_rounded(ctx._h, _lib.OP_SFILTER, 0, ctx._ptr(v), ctx._ptr(out),
+                                            ctx._ptr(out32)))
+    ctx.close()
