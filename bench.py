@@ -47,6 +47,7 @@ MEASURED_TRAFFIC = {
     ("fbar", "y", "torch.float32"): (65952 + 65536) * 1024,
     ("gbar", "y", "torch.float64"): (140248 + 131072) * 1024,
     ("gbar", "y", "torch.float32"): (65844 + 65536) * 1024,
+    ("fbar", "x", "torch.float64"): (131490 + 131072) * 1024,
     # FP64 fbar z sweep that also stores the FP32 rounding (20 B/pt)
     ("fbar", "z", "torch.float64+float32"): (131502 + 196608) * 1024,
 }

@@ -116,20 +116,24 @@ class ParcopContext:
         _lib.check(self._L.pyr_set_tuning(self._h, op, direction, prec, int(x_lds_kb), int(yz_pipe),
                                           int(yz_lines)))
 
-    def autotune(self, dtype, ops=None, reps=5, x_lds_kb=(8, 12, 16, 24, 40, 64), seed=7):
+    def autotune(self, dtype, ops=None, reps=5, x_lds_kb=(8, 12, 16, 24, 40, 64), seed=7, rounded=False):
         """Time each operator and direction at this precision with every
         candidate (x: LDS budget, hence lines per workgroup; y/z: pipelined
-        kernel or not) on a random field and keep the fastest. Returns
+        kernel or not) on a random field and keep the fastest. rounded: FP64
+        z sweeps are timed storing the FP32 copy of their result as well (the
+        last sweep of the filter of an FP32 run's FP64 state). Returns
         {(op, dir): (choice, {choice: mean ms})}."""
         ops = ops if ops is not None else (_lib.OP_D1, _lib.OP_D2, _lib.OP_D8, _lib.OP_SFILTER, _lib.OP_GFILTER)
         g = torch.Generator(device=self.device).manual_seed(seed)
         v = torch.randn(tuple(reversed(self.nn)), generator=g, device=self.device, dtype=dtype).permute(2, 1, 0)
         out = self._out(v)
+        out32 = empty_field(self.nn, torch.float32, self.device) if rounded and dtype == torch.float64 else None
         report = {}
         for op in ops:
             for d in range(3):
                 if self.nn[d] <= 3:
                     continue
+                kw = {"out32": out32} if out32 is not None and d == 2 else {}
                 if d == 0:
                     cands = [(("x_lds_kb", kb),) for kb in x_lds_kb]
                 else:
@@ -138,12 +142,12 @@ class ParcopContext:
                 times = {}
                 for cand in cands:
                     self.set_tuning(op, d, dtype, **dict(cand))
-                    self.apply(op, d, v, out=out)
+                    self.apply(op, d, v, out=out, **kw)
                     ev0 = torch.cuda.Event(enable_timing=True)
                     ev1 = torch.cuda.Event(enable_timing=True)
                     ev0.record()
                     for _ in range(reps):
-                        self.apply(op, d, v, out=out)
+                        self.apply(op, d, v, out=out, **kw)
                     ev1.record()
                     torch.cuda.synchronize(self.device)
                     times[cand] = ev0.elapsed_time(ev1) / reps

@@ -116,6 +116,7 @@ class pyrandaSim:
         if self.dtype != torch.float64 and self._self_filtered():
             kw64 = dict(kw)
             kw64["ops"] = (_lib.OP_SFILTER,)
+            kw64["rounded"] = self.dtype == torch.float32
             for (op, d), r in ctx.autotune(torch.float64, **kw64).items():
                 rep[(op, d, "f64")] = r
         return rep
