@@ -119,13 +119,20 @@ class ParcopContext:
     def autotune(self, dtype, ops=None, reps=5, x_lds_kb=(8, 12, 16, 24, 40, 64), seed=7, rounded=False):
         """Time each operator and direction at this precision with every
         candidate (x: LDS budget, hence lines per workgroup; y/z: pipelined
-        kernel or not) on a random field and keep the fastest. rounded: FP64
+        kernel or not) on random fields and keep the fastest. The timed
+        launches cycle through enough fields to exceed the device's 256 MB
+        last-level cache, so every launch reads its input from HBM as the
+        operators of a time step do. rounded: FP64
         z sweeps are timed storing the FP32 copy of their result as well (the
         last sweep of the filter of an FP32 run's FP64 state). Returns
         {(op, dir): (choice, {choice: mean ms})}."""
         ops = ops if ops is not None else (_lib.OP_D1, _lib.OP_D2, _lib.OP_D8, _lib.OP_SFILTER, _lib.OP_GFILTER)
         g = torch.Generator(device=self.device).manual_seed(seed)
-        v = torch.randn(tuple(reversed(self.nn)), generator=g, device=self.device, dtype=dtype).permute(2, 1, 0)
+        field_bytes = self.nn[0] * self.nn[1] * self.nn[2] * (8 if dtype == torch.float64 else 4)
+        nfield = min(8, max(1, -(-(384 << 20) // field_bytes)))
+        vs = [torch.randn(tuple(reversed(self.nn)), generator=g, device=self.device, dtype=dtype).permute(2, 1, 0)
+              for _ in range(nfield)]
+        v = vs[0]
         out = self._out(v)
         out32 = empty_field(self.nn, torch.float32, self.device) if rounded and dtype == torch.float64 else None
         report = {}
@@ -146,8 +153,8 @@ class ParcopContext:
                     ev0 = torch.cuda.Event(enable_timing=True)
                     ev1 = torch.cuda.Event(enable_timing=True)
                     ev0.record()
-                    for _ in range(reps):
-                        self.apply(op, d, v, out=out, **kw)
+                    for i in range(reps):
+                        self.apply(op, d, vs[(i + 1) % nfield], out=out, **kw)
                     ev1.record()
                     torch.cuda.synchronize(self.device)
                     times[cand] = ev0.elapsed_time(ev1) / reps
