@@ -17,8 +17,8 @@ whole fields (slices, vectors) make that subtree evaluate with torch ops.
 """
 import ast
 import ctypes
-import ctypes
 import hashlib
+import os
 
 import torch
 
@@ -30,6 +30,10 @@ _UNARY = {"sqrt": "sqrt", "abs": "fabs", "absolute": "fabs", "exp": "exp", "log"
           "floor": "floor"}
 _BINARY = {"minimum": "pyr_min", "maximum": "pyr_max", "arctan2": "atan2"}
 _CMP = {ast.Lt: "<", ast.Gt: ">", ast.LtE: "<=", ast.GtE: ">=", ast.Eq: "==", ast.NotEq: "!="}
+
+# sibling subtrees of one right-hand side with plain-read leaves share one
+# multi-output kernel (FusedGroup); PYRANDA_AMD_FUSE_GROUP=0 keeps one kernel each
+GROUP_SIBLINGS = os.environ.get("PYRANDA_AMD_FUSE_GROUP", "1") != "0"
 
 PRELUDE = r"""
 struct PyrPwArgs { long long n; void* p[24]; double s[16]; };
@@ -173,6 +177,7 @@ class FusedSubtree:
     """One maximal pointwise subtree: a callable over its leaf values."""
 
     def __init__(self, node, src_tag):
+        self.node = node
         g = _Gen()
         self.body = g.expr(node)
         self.leaves = g.leaves
@@ -286,6 +291,179 @@ class FusedSubtree:
         return out
 
 
+def _simple_load(node):
+    """A leaf that is a plain read (names, attributes, constant subscripts; no
+    call): evaluating it early has no effect and cannot fail differently."""
+    if isinstance(node, ast.Name):
+        return True
+    if isinstance(node, ast.Attribute):
+        return _simple_load(node.value)
+    if isinstance(node, ast.Subscript):
+        return isinstance(node.slice, ast.Constant) and _simple_load(node.value)
+    return _const(node)
+
+
+class FusedGroup:
+    """Sibling pointwise subtrees of one right-hand side whose leaves are
+    plain reads (the arguments of `-ddx(A) - ddy(B) - ddz(C)`): one kernel
+    reads the union of their inputs once and writes one output per subtree.
+    Each output is computed with its own subtree's expression and stored in
+    its own type, so the results equal the separate kernels' bit for bit;
+    where a separate evaluation would not take the kernel path the group
+    falls back to the separate subtrees."""
+
+    def __init__(self, subs, src_tag):
+        self.subs = subs
+        g = _Gen()
+        self.bodies = [g.expr(sub.node) for sub in subs]
+        self.leaves = g.leaves
+        # each subtree's leaves as indices into the union
+        self.maps = [[g.leaf_src[ast.unparse(l)] for l in sub.leaves] for sub in subs]
+        self._kernels = {}
+        self.tag = src_tag
+
+    def _source(self, sig, out_dts):
+        key = "|".join(self.bodies) + repr(sig) + repr(out_dts)
+        name = "pyr_pwg_" + hashlib.sha1(key.encode()).hexdigest()[:16]
+        lines = [PRELUDE, 'extern "C" __global__ void %s(PyrPwArgs a) {' % name]
+        ip = isc = 0
+        for i, s in enumerate(sig):
+            if s == "S":
+                lines.append("  const double x%d = a.s[%d];" % (i, isc))
+                isc += 1
+            elif s in ("D64", "D32"):
+                ct = "double" if s == "D64" else "float"
+                lines.append("  const double x%d = (double)*(const %s*)a.p[%d];" % (i, ct, ip))
+                ip += 1
+            else:
+                ct = "double" if s in ("T64", "G64") else "float"
+                lines.append("  const %s* __restrict__ p%d = (const %s*)a.p[%d];" % (ct, i, ct, ip))
+                ip += 1
+        if any(s[0] == "G" for s in sig):
+            lines.append("  const int* __restrict__ pmid = (const int*)a.p[%d];" % ip)
+            ip += 1
+        for k, dt in enumerate(out_dts):
+            ot = "double" if dt == "f64" else "float"
+            lines.append("  %s* __restrict__ out%d = (%s*)a.p[%d];" % (ot, k, ot, ip + k))
+        lines.append("  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;"
+                     " i += (long long)gridDim.x * blockDim.x) {")
+        for i, s in enumerate(sig):
+            if s in ("T64", "T32"):
+                lines.append("    const double x%d = (double)p%d[i];" % (i, i))
+            elif s in ("G64", "G32"):
+                lines.append("    const double x%d = (double)p%d[pmid[i]];" % (i, i))
+        for k, (body, dt) in enumerate(zip(self.bodies, out_dts)):
+            lines.append("    out%d[i] = (%s)(%s);" % (k, "double" if dt == "f64" else "float", body))
+        lines.append("  }")
+        lines.append("}")
+        return name, "\n".join(lines)
+
+    def kernel(self, sig, out_dts):
+        key = (sig, out_dts)
+        fn = self._kernels.get(key)
+        if fn is None:
+            name, src = self._source(sig, out_dts)
+            h = ctypes.c_void_p()
+            L = _lib.lib()
+            if L.pyr_pw_compile(src.encode(), name.encode(), ctypes.byref(h)) != 0:
+                raise _lib.PyrandaAMDError(L.pyr_pw_last_error().decode())
+            fn = h
+            self._kernels[key] = fn
+        return fn
+
+    def _separate(self, shape, vals):
+        return tuple(sub(shape, *[vals[j] for j in m]) for sub, m in zip(self.subs, self.maps))
+
+    def __call__(self, shape, *vals):
+        sig, fields, scalars = [], [], []
+        is64 = []
+        dev = None
+        for v in vals:
+            if torch.is_tensor(v) and v.dim() == 0 and not _device_scalar(v):
+                v = v.item()
+            if isinstance(v, (int, float)) or (hasattr(v, "dtype") and not torch.is_tensor(v)
+                                                and getattr(v, "shape", None) == ()):
+                sig.append("S")
+                scalars.append(float(v))
+            elif _device_scalar(v):
+                sig.append("D64" if v.dtype == torch.float64 else "D32")
+                fields.append(v)
+            elif _member_scalar(v, shape):
+                sig.append("G64" if v.dtype == torch.float64 else "G32")
+                fields.append(v)
+            elif _field_ok(v, shape):
+                sig.append("T64" if v.dtype == torch.float64 else "T32")
+                fields.append(v)
+                dev = v.device
+            else:
+                return self._separate(shape, vals)
+            is64.append(sig[-1] == "T64")
+        out_dts = []
+        for m in self.maps:
+            # every subtree needs a field operand to take the kernel path
+            if not any(sig[j][0] == "T" for j in m):
+                return self._separate(shape, vals)
+            out_dts.append("f64" if any(is64[j] for j in m) else "f32")
+        if any(k[0] == "G" for k in sig):
+            fields.append(shape.mid)
+        nout = len(self.subs)
+        if len(fields) + nout > _lib.PW_MAX_PTRS or len(scalars) > _lib.PW_MAX_SCALARS:
+            return self._separate(shape, vals)
+        fn = self.kernel(tuple(sig), tuple(out_dts))
+        nx, ny, nz = shape
+        outs = [torch.empty((nz, ny, nx), dtype=torch.float64 if dt == "f64" else torch.float32,
+                            device=dev).permute(2, 1, 0) for dt in out_dts]
+        ptrs = (ctypes.c_void_p * (len(fields) + nout))(*[f.data_ptr() for f in fields],
+                                                         *[o.data_ptr() for o in outs])
+        sc = (ctypes.c_double * max(1, len(scalars)))(*scalars)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
+        _lib.check_pw(_lib.lib().pyr_pw_launch(fn, stream, nx * ny * nz, len(fields) + nout, ptrs,
+                                               len(scalars), sc))
+        return tuple(outs)
+
+
+def _group_siblings(body, subtrees, tag):
+    """Rewrite `body` so the fused subtrees whose leaves are all plain reads,
+    outside conditional parts of the expression, are evaluated by one
+    FusedGroup call ahead of the rest: `(lambda _g: body')(_pwg(_shape, ...))`
+    with `_g[k]` in their places. Returns (new body, group or None)."""
+    picks = []
+
+    def walk(n, cond):
+        if (isinstance(n, ast.Call) and isinstance(n.func, ast.Subscript)
+                and isinstance(n.func.value, ast.Name) and n.func.value.id == "_pw"):
+            sub = subtrees[n.func.slice.value]
+            if not cond and not sub.is_bool and all(_simple_load(l) for l in sub.leaves):
+                picks.append(n)
+                return
+        c = cond or isinstance(n, (ast.IfExp, ast.BoolOp, ast.Lambda, ast.ListComp, ast.GeneratorExp,
+                                   ast.SetComp, ast.DictComp))
+        for ch in ast.iter_child_nodes(n):
+            walk(ch, c)
+
+    walk(body, False)
+    if len(picks) < 2:
+        return body, None
+    group = FusedGroup([subtrees[n.func.slice.value] for n in picks], tag)
+    slot = {id(n): k for k, n in enumerate(picks)}
+
+    class Put(ast.NodeTransformer):
+        def visit_Call(self, n):
+            if id(n) in slot:
+                return ast.copy_location(ast.Subscript(value=ast.Name(id="_g", ctx=ast.Load()),
+                                                       slice=ast.Constant(value=slot[id(n)]),
+                                                       ctx=ast.Load()), n)
+            return self.generic_visit(n)
+
+    new = Put().visit(body)
+    inner = ast.Lambda(args=ast.arguments(posonlyargs=[], args=[ast.arg(arg="_g")], kwonlyargs=[],
+                                          kw_defaults=[], defaults=[]), body=new)
+    call = ast.Call(func=ast.Name(id="_pwg", ctx=ast.Load()),
+                    args=[ast.Name(id="_shape", ctx=ast.Load())] + [_copy(l) for l in group.leaves],
+                    keywords=[])
+    return ast.Call(func=inner, args=[call], keywords=[]), group
+
+
 def _rename_leaves(node, gen):
     """Copy of the subtree with leaves replaced by their x<i> names."""
 
@@ -354,9 +532,10 @@ class FusedRHS:
         body = f.visit(tree.body)
         self.subtrees = f.subtrees
         self.src = src
+        body, self.group = _group_siblings(body, self.subtrees, src[:40]) if GROUP_SIBLINGS else (body, None)
         lam = ast.Expression(body=ast.Lambda(
             args=ast.arguments(posonlyargs=[], args=[ast.arg(arg="self"), ast.arg(arg="_shape"),
-                                                     ast.arg(arg="_pw")],
+                                                     ast.arg(arg="_pw"), ast.arg(arg="_pwg")],
                                kwonlyargs=[], kw_defaults=[], defaults=[]),
             body=body))
         ast.fix_missing_locations(lam)
@@ -364,7 +543,8 @@ class FusedRHS:
 
     def __call__(self, sim):
         shape = sim.mesh.shape
-        return self._fn(sim, shape if isinstance(shape, GroupShape) else tuple(shape), self.subtrees)
+        return self._fn(sim, shape if isinstance(shape, GroupShape) else tuple(shape), self.subtrees,
+                        self.group)
 
 
 _RED = {

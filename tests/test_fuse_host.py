@@ -68,6 +68,26 @@ def test_generated_kernels_compile(amd):
     assert n > 20
 
 
+def test_sibling_groups(amd):
+    """The three flux arguments of every Taylor-Green PDE line form one
+    multi-output group (their leaves are plain field reads); operator
+    results never join a group; the group kernels compile with hiprtc."""
+    L = amd.lib()
+    groups = []
+    for src in _tg_rhs():
+        f = fuse.FusedRHS(src, {})
+        if f.group is not None:
+            groups.append(f.group)
+            assert src.startswith("-self.ddx("), src
+    assert [len(g.subs) for g in groups] == [3] * 5
+    for g in groups:
+        for t in ("T64", "T32"):
+            sig = tuple(t for _ in g.leaves)
+            dts = tuple("f64" if t == "T64" else "f32" for _ in g.subs)
+            name, code = g._source(sig, dts)
+            assert L.pyr_pw_check_source(code.encode(), name.encode()) == 0, L.pyr_pw_last_error().decode()
+
+
 class _Sim:
     """CPU stand-in: fields are CPU tensors, operators are simple functions."""
 

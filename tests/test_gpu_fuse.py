@@ -87,3 +87,51 @@ def test_timestep_reductions(dev, dtype):
     got = T._DIFF(shape, [bulk, rho], [gl])
     ref = ((rho.double() * gl) * gl / torch.clamp(bulk.double(), min=1e-12)).min().item()
     assert got == ref
+
+
+@pytest.mark.parametrize("mixed", [False, True])
+def test_sibling_group_equals_separate_kernels(dev, mixed):
+    """The flux arguments of each Taylor-Green PDE line, evaluated by one
+    multi-output kernel (FusedGroup), give bit for bit the right-hand sides
+    of one kernel per subtree; FP32 fields, or with one FP64 field mixed in
+    (then only the outputs that read it are FP64)."""
+    sys.path.insert(0, __import__("os").path.dirname(__file__))
+    from test_fuse_host import _tg_rhs
+    from pyranda_amd import devnumpy, fuse
+    shape = (40, 33, 17)
+    gg = torch.Generator().manual_seed(11)
+    names = ["rho", "u", "v", "w", "rhou", "rhov", "rhow", "Et", "tauxx", "tauyy", "tauzz", "tauxy",
+             "tauxz", "tauyz"]
+    sim = types.SimpleNamespace(mesh=types.SimpleNamespace(shape=list(shape)), variables={})
+    for k in names:
+        dt = torch.float64 if mixed and k == "tauxy" else torch.float32
+        sim.variables[k] = types.SimpleNamespace(data=_field(shape, dev, dt, gg))
+    # operators stand-ins that keep the arguments (the pointwise results are compared)
+    seen = []
+    sim.ddx = sim.ddy = sim.ddz = lambda v: (seen.append(v), v)[1]
+    g = {"numpy": devnumpy.numpy}
+    pdes = [src for src in _tg_rhs() if src.startswith("-self.ddx(")]
+    assert len(pdes) == 5
+    for src in pdes:
+        old = fuse.GROUP_SIBLINGS
+        try:
+            fuse.GROUP_SIBLINGS = False
+            sep = fuse.FusedRHS(src, g)
+            fuse.GROUP_SIBLINGS = True
+            grp = fuse.FusedRHS(src, g)
+        finally:
+            fuse.GROUP_SIBLINGS = old
+        assert sep.group is None and grp.group is not None and len(grp.group.subs) == 3
+        seen.clear()
+        a = sep(sim)
+        args_a = list(seen)
+        seen.clear()
+        b = grp(sim)
+        args_b = list(seen)
+        torch.cuda.synchronize()
+        assert torch.equal(a, b), src
+        for x, y in zip(args_a, args_b):
+            assert x.dtype == y.dtype and torch.equal(x, y), src
+        if mixed:
+            assert any(x.dtype == torch.float64 for x in args_b) == ("tauxy" in src), src
+    _say("sibling groups equal", "mixed" if mixed else "fp32")
