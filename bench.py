@@ -64,13 +64,14 @@ def tg_mesh(n):
 
 
 def make_tg(n, precision, p0, solver="circulant"):
-    """precision: a pyrandaSim precision, or "fp32-rounded-state" (fp32 with
-    fp64_state=False: the conservative filter rounds the state to FP32)."""
+    """precision: a pyrandaSim precision, "fp32-rounded-state" (fp32 with
+    fp64_state=False: the conservative filter rounds the state to FP32) or
+    "fp32-refined-only" (fp32 without the per-operator solve choice)."""
     import pyranda_amd as pr
     from pyranda_amd.workloads import TG_EOM, TG_IC
     rounded = precision == "fp32-rounded-state"
-    ss = pr.pyrandaSim("TGvortex", tg_mesh(n), silent=True, precision="fp32" if rounded else precision,
-                       fp64_state=not rounded)
+    base = "fp32" if precision in ("fp32-rounded-state", "fp32-refined-only") else precision
+    ss = pr.pyrandaSim("TGvortex", tg_mesh(n), silent=True, precision=base, fp64_state=not rounded)
     ss.PyMPI.ctx.set_periodic_solver(solver)
     ss.addPackage(pr.pyrandaTimestep(ss))
     ss.EOM(TG_EOM)
@@ -415,9 +416,11 @@ def main():
                     help="solver of periodic implicit lines")
     ap.add_argument("--r32-solve", default="refined", choices=["refined", "fp64"],
                     help="--ops: arithmetic of the solves on FP32 fields")
-    ap.add_argument("--precision", default="fp64,fp32,fp32-fp64solve,fp32-rounded-state",
-                    help="comma-separated pyrandaSim precisions to time (the headline is fp32; "
-                         "fp32-rounded-state is fp32 without the FP64 state through the filter)")
+    ap.add_argument("--precision", default="fp64,fp32,fp32-refined-only,fp32-fp64solve,fp32-rounded-state",
+                    help="comma-separated pyrandaSim precisions to time (the headline is fp32, whose "
+                         "autotuning also picks refined-FP32 or FP64 solves per operator; "
+                         "fp32-refined-only keeps the refined solves everywhere; fp32-rounded-state is "
+                         "fp32 without the FP64 state through the filter)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (no per-operator autotuning before timing)")
     ap.add_argument("--ensemble", action="store_true",
@@ -471,7 +474,7 @@ def main():
             dt = float(ss.variables["dt"].data) * 0.5  # CFL 0.5 from the initial state
         ctx = ss.PyMPI.ctx
         if not args.no_autotune:
-            rep = ss.autotune()
+            rep = ss.autotune(solve_modes=precision in ("fp32", "fp32-rounded-state"))
             tuned[precision] = {"_".join([OPS[key[0]], DIRS[key[1]]] + list(key[2:])):
                                 ",".join("%s=%d" % kv for kv in best)
                                 for key, (best, _) in sorted(rep.items(), key=lambda kv: str(kv[0]))}
@@ -516,6 +519,8 @@ def main():
                 acc_s[k] = rel_err(states["fp32-fp64solve"][k], states["fp64"][k])
     acc_r = {k: rel_err(states["fp32-rounded-state"][k], states["fp64"][k])
              for k in states["fp32"]} if "fp64" in states and "fp32-rounded-state" in states else {}
+    acc_o = {k: rel_err(states["fp32-refined-only"][k], states["fp64"][k])
+             for k in states["fp32"]} if "fp64" in states and "fp32-refined-only" in states else {}
 
     cpu = None
     if not args.no_cpu:
@@ -538,7 +543,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 fields and sweeps, f64 residual refinement, f64 RK4 state",
+        "dtype": "f32 fields; per operator f32 sweeps with f64 residual refinement or f64 sweeps "
+                 "(autotuned); f64 RK4 state",
         "data": "synthetic (Taylor-Green initial condition)",
         "config": {"workload": "taylor_green_euler_lad_%d^3" % n, "grid": [n, n, n], "gamma": gamma,
                    "mach": 0.1, "cfl": 0.5, "dt": dt, "rk_stages": 5, "lad": True,
@@ -554,6 +560,12 @@ def main():
             if "fp64" in results else None,
             "accuracy_vs_fp64": {k: {"rel_max": a, "rel_l2": b} for k, (a, b) in acc_s.items()},
         } if "fp32-fp64solve" in results else None,
+        # fp32 with the refined FP32 solve for every operator (no per-operator choice)
+        "fp32_refined_only": {
+            "value": results["fp32-refined-only"]["pups"],
+            "ms_per_step": results["fp32-refined-only"]["ms_per_step"],
+            "accuracy_vs_fp64": {k: {"rel_max": a, "rel_l2": b} for k, (a, b) in acc_o.items()},
+        } if "fp32-refined-only" in results else None,
         # fp32 whose conservative filter rounds the RK state to FP32 each stage
         "fp32_rounded_state": {
             "value": results["fp32-rounded-state"]["pups"],

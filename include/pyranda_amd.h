@@ -75,6 +75,12 @@ int pyr_set_periodic_solver(pyr_ctx* ctx, int mode);
  * same rate, so this is the cheaper of the two). */
 enum { PYR_R32_SOLVE_REFINED = 0, PYR_R32_SOLVE_FP64 = 1 };
 int pyr_set_r32_solve(pyr_ctx* ctx, int mode);
+/* The same choice for one operator (PYR_OP_*) along one direction, over the
+ * context's (-1 restores it); composites (filters, ring, Laplacian, grad,
+ * divergence) follow their operator per direction. Set by pyranda_amd's
+ * autotuner (ParcopContext.autotune with solve_modes). No counterpart in the
+ * reference. */
+int pyr_set_solve_tuning(pyr_ctx* ctx, int op, int dir, int r32_solve);
 const char* pyr_last_error(void);
 
 /* --- mesh (parcop.f90 xgrid/ygrid/zgrid/dxgrid/mesh_getgridlen; mesh.f90:173-212)

@@ -61,6 +61,7 @@ SIGNATURES = {
                                             ctypes.POINTER(_vp), ctypes.c_int, ctypes.POINTER(ctypes.c_double)]),
     "pyr_pw_last_error": (ctypes.c_char_p, []),
     "pyr_set_r32_solve": (ctypes.c_int, [_vp, ctypes.c_int]),
+    "pyr_set_solve_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pyr_sfilter_f64_rounded": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "pyr_apply_rounded": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]),
 }

@@ -808,10 +808,14 @@ struct pyr_ctx {
   int tune_xlds[NOPS][3][3];
   int tune_pipe[NOPS][3][3];
   int tune_ly[NOPS][3][3];
+  // per operator and direction: FP32 solve arithmetic (PYR_R32_SOLVE_*), -1 = r32_solve
+  int tune_solve[NOPS][3];
   pyr_ctx() {
     for (int o = 0; o < NOPS; ++o)
-      for (int d = 0; d < 3; ++d)
+      for (int d = 0; d < 3; ++d) {
+        tune_solve[o][d] = -1;
         for (int m = 0; m < 3; ++m) tune_xlds[o][d][m] = tune_pipe[o][d][m] = tune_ly[o][d][m] = -1;
+      }
   }
   int ncu = 256;                             // compute units of the device
   float* out32 = nullptr;                    // FP32 copy of the next FP64 y/z sweep's output
@@ -1155,6 +1159,12 @@ static int check_ctx(pyr_ctx* c) {
   return 0;
 }
 
+// FP32 solve arithmetic of one operator and direction: its tuned choice, else
+// the context's
+static int r32_mode(const pyr_ctx* c, int op, int dir) {
+  return c->tune_solve[op][dir] >= 0 ? c->tune_solve[op][dir] : c->r32_solve;
+}
+
 static int apply_any(pyr_ctx* c, int op, int dir, int prec, const void* in, void* out, int epi,
                      double ec) {
   if (check_ctx(c)) return 1;
@@ -1164,7 +1174,7 @@ static int apply_any(pyr_ctx* c, int op, int dir, int prec, const void* in, void
   if (prec == PYR_F64)
     return launch_op<double, MODE_F64>(c, op, dir, (const double*)in, (double*)out, epi, ec);
   if (prec == PYR_R32) {
-    if (c->r32_solve == PYR_R32_SOLVE_FP64)
+    if (r32_mode(c, op, dir) == PYR_R32_SOLVE_FP64)
       return launch_op<float, MODE_S32>(c, op, dir, (const float*)in, (float*)out, epi, ec);
     return launch_op<float, MODE_R32>(c, op, dir, (const float*)in, (float*)out, epi, ec);
   }
@@ -1680,11 +1690,20 @@ int pyr_set_tuning(pyr_ctx* c, int op, int dir, int prec, int x_lds_kb, int yz_p
   if (yz_lines >= 0 && yz_lines != 32 && yz_lines != 64) return fail("yz_lines must be 32 or 64 (or -1)");
   int mode;
   if (prec == PYR_F64) mode = MODE_F64;
-  else if (prec == PYR_R32) mode = c->r32_solve == PYR_R32_SOLVE_FP64 ? MODE_S32 : MODE_R32;
+  else if (prec == PYR_R32) mode = r32_mode(c, op, dir) == PYR_R32_SOLVE_FP64 ? MODE_S32 : MODE_R32;
   else return fail("bad precision");
   c->tune_xlds[op][dir][mode] = x_lds_kb < 0 ? -1 : x_lds_kb;
   c->tune_pipe[op][dir][mode] = yz_pipe < 0 ? -1 : yz_pipe;
   c->tune_ly[op][dir][mode] = yz_lines < 0 ? -1 : yz_lines;
+  return 0;
+}
+
+int pyr_set_solve_tuning(pyr_ctx* c, int op, int dir, int r32_solve) {
+  if (!c) return fail("null context");
+  if (op < 0 || op >= NOPS || dir < 0 || dir > 2) return fail("bad operator or direction");
+  if (r32_solve != -1 && r32_solve != PYR_R32_SOLVE_REFINED && r32_solve != PYR_R32_SOLVE_FP64)
+    return fail("bad r32 solve mode");
+  c->tune_solve[op][dir] = r32_solve;
   return 0;
 }
 
@@ -1917,7 +1936,7 @@ int pyr_host_emulate(pyr_ctx* c, int op, int dir, int prec, const void* in, void
   if (op < 0 || op >= NOPS || dir < 0 || dir > 2) return fail("bad operator or direction");
   if (prec == PYR_F64) return emulate<double, MODE_F64>(c, op, dir, (const double*)in, (double*)out);
   if (prec == PYR_R32) {
-    if (c->r32_solve == PYR_R32_SOLVE_FP64) return emulate<float, MODE_S32>(c, op, dir, (const float*)in, (float*)out);
+    if (r32_mode(c, op, dir) == PYR_R32_SOLVE_FP64) return emulate<float, MODE_S32>(c, op, dir, (const float*)in, (float*)out);
     return emulate<float, MODE_R32>(c, op, dir, (const float*)in, (float*)out);
   }
   return fail("bad precision");

@@ -108,6 +108,12 @@ class ParcopContext:
         """'circulant' (default) or 'lu': solver of periodic implicit lines."""
         _lib.check(self._L.pyr_set_periodic_solver(self._h, _lib.PSOLVE[name]))
 
+    def set_solve_tuning(self, op, direction, name=None):
+        """FP32 solve arithmetic of one operator along one direction
+        (pyr_set_solve_tuning): 'refined', 'fp64', or None for the context's."""
+        _lib.check(self._L.pyr_set_solve_tuning(self._h, op, direction,
+                                                -1 if name is None else _lib.R32_SOLVE[name]))
+
     def set_tuning(self, op, direction, dtype, x_lds_kb=-1, yz_pipe=-1, yz_lines=-1):
         """Per-operator knobs (pyr_set_tuning): LDS budget of x-line
         workgroups (KB), the pipelined y/z kernel, lines per plain y/z
@@ -116,7 +122,8 @@ class ParcopContext:
         _lib.check(self._L.pyr_set_tuning(self._h, op, direction, prec, int(x_lds_kb), int(yz_pipe),
                                           int(yz_lines)))
 
-    def autotune(self, dtype, ops=None, reps=5, x_lds_kb=(8, 12, 16, 24, 40, 64), seed=7, rounded=False):
+    def autotune(self, dtype, ops=None, reps=5, x_lds_kb=(8, 12, 16, 24, 40, 64), seed=7, rounded=False,
+                 solve_modes=False):
         """Time each operator and direction at this precision with every
         candidate (x: LDS budget, hence lines per workgroup; y/z: pipelined
         kernel or not) on random fields and keep the fastest. The timed
@@ -124,7 +131,11 @@ class ParcopContext:
         last-level cache, so every launch reads its input from HBM as the
         operators of a time step do. rounded: FP64
         z sweeps are timed storing the FP32 copy of their result as well (the
-        last sweep of the filter of an FP32 run's FP64 state). Returns
+        last sweep of the filter of an FP32 run's FP64 state). solve_modes
+        (FP32 fields): the candidates also range over the solve arithmetic,
+        refined FP32 or FP64 (("r32_solve", 0 or 1) in the choice); the two
+        agree to the FP32 rounding of the result, so unlike the launch knobs
+        this choice can change results in the last bits. Returns
         {(op, dir): (choice, {choice: mean ms})}."""
         ops = ops if ops is not None else (_lib.OP_D1, _lib.OP_D2, _lib.OP_D8, _lib.OP_SFILTER, _lib.OP_GFILTER)
         g = torch.Generator(device=self.device).manual_seed(seed)
@@ -146,9 +157,11 @@ class ParcopContext:
                 else:
                     cands = [(("yz_pipe", 1),), (("yz_pipe", 0), ("yz_lines", 64)),
                              (("yz_pipe", 0), ("yz_lines", 32))]
+                if solve_modes and dtype == torch.float32:
+                    cands = [(("r32_solve", m),) + c for m in (0, 1) for c in cands]
                 times = {}
                 for cand in cands:
-                    self.set_tuning(op, d, dtype, **dict(cand))
+                    self._set_choice(op, d, dtype, cand)
                     self.apply(op, d, v, out=out, **kw)
                     ev0 = torch.cuda.Event(enable_timing=True)
                     ev1 = torch.cuda.Event(enable_timing=True)
@@ -159,9 +172,16 @@ class ParcopContext:
                     torch.cuda.synchronize(self.device)
                     times[cand] = ev0.elapsed_time(ev1) / reps
                 best = min(times, key=times.get)
-                self.set_tuning(op, d, dtype, **dict(best))
+                self._set_choice(op, d, dtype, best)
                 report[(op, d)] = (best, times)
         return report
+
+    def _set_choice(self, op, d, dtype, choice):
+        kw = dict(choice)
+        if "r32_solve" in kw:
+            # the solve mode first: the launch knobs are kept per solve mode
+            self.set_solve_tuning(op, d, ("refined", "fp64")[kw.pop("r32_solve")])
+        self.set_tuning(op, d, dtype, **kw)
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:

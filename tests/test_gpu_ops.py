@@ -236,3 +236,44 @@ def test_filter_with_rounded_copy(dev, nn, kinds):
         _lib.check(ctx._L.pyr_apply_rounded(ctx._h, _lib.OP_SFILTER, 0, ctx._ptr(v), ctx._ptr(out),
                                             ctx._ptr(out32)))
     ctx.close()
+
+
+def test_autotune_solve_modes(dev):
+    """With solve_modes the autotuner also picks, per operator and direction,
+    refined-FP32 or FP64 solves for FP32 fields: every result is bit for bit
+    the one of a context using the chosen arithmetic everywhere, and within
+    the FP32-refined bar of the FP64 path on the same input."""
+    from pyranda_amd import _lib
+    from pyranda_amd.parcop import ParcopContext
+    nn = (64, 96, 64)
+    L = 2 * math.pi
+    xn = tuple(L * (n - 1) / n for n in nn)
+    ctx = ParcopContext(nn, (0.0, 0.0, 0.0), xn, ["PERI"] * 6, device=dev)
+    ref = {m: ParcopContext(nn, (0.0, 0.0, 0.0), xn, ["PERI"] * 6, device=dev) for m in ("refined", "fp64")}
+    ref["fp64"].set_r32_solve("fp64")
+    g = torch.Generator(device=dev).manual_seed(5)
+    v = torch.randn(tuple(reversed(nn)), generator=g, device=dev, dtype=torch.float64).permute(2, 1, 0)
+    v32 = v.float()
+    ops = (_lib.OP_D1, _lib.OP_D2, _lib.OP_D8, _lib.OP_SFILTER, _lib.OP_GFILTER)
+    rep = ctx.autotune(torch.float32, reps=2, solve_modes=True)
+    chosen = {}
+    for (op, d), (best, times) in rep.items():
+        assert best[0][0] == "r32_solve" and best in times
+        mode = ("refined", "fp64")[best[0][1]]
+        chosen[(op, d)] = mode
+        got = ctx.apply(op, d, v32)
+        assert torch.equal(got, ref[mode].apply(op, d, v32)), (op, d, mode)
+        exact = ref["refined"].apply(op, d, v32.double())
+        err = float((got.double() - exact).abs().max() / exact.abs().max())
+        assert err <= 1e-6, (op, d, mode, err)
+    # composites follow the per-direction choice
+    f = ctx.sfilter(v32)
+    a = ref[chosen[(_lib.OP_SFILTER, 0)]].apply(_lib.OP_SFILTER, 0, v32)
+    b = ref[chosen[(_lib.OP_SFILTER, 1)]].apply(_lib.OP_SFILTER, 1, a)
+    c = ref[chosen[(_lib.OP_SFILTER, 2)]].apply(_lib.OP_SFILTER, 2, b)
+    assert torch.equal(f, c)
+    with pytest.raises(_lib.PyrandaAMDError):
+        _lib.check(ctx._L.pyr_set_solve_tuning(ctx._h, 0, 0, 7))
+    _say("solve modes", {k: m for k, m in chosen.items()})
+    for c_ in [ctx] + list(ref.values()):
+        c_.close()
