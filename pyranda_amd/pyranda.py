@@ -406,14 +406,15 @@ class pyrandaSim:
 
     def _mark_state(self, U):
         d = self.variables[U].data
-        self._state_src[U] = (d, d._version)
+        self._state_src[U] = (d, getattr(d, "_version", None))
 
     def _state_current(self, U):
         """The FP64 state of U still matches its field: the field is the
         tensor derived from it and nothing wrote into it since."""
         src = self._state_src.get(U)
         d = self.variables[U].data
-        return U in self._state64 and src is not None and src[0] is d and d._version == src[1]
+        return (U in self._state64 and src is not None and torch.is_tensor(d) and src[0] is d
+                and d._version == src[1])
 
     def _self_filtered(self):
         """Conserved variables whose only ALG writer is the conservative
