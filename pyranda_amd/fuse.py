@@ -553,6 +553,7 @@ _RED = {
     "min": ("__builtin_huge_val()", "((v < acc || v != v || acc != acc) ? (acc != acc ? acc : v) : acc)"),
 }
 _RED_BLOCKS = 1024
+_RED_UNROLL = int(os.environ.get("PYRANDA_AMD_RED_UNROLL", "4"))
 
 
 class FusedReduce:
@@ -568,7 +569,8 @@ class FusedReduce:
 
     def _source(self, sig):
         ident, comb = _RED[self.op]
-        name = "pyr_red_" + hashlib.sha1((self.op + self.body + repr(sig)).encode()).hexdigest()[:16]
+        name = "pyr_red_" + hashlib.sha1((self.op + self.body + repr(sig) + "u%d" % _RED_UNROLL).encode()
+                                         ).hexdigest()[:16]
         L = [PRELUDE, 'extern "C" __global__ void __launch_bounds__(256) %s(PyrPwArgs a) {' % name]
         for i, t in enumerate(sig):
             ct = "double" if t == "T64" else "float"
@@ -577,6 +579,8 @@ class FusedReduce:
             L.append("  const double x%d = a.s[%d];" % (self.nf + j, j))
         L.append("  double* __restrict__ part = (double*)a.p[%d];" % len(sig))
         L.append("  double acc = %s;" % ident)
+        # several iterations' loads in flight (max/min do not depend on the order)
+        L.append("#pragma unroll %d" % _RED_UNROLL)
         L.append("  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;"
                  " i += (long long)gridDim.x * blockDim.x) {")
         for i in range(len(sig)):
