@@ -36,6 +36,11 @@ enum { PYR_BC_PERI = 0, PYR_BC_NONE = 1, PYR_BC_SYMM = 2 };
 
 /* operator ids for the generic entry point */
 enum { PYR_OP_D1 = 0, PYR_OP_D2 = 1, PYR_OP_D8 = 2, PYR_OP_SFILTER = 3, PYR_OP_GFILTER = 4 };
+/* or-ed into an operator id: the antisymmetric member of the SYMM pair
+ * (compact.f90:78-84) where the direction has a SYMM end, as
+ * d1x/d8x(v, dv, bc=-1) select it (compact_operators.f90:300-303); elsewhere
+ * the default operator */
+enum { PYR_OP_ANTISYM = 0x10 };
 
 /* how a directional result lands in `out` (fuses the pointwise combination of
  * lap/div/ring into the operator's store) */
@@ -89,7 +94,11 @@ const char* pyr_last_error(void);
 int pyr_axis_coords(pyr_ctx* ctx, int axis, double* coords_host);
 int pyr_mesh_scalars(pyr_ctx* ctx, double* dpatch3, double* dmesh3, double* gridlen);
 
-/* --- FP64 operators (device pointers) ---------------------------------- */
+/* --- FP64 operators (device pointers) ----------------------------------
+ * divergence and pRingV differentiate the normal component of each
+ * direction with the antisymmetric member of the SYMM pair where that
+ * direction has a SYMM end (ddx(fx,..,isymX), ringx(f,..,isymX);
+ * operators.f90:38-53,661-671); every other operator uses the default. */
 int pyr_ddx_f64(pyr_ctx* ctx, const double* val, double* dval);      /* parcop.f90:225 ddx */
 int pyr_ddy_f64(pyr_ctx* ctx, const double* val, double* dval);      /* parcop.f90:235 ddy */
 int pyr_ddz_f64(pyr_ctx* ctx, const double* val, double* dval);      /* parcop.f90:245 ddz */

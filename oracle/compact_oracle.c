@@ -647,6 +647,10 @@ struct ora_ctx {
   double d[3];      /* nominal spacing for the metric (compact_operators) */
   double mesh_d[3]; /* mesh d1,d2,d3 (mesh.f90:190-211) */
   ora_op1 ops[ORA_NOPS][3];
+  /* second member of the symmetric/antisymmetric pair (compact.f90:78-84):
+     variant -1 at each SYMM end, built only where a direction has one */
+  int has_anti[3];
+  ora_op1 anti[ORA_NOPS][3];
 };
 
 ora_ctx* ora_create(int nx, int ny, int nz, double dx, double dy, double dz, const int* bc6) {
@@ -663,6 +667,9 @@ ora_ctx* ora_create(int nx, int ny, int nz, double dx, double dy, double dz, con
     const int v1 = (b1 == ORA_BC_SYMM) ? 1 : 0;
     const int v2 = (b2 == ORA_BC_SYMM) ? 1 : 0;
     for (int op = 0; op < ORA_NOPS; ++op) op1_setup(&c->ops[op][dir], op, c->n[dir], periodic, v1, v2);
+    c->has_anti[dir] = !periodic && (v1 || v2);
+    if (c->has_anti[dir])
+      for (int op = 0; op < ORA_NOPS; ++op) op1_setup(&c->anti[op][dir], op, c->n[dir], 0, -v1, -v2);
   }
   return c;
 }
@@ -670,15 +677,21 @@ ora_ctx* ora_create(int nx, int ny, int nz, double dx, double dy, double dz, con
 void ora_destroy(ora_ctx* c) {
   if (!c) return;
   for (int op = 0; op < ORA_NOPS; ++op)
-    for (int d = 0; d < 3; ++d) op1_free(&c->ops[op][d]);
+    for (int d = 0; d < 3; ++d) {
+      op1_free(&c->ops[op][d]);
+      if (c->has_anti[d]) op1_free(&c->anti[op][d]);
+    }
   free(c);
 }
 
-int ora_apply(ora_ctx* c, int op, int dir, const double* v, double* dv) {
+/* isym = -1 selects the antisymmetric member of a SYMM pair where the
+   direction has one, as d1x/d8x(...,bc) do (compact_operators.f90:300-303);
+   otherwise (and for isym = +1) the default operator. */
+int ora_apply_sym(ora_ctx* c, int op, int dir, int isym, const double* v, double* dv) {
   if (op < 0 || op >= ORA_NOPS || dir < 0 || dir > 2) return -1;
   const int nx = c->n[0], ny = c->n[1], nz = c->n[2];
   const int n = c->n[dir];
-  const ora_op1* o = &c->ops[op][dir];
+  const ora_op1* o = (isym == -1 && c->has_anti[dir]) ? &c->anti[op][dir] : &c->ops[op][dir];
   const size_t stride = dir == 0 ? 1 : (dir == 1 ? (size_t)nx : (size_t)nx * ny);
   const int n1 = dir == 0 ? ny : nx;
   const int n2 = dir == 2 ? ny : nz;
@@ -705,6 +718,13 @@ int ora_apply(ora_ctx* c, int op, int dir, const double* v, double* dv) {
   }
   return 0;
 }
+
+int ora_apply(ora_ctx* c, int op, int dir, const double* v, double* dv) {
+  return ora_apply_sym(c, op, dir, 1, v, dv);
+}
+
+/* isymX/Y/Z of a patch: -1 where the direction has a SYMM end (patch.f90:89-91) */
+static int isym(const ora_ctx* c, int dir) { return c->has_anti[dir] ? -1 : 1; }
 
 static size_t npts(const ora_ctx* c) { return (size_t)c->n[0] * c->n[1] * c->n[2]; }
 
@@ -743,7 +763,8 @@ int ora_ring(ora_ctx* c, const double* v, double* out) {
   return 0;
 }
 
-/* ringV with L=1, operators.f90:645-699 */
+/* ringV with L=1, operators.f90:645-699; the normal component of each
+   direction takes the antisymmetric operator at SYMM ends (:661-671) */
 int ora_ringv(ora_ctx* c, const double* vx, const double* vy, const double* vz, double* out) {
   const size_t N = npts(c);
   const double* f[3] = {vx, vy, vz};
@@ -752,7 +773,7 @@ int ora_ringv(ora_ctx* c, const double* vx, const double* vy, const double* vz, 
     for (int d = 0; d < 3; ++d) {
       r[comp][d] = (double*)malloc(sizeof(double) * N);
       if (c->n[d] == 1) memset(r[comp][d], 0, sizeof(double) * N);
-      else ora_apply(c, ORA_D8, d, f[comp], r[comp][d]);
+      else ora_apply_sym(c, ORA_D8, d, comp == d ? isym(c, d) : 1, f[comp], r[comp][d]);
     }
   for (size_t i = 0; i < N; ++i) {
     double L[3];
@@ -783,15 +804,15 @@ int ora_laplacian(ora_ctx* c, const double* v, double* out) {
   return 0;
 }
 
-/* divV Cartesian, operators.f90:38-53 */
+/* divV Cartesian, operators.f90:38-53 (ddx(fx,..,isymX) etc.) */
 int ora_divergence(ora_ctx* c, const double* fx, const double* fy, const double* fz, double* out) {
   const size_t N = npts(c);
   double* a = (double*)malloc(sizeof(double) * N);
   double* b = (double*)malloc(sizeof(double) * N);
   double* e = (double*)malloc(sizeof(double) * N);
-  ora_apply(c, ORA_D1, 0, fx, a);
-  ora_apply(c, ORA_D1, 1, fy, b);
-  ora_apply(c, ORA_D1, 2, fz, e);
+  ora_apply_sym(c, ORA_D1, 0, isym(c, 0), fx, a);
+  ora_apply_sym(c, ORA_D1, 1, isym(c, 1), fy, b);
+  ora_apply_sym(c, ORA_D1, 2, isym(c, 2), fz, e);
   for (size_t i = 0; i < N; ++i) out[i] = a[i] + b[i] + e[i];
   free(a); free(b); free(e);
   return 0;

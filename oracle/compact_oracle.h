@@ -73,6 +73,9 @@ void ora_destroy(ora_ctx* c);
 /* One directional operator with the reference's metric scaling and +v for
  * filters: dir 0=x 1=y 2=z. v and dv are (nx,ny,nz) column-major. */
 int ora_apply(ora_ctx* c, int op, int dir, const double* v, double* dv);
+/* isym=-1: antisymmetric member of the SYMM pair where the direction has one
+   (compact_operators.f90:300-303); +1: the default operator */
+int ora_apply_sym(ora_ctx* c, int op, int dir, int isym, const double* v, double* dv);
 
 /* Composites as parcop.f90 exposes them. */
 int ora_sfilter(ora_ctx* c, const double* v, double* out);   /* fbar */

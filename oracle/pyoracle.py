@@ -34,6 +34,7 @@ def load():
     lib.ora_create.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 3 + [ctypes.POINTER(ctypes.c_int)]
     lib.ora_destroy.argtypes = [ctypes.c_void_p]
     lib.ora_apply.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, _dp, _dp]
+    lib.ora_apply_sym.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _dp, _dp]
     for name in ("ora_sfilter", "ora_gfilter", "ora_ring", "ora_laplacian"):
         getattr(lib, name).argtypes = [ctypes.c_void_p, _dp, _dp]
     lib.ora_gfilterdir.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_int]
@@ -136,6 +137,15 @@ class Oracle:
         self._lib.ora_apply(self._h, op, direction, _ptr(v), _ptr(out))
         return out
 
+    def apply_sym(self, op, direction, isym, v):
+        """The operator with the SYMM-pair member selected by isym (+1: the
+        default, -1: the antisymmetric one where the direction has a SYMM
+        end), as d1x/d8x(v, dv, bc) select it (compact_operators.f90:300-303)."""
+        v = self._in(v)
+        out = self._out()
+        self._lib.ora_apply_sym(self._h, op, direction, int(isym), _ptr(v), _ptr(out))
+        return out
+
     def ddx(self, v): return self.apply(D1, 0, v)
     def ddy(self, v): return self.apply(D1, 1, v)
     def ddz(self, v): return self.apply(D1, 2, v)
@@ -211,3 +221,4 @@ def ref_weights(op):
     return {"nol": nol, "nor": nor, "ncl": ncl, "ncr": ncr, "null_option": nullopt,
             "implicit_op": impl, "ali": ali, "ari": ari, "alb1": alb1, "alb2": alb2,
             "arb1": arb1, "arb2": arb2}
+

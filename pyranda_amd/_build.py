@@ -17,12 +17,27 @@ FLAGS = [
 
 def build(verbose=False, out=OUT, defines=()):
     """Compile the library; `defines` (e.g. ["PYR_PREFETCH=0"]) set the tuning
-    knobs of pyr_core.hip for a variant written to `out`."""
+    knobs of pyr_core.hip for a variant written to `out`. The sources compile
+    to objects in parallel, then link."""
     tmp = out + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-D" + d for d in defines] + ["-o", tmp] + SRCS + ["-lhiprtc"]
+    defs = ["-D" + d for d in defines]
+    objs = [out + ".%d.o" % i for i in range(len(SRCS))]
+    flags = [f for f in FLAGS if f != "-shared"]
+    procs = [subprocess.Popen([HIPCC] + flags + defs + ["-c", "-o", o, s], stdout=subprocess.PIPE,
+                              stderr=subprocess.STDOUT, text=True) for s, o in zip(SRCS, objs)]
+    errs = []
+    for p in procs:
+        msg, _ = p.communicate()
+        if p.returncode != 0:
+            errs.append(msg)
+    if errs:
+        raise RuntimeError("hipcc failed:\n" + "\n".join(errs))
+    cmd = [HIPCC] + FLAGS + ["-o", tmp] + objs + ["-lhiprtc"]
     r = subprocess.run(cmd, capture_output=True, text=True)
+    for o in objs:
+        os.remove(o)
     if r.returncode != 0:
-        raise RuntimeError("hipcc failed:\n" + r.stdout + r.stderr)
+        raise RuntimeError("hipcc link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, out)
     if verbose:
         print("built", out)

@@ -13,6 +13,7 @@ LIB_PATH = os.environ.get("PYRANDA_AMD_LIB") or os.path.join(HERE, "libpyranda_a
 
 BC = {"PERI": 0, "NONE": 1, "SYMM": 2}
 OP_D1, OP_D2, OP_D8, OP_SFILTER, OP_GFILTER = 0, 1, 2, 3, 4
+OP_ANTISYM = 0x10  # or-ed into an op: antisymmetric member of the SYMM pair
 EPI_STORE, EPI_ADD, EPI_ABS_SCALED, EPI_MAXABS_SCALED = 0, 1, 2, 3
 F64, R32 = 0, 1
 PSOLVE = {"circulant": 0, "lu": 1}

@@ -797,6 +797,11 @@ struct pyr_ctx {
   hipStream_t stream = nullptr;
   OpPlan plan[NOPS][3];
   DevPlan dplan[NOPS][3];
+  // second member of the symmetric/antisymmetric pair (compact.f90:78-84):
+  // variant -1 at each SYMM end of a direction that has one
+  bool has_anti[3] = {false, false, false};
+  OpPlan plan_a[NOPS][3];
+  DevPlan dplan_a[NOPS][3];
   void* scratch[2] = {nullptr, nullptr};  // one field-sized temp per precision
   int periodic_solver = PYR_PSOLVE_CIRCULANT;
   int r32_solve = PYR_R32_SOLVE_REFINED;
@@ -1121,8 +1126,11 @@ static int launch_kind(pyr_ctx* c, int dir, int kind, int sv, const LineArgs& a,
 }
 
 template <typename TS, int MODE>
-static int launch_op(pyr_ctx* c, int op, int dir, const TS* in, TS* out, int epi, double ec) {
-  const OpPlan& pl = c->plan[op][dir];
+static int launch_op(pyr_ctx* c, int op, int dir, const TS* in, TS* out, int epi, double ec, bool anti) {
+  // anti: the antisymmetric member where the direction has a SYMM end, as
+  // d1x/d8x(v, dv, bc=-1) select it (compact_operators.f90:300-303)
+  const bool ua = anti && c->has_anti[dir];
+  const OpPlan& pl = ua ? c->plan_a[op][dir] : c->plan[op][dir];
   if (pl.null_op) {
     const long long n = (long long)c->npts();
     hipLaunchKernelGGL((k_null<TS>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n,
@@ -1130,7 +1138,8 @@ static int launch_op(pyr_ctx* c, int op, int dir, const TS* in, TS* out, int epi
     HIPCHECK(hipGetLastError());
     return 0;
   }
-  LineArgs a = make_args(c, pl, c->dplan[op][dir], dir, epi, ec, MODE == MODE_F64 ? PYR_F64 : PYR_R32);
+  LineArgs a = make_args(c, pl, ua ? c->dplan_a[op][dir] : c->dplan[op][dir], dir, epi, ec,
+                         MODE == MODE_F64 ? PYR_F64 : PYR_R32);
   if (c->out32) {
     // only the y/z kernels of FP64 fields write the rounded copy
     if (sizeof(TS) != 8 || dir == 0) return fail("an FP32 copy of the output needs an FP64 y or z sweep");
@@ -1166,17 +1175,17 @@ static int r32_mode(const pyr_ctx* c, int op, int dir) {
 }
 
 static int apply_any(pyr_ctx* c, int op, int dir, int prec, const void* in, void* out, int epi,
-                     double ec) {
+                     double ec, bool anti = false) {
   if (check_ctx(c)) return 1;
   if (op < 0 || op >= NOPS || dir < 0 || dir > 2) return fail("bad operator or direction");
   if (!in || !out) return fail("null field pointer");
   if (in == out) return fail("input and output must not alias");
   if (prec == PYR_F64)
-    return launch_op<double, MODE_F64>(c, op, dir, (const double*)in, (double*)out, epi, ec);
+    return launch_op<double, MODE_F64>(c, op, dir, (const double*)in, (double*)out, epi, ec, anti);
   if (prec == PYR_R32) {
     if (r32_mode(c, op, dir) == PYR_R32_SOLVE_FP64)
-      return launch_op<float, MODE_S32>(c, op, dir, (const float*)in, (float*)out, epi, ec);
-    return launch_op<float, MODE_R32>(c, op, dir, (const float*)in, (float*)out, epi, ec);
+      return launch_op<float, MODE_S32>(c, op, dir, (const float*)in, (float*)out, epi, ec, anti);
+    return launch_op<float, MODE_R32>(c, op, dir, (const float*)in, (float*)out, epi, ec, anti);
   }
   return fail("bad precision");
 }
@@ -1206,11 +1215,14 @@ static int ring(pyr_ctx* c, int prec, const void* in, void* out) {
   return 0;
 }
 
+// sum over directions of op_d(f_d); anti: each f_d is the normal component of
+// direction d and takes the antisymmetric member at SYMM ends (divV,
+// operators.f90:38-53)
 static int sum3(pyr_ctx* c, int op, int prec, const void* f0, const void* f1, const void* f2,
-                void* out) {
+                void* out, bool anti = false) {
   const void* f[3] = {f0, f1, f2};
   for (int d = 0; d < 3; ++d)
-    if (apply_any(c, op, d, prec, f[d], out, d == 0 ? EPI_STORE : EPI_ADD, 0.0)) return 1;
+    if (apply_any(c, op, d, prec, f[d], out, d == 0 ? EPI_STORE : EPI_ADD, 0.0, anti)) return 1;
   return 0;
 }
 
@@ -1256,6 +1268,10 @@ static int setup_common(pyr_ctx** out, int nx, int ny, int nz, double x1, double
       const int v2 = c->bc[2 * d + 1] == PYR_BC_SYMM ? 1 : 0;
       for (int op = 0; op < NOPS; ++op)
         c->plan[op][d] = build_plan(op, c->n[d], periodic, v1, v2, c->dpatch[d]);
+      c->has_anti[d] = !periodic && (v1 || v2) && c->n[d] >= 4;
+      if (c->has_anti[d])
+        for (int op = 0; op < NOPS; ++op)
+          c->plan_a[op][d] = build_plan(op, c->n[d], periodic, -v1, -v2, c->dpatch[d]);
     }
   } catch (const std::exception& e) {
     return fail(std::string("setup: ") + e.what());
@@ -1267,8 +1283,10 @@ static int setup_common(pyr_ctx** out, int nx, int ny, int nz, double x1, double
         prop.multiProcessorCount > 0)
       c->ncu = prop.multiProcessorCount;
     for (int op = 0; op < NOPS; ++op)
-      for (int d = 0; d < 3; ++d)
+      for (int d = 0; d < 3; ++d) {
         if (upload_plan(c->plan[op][d], c->dplan[op][d])) return 1;
+        if (c->has_anti[d] && upload_plan(c->plan_a[op][d], c->dplan_a[op][d])) return 1;
+      }
     c->device = true;
   }
   *out = c.release();
@@ -1475,13 +1493,20 @@ struct GrpPlan {
   std::vector<GrpLaunch> launches;
   std::vector<GrpNull> nulls;
 };
+// A plan depends on the solve arithmetic (carry types and LDS sizes) and on
+// the members' launch knobs at build time; all of them are part of the key so
+// that a change of tuning builds a new plan instead of reusing a stale one.
 struct GrpKey {
   int op, dir, prec, epi, power, aligned;
+  int mode, psolve, xlds;
   double ec;
   bool operator<(const GrpKey& o) const {
     if (op != o.op) return op < o.op;
     if (dir != o.dir) return dir < o.dir;
     if (prec != o.prec) return prec < o.prec;
+    if (mode != o.mode) return mode < o.mode;
+    if (psolve != o.psolve) return psolve < o.psolve;
+    if (xlds != o.xlds) return xlds < o.xlds;
     if (epi != o.epi) return epi < o.epi;
     if (power != o.power) return power < o.power;
     if (aligned != o.aligned) return aligned < o.aligned;
@@ -1661,7 +1686,8 @@ int pyr_destroy(pyr_ctx* c) {
   if (c->device) {
     for (int op = 0; op < NOPS; ++op)
       for (int d = 0; d < 3; ++d)
-        if (c->dplan[op][d].mem) (void)hipFree(c->dplan[op][d].mem);
+        for (DevPlan* dp : {&c->dplan[op][d], &c->dplan_a[op][d]})
+          if (dp->mem) (void)hipFree(dp->mem);
     for (int i = 0; i < 2; ++i)
       if (c->scratch[i]) (void)hipFree(c->scratch[i]);
   }
@@ -1735,7 +1761,8 @@ int pyr_mesh_scalars(pyr_ctx* c, double* dpatch3, double* dmesh3, double* gridle
 int pyr_apply(pyr_ctx* c, int op, int dir, int prec, const void* in, void* out, int epi,
               double epi_scale) {
   if (epi < PYR_EPI_STORE || epi > PYR_EPI_MAXABS_SCALED) return fail("bad epilogue");
-  return apply_any(c, op, dir, prec, in, out, epi, epi_scale);
+  const bool anti = (op & PYR_OP_ANTISYM) != 0;
+  return apply_any(c, op & ~PYR_OP_ANTISYM, dir, prec, in, out, epi, epi_scale, anti);
 }
 
 #define PYR_DIR_OP(name, op, dir)                                                     \
@@ -1768,7 +1795,10 @@ int pyr_pringv_f64(pyr_ctx* c, const double* vx, const double* vy, const double*
   bool first = true;
   for (int d = 0; d < 3; ++d)
     for (int comp = 0; comp < 3; ++comp) {
-      if (apply_any(c, OP_D8, d, PYR_F64, f[comp], o, first ? EPI_ABS : EPI_MAXABS, c ? c->dmesh[d] : 0.0))
+      // the normal component takes the antisymmetric member at SYMM ends
+      // (ringx(f,..,isymX) etc., operators.f90:661-671)
+      if (apply_any(c, OP_D8, d, PYR_F64, f[comp], o, first ? EPI_ABS : EPI_MAXABS, c ? c->dmesh[d] : 0.0,
+                    comp == d))
         return 1;
       first = false;
     }
@@ -1820,10 +1850,10 @@ int pyr_grads_r32(pyr_ctx* c, const float* v, float* a, float* b, float* e) {
   return apply_any(c, OP_D1, 2, PYR_R32, v, e, EPI_STORE, 0.0);
 }
 int pyr_divergence_f64(pyr_ctx* c, const double* fx, const double* fy, const double* fz, double* o) {
-  return sum3(c, OP_D1, PYR_F64, fx, fy, fz, o);
+  return sum3(c, OP_D1, PYR_F64, fx, fy, fz, o, true);
 }
 int pyr_divergence_r32(pyr_ctx* c, const float* fx, const float* fy, const float* fz, float* o) {
-  return sum3(c, OP_D1, PYR_R32, fx, fy, fz, o);
+  return sum3(c, OP_D1, PYR_R32, fx, fy, fz, o, true);
 }
 
 static int rk_update(hipStream_t stream, int prec, long long n, const void* F, double* PHI, const void* Uin,
@@ -1905,11 +1935,22 @@ int pyr_group_apply(pyr_group* g, int op, int dir, int prec, const void* in, voi
   key.power = mesh_power;
   key.aligned = ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 16 == 0);
   key.ec = ec;
-  if (prec == PYR_F64) return grp_apply<double, MODE_F64>(g, key, (const double*)in, (double*)out);
+  key.psolve = g->mem[0]->periodic_solver;
+  key.xlds = g->mem[0]->x_lds_kb;
+  for (pyr_ctx* c : g->mem)
+    if (c->periodic_solver != key.psolve || c->x_lds_kb != key.xlds)
+      return fail("group members differ in their periodic solver or x-line LDS budget");
+  if (prec == PYR_F64) {
+    key.mode = MODE_F64;
+    return grp_apply<double, MODE_F64>(g, key, (const double*)in, (double*)out);
+  }
   if (prec == PYR_R32) {
-    const int r = g->mem[0]->r32_solve;
+    // the solve arithmetic of this operator and direction (per-operator
+    // tuning included), the same for every member
+    const int r = r32_mode(g->mem[0], op, dir);
     for (pyr_ctx* c : g->mem)
-      if (c->r32_solve != r) return fail("group members differ in their FP32 solve mode");
+      if (r32_mode(c, op, dir) != r) return fail("group members differ in their FP32 solve mode");
+    key.mode = r == PYR_R32_SOLVE_FP64 ? MODE_S32 : MODE_R32;
     if (r == PYR_R32_SOLVE_FP64) return grp_apply<float, MODE_S32>(g, key, (const float*)in, (float*)out);
     return grp_apply<float, MODE_R32>(g, key, (const float*)in, (float*)out);
   }

@@ -37,9 +37,9 @@ GROUP_SIBLINGS = os.environ.get("PYRANDA_AMD_FUSE_GROUP", "1") != "0"
 
 PRELUDE = r"""
 struct PyrPwArgs { long long n; void* p[24]; double s[16]; };
-__device__ __forceinline__ double pyr_min(double a, double b) { return a < b ? a : b; }
-__device__ __forceinline__ double pyr_max(double a, double b) { return a > b ? a : b; }
-__device__ __forceinline__ double pyr_sign(double a) { return (double)((a > 0.0) - (a < 0.0)); }
+__device__ __forceinline__ double pyr_min(double a, double b) { return (a < b || a != a) ? a : b; }
+__device__ __forceinline__ double pyr_max(double a, double b) { return (a > b || a != a) ? a : b; }
+__device__ __forceinline__ double pyr_sign(double a) { return a != a ? a : (double)((a > 0.0) - (a < 0.0)); }
 """
 
 

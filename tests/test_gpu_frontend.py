@@ -3,9 +3,10 @@
 The reference's own regression scripts, restated through pyranda_amd's
 pyrandaSim (EOM/setIC/rk4/eval), against the values the reference holds:
 unit-test-3D (tests/cases/testUnit.py:3), advec-1d-* (test1DAdvection.py),
-integration-* (testIntegration.py) and taylorgreen (testTaylorGreen.py:3),
-plus the 1-D advection + fbar workload (100 points, 100 RK4 steps) against
-the oracle driven by the same RK4 on the CPU.
+integration-* (testIntegration.py), heat1D-analytic-* (testHeat1D.py) and
+taylorgreen (testTaylorGreen.py:3), plus the 1-D advection + fbar workload
+(100 points, 100 RK4 steps) against the oracle driven by the same RK4 on the
+CPU (test_advection_fbar_100_vs_oracle).
 """
 import json
 import math
@@ -142,6 +143,74 @@ def test_rk4_integration(pr, n):
     want = _golden("integration-%d" % n)
     _say("integration", n, err, want)
     assert abs(err - want) / want < 1e-4
+
+
+def _host(t):
+    return t.permute(2, 1, 0).contiguous().cpu().numpy().transpose(2, 1, 0)
+
+
+def test_advection_fbar_100_vs_oracle(pr):
+    """The small 1-D workload: xdom=(0,0.99,100,periodic), ddt(phi)=-ddx(phi)
+    with the ALG line phi=fbar(phi) after every stage, dt=1e-3, `while
+    t < 0.1` (examples/tutorials/Advection1D.py:15-30 plus the filter), against
+    the oracle driven by the same RK4 (oracle/line1d_cpu.py), FP64 to 1e-12."""
+    from oracle.line1d_cpu import advection_fbar_100
+    ss = pr.pyrandaSim("advection", "xdom = (0.0 , 0.99 , 100 , periodic=True)", silent=True)
+    ss.EOM("""
+    ddt(:phi:)  =  - ddx(:phi:)
+    :phi:       =  fbar( :phi: )
+    """)
+    ss.setIC(":phi: = 1.0 + 0.1 * exp( -(abs(meshx-.5)/.1 )**2 )")
+    dt, time, steps = 1e-3, 0.0, 0
+    while time < .1:
+        time = ss.rk4(time, dt)
+        steps += 1
+    x, phi0, ref, nref = advection_fbar_100()
+    got = _host(ss.var("phi").data)
+    err = np.abs(got - ref).max() / np.abs(ref).max()
+    moved = np.abs(ref - phi0).max() / np.abs(phi0).max()
+    _say("advection+fbar 100 pts", steps, "steps, rel err", err, "moved", moved)
+    assert steps == nref == 100
+    assert np.abs(_host(ss.mesh.coords[0].data)[:, 0, 0] - x).max() <= 1e-15
+    assert moved > 1e-3  # the pulse moved (the check is not of a static field)
+    assert err <= 1e-12
+
+
+@pytest.mark.parametrize("n", [16, 32, 64, 128, 256])
+def test_heat1d_analytic(pr, n):
+    """examples/heat1D.py N 1: bounded x, lap with bc.const at both ends
+    after every stage; the reference-held heat1D-analytic-N is 0.0
+    (tests/cases/testHeat1D.py:3-7; zero baselines compare absolutely to
+    1e-4, tests/testObj.py:45-50)."""
+    L = np.pi * 2.0
+    Lp = L * (n - 1.0) / n
+    mesh_options = {"coordsys": 0, "periodic": np.array([False, True, True]), "dim": 1,
+                    "x1": [0.0, 0.0, 0.0], "xn": [Lp, Lp, Lp], "nn": [n, 1, 1]}
+    ss = pr.pyrandaSim("heat_equation", mesh_options, silent=True)
+    ss.addPackage(pr.pyrandaBC(ss))
+    ss.EOM("""
+    ddt(:phi:)  =  :c: * lap(:phi:)
+    bc.const(['phi'],['x1'],2.0)
+    bc.const(['phi'],['xn'],1.0)
+    """)
+    ss.setIC("""
+    xnn = meshx[-1,0,0]
+    :phi: = 1.0 + 1.0*(xnn - meshx)/xnn
+    :c:   = 1.0
+    """)
+    x = _host(ss.mesh.coords[0].data)
+    dt_max = L / ss.mesh.nn[0] * .005
+    tt = dt_max * 500
+    dt, time = dt_max, 0.0
+    while tt > time:
+        time = ss.rk4(time, dt)
+        dt = min(dt_max, (tt - time))
+    xnn = x[-1, 0, 0]
+    anl = 1.0 + 1.0 * (xnn - x) / xnn
+    err = float(np.sum(np.abs(anl - _host(ss.var("phi").data))[:, 0, 0]))
+    _say("heat1D", n, err)
+    assert err < 1e-4
+    assert err < 1e-10  # FP64 keeps the linear steady state to rounding
 
 
 from pyranda_amd.workloads import TG_EOM, TG_IC  # noqa: E402

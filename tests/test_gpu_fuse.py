@@ -135,3 +135,39 @@ def test_sibling_group_equals_separate_kernels(dev, mixed):
         if mixed:
             assert any(x.dtype == torch.float64 for x in args_b) == ("tauxy" in src), src
     _say("sibling groups equal", "mixed" if mixed else "fp32")
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_nan_propagates_like_numpy(dev, dtype):
+    """minimum/maximum return NaN when either argument is NaN and sign(NaN)
+    is NaN, as numpy does (a NaN in an EOM must reach the NaN checks)."""
+    import ast
+    import numpy as np
+    from pyranda_amd import fuse
+    shape = (64, 3, 2)
+    n = shape[0] * shape[1] * shape[2]
+    g = torch.Generator().manual_seed(5)
+    a = torch.rand(n, generator=g, dtype=torch.float64) - 0.5
+    b = torch.rand(n, generator=g, dtype=torch.float64) - 0.5
+    a[::3] = float("nan")
+    b[1::4] = float("nan")
+    a[7] = b[7] = float("nan")
+    a[8] = b[8] = 0.25
+    an, bn = a.numpy().astype(np.float32 if dtype == torch.float32 else np.float64), b.numpy()
+    bn = bn.astype(an.dtype)
+
+    def field(x):
+        return torch.from_numpy(x.copy()).reshape(shape[2], shape[1], shape[0]).to(dev).permute(2, 1, 0)
+
+    for src, ref in [("numpy.minimum(a, b)", np.minimum(an, bn)), ("numpy.maximum(a, b)", np.maximum(an, bn)),
+                     ("numpy.minimum(b, a)", np.minimum(bn, an)), ("numpy.maximum(b, a)", np.maximum(bn, an)),
+                     ("numpy.sign(a)", np.sign(an)), ("numpy.sign(b)", np.sign(bn))]:
+        node = ast.parse(src, mode="eval").body
+        sub = fuse.FusedSubtree(node, "nan")
+        names = [ast.unparse(leaf) for leaf in sub.leaves]
+        got = sub(shape, *[field(an) if nm == "a" else field(bn) for nm in names])
+        got = got.permute(2, 1, 0).reshape(-1).cpu().numpy()
+        _say(src, int(np.isnan(ref).sum()), "NaNs")
+        np.testing.assert_array_equal(np.isnan(got), np.isnan(ref), err_msg=src)
+        ok = ~np.isnan(ref)
+        np.testing.assert_array_equal(got[ok], ref[ok], err_msg=src)

@@ -103,7 +103,33 @@ def test_directional_ops_f64_and_r32(dev, oracle_mod, nn, kinds, solver):
     ctx.close()
 
 
-@pytest.mark.parametrize("nn,kinds", CASES[:3] + CASES[4:5])
+@pytest.mark.parametrize("nn,kinds", [((24, 9, 12), ("SYMM", "SYMM", "NONE")),
+                                      ((40, 33, 64), ("NONE", "SYMM", "SYMM"))])
+def test_antisymmetric_member(dev, oracle_mod, nn, kinds):
+    """The antisymmetric member of the SYMM pair (variant -1 at SYMM ends,
+    compact.f90:78-84) through pyr_apply(op | PYR_OP_ANTISYM): every operator
+    and direction against the oracle's member (directions without a SYMM end
+    keep the default operator)."""
+    from pyranda_amd import _lib
+    _say("antisymmetric member", nn, kinds)
+    ctx, P, v = _setup(nn, kinds, oracle_mod)
+    vd = to_dev(v, dev)
+    v32 = v.astype(np.float32).astype(np.float64)
+    vd32 = to_dev(v32, dev, torch.float32)
+    for op in range(5):
+        for d in range(3):
+            ref = P.apply_sym(op, d, -1, v)
+            got = to_host(ctx.apply(op | _lib.OP_ANTISYM, d, vd))
+            assert _rel(got, ref) <= 1e-12, ("f64", op, d, _rel(got, ref))
+            if kinds[d] == "SYMM":
+                assert _rel(ref, P.apply(op, d, v)) > 1e-6, ("members differ", op, d)
+            ref32 = P.apply_sym(op, d, -1, v32)
+            got32 = to_host(ctx.apply(op | _lib.OP_ANTISYM, d, vd32))
+            assert _rel(got32, ref32) <= 1e-6, ("r32", op, d, _rel(got32, ref32))
+    ctx.close()
+
+
+@pytest.mark.parametrize("nn,kinds", CASES[:3] + CASES[4:5] + [((40, 33, 64), ("NONE", "SYMM", "SYMM"))])
 def test_composites(dev, oracle_mod, nn, kinds):
     _say("composites", nn, kinds)
     ctx, P, v = _setup(nn, kinds, oracle_mod)

@@ -312,3 +312,202 @@ def test_kh_2d_curve_golden(oracle_mod):
     # restatement holds the curve to rounding after 249 steps of shear growth
     err = np.abs(rho - base[1]).max() / np.abs(base[1]).max()
     assert err <= 1e-11, err
+
+
+# ---------------------------------------------------------------------------
+# Composites: closed forms with a distinct mode per component and direction
+# ---------------------------------------------------------------------------
+def _box(ns=(24, 20, 16), lens=(2 * np.pi, 4 * np.pi, np.pi)):
+    """Periodic box with a different spacing per direction (a swapped metric
+    or direction changes the result)."""
+    dom = [(0.0, L * (n - 1) / n, n) for n, L in zip(ns, lens)]
+    (X, Y, Z), dd = mesh(dom)
+    ks = [2 * np.pi / L for L in lens]  # wavenumber of one period per axis
+    return (X, Y, Z), dd, ks
+
+
+def test_laplacian_closed_form(oracle_mod):
+    """lap = d2x + d2y + d2z (operators.f90:522-526) on one mode per axis:
+    the sum of the published d2 symbols, each over its own h^2."""
+    po = oracle_mod
+    (X, Y, Z), dd, ks = _box()
+    P = po.Oracle(X.shape, dd, (True, True, True))
+    m = (3, 2, 5)            # modes along x, y, z
+    A = (1.3, -0.7, 0.45)
+    ph = (0.2, 1.1, -0.6)
+    C = (X, Y, Z)
+    f = sum(A[d] * np.cos(m[d] * ks[d] * C[d] + ph[d]) for d in range(3))
+    want = sum(A[d] * _symbols(m[d] * ks[d] * dd[d])[1] * np.cos(m[d] * ks[d] * C[d] + ph[d]) / dd[d] ** 2
+               for d in range(3))
+    got = P.laplacian(f)
+    assert np.abs(got - want).max() / np.abs(want).max() < 1e-12
+
+
+def test_divergence_closed_form(oracle_mod):
+    """div(fx, fy, fz) = ddx(fx) + ddy(fy) + ddz(fz) (operators.f90:38-53):
+    each component also carries modes along the other axes, which only a
+    swapped component or direction would differentiate."""
+    po = oracle_mod
+    (X, Y, Z), dd, ks = _box()
+    P = po.Oracle(X.shape, dd, (True, True, True))
+    C = (X, Y, Z)
+    rng = np.random.default_rng(21)
+    A = rng.uniform(0.5, 1.5, (3, 3))
+    m = rng.integers(1, 6, (3, 3))
+    ph = rng.uniform(0, 2 * np.pi, (3, 3))
+    f = [sum(A[c, d] * np.cos(m[c, d] * ks[d] * C[d] + ph[c, d]) for d in range(3)) for c in range(3)]
+    want = sum(-A[c, c] * _symbols(m[c, c] * ks[c] * dd[c])[0] * np.sin(m[c, c] * ks[c] * C[c] + ph[c, c]) / dd[c]
+               for c in range(3))
+    got = P.divergence(*f)
+    assert np.abs(got - want).max() / np.abs(want).max() < 1e-12
+
+
+def test_ringv_closed_form(oracle_mod):
+    """ringV (pRingV, L=1; operators.f90:645-699): the largest over
+    directions d of max_component |d8_d f_c| * h_d, from the published d8
+    symbol; distinct amplitudes per component and direction (an L=2 metric,
+    a swapped component or a dropped direction changes the value)."""
+    po = oracle_mod
+    (X, Y, Z), dd, ks = _box()
+    P = po.Oracle(X.shape, dd, (True, True, True))
+    C = (X, Y, Z)
+    rng = np.random.default_rng(22)
+    A = rng.uniform(0.2, 2.0, (3, 3))
+    m = rng.integers(1, 7, (3, 3))
+    ph = rng.uniform(0, 2 * np.pi, (3, 3))
+    f = [sum(A[c, d] * np.cos(m[c, d] * ks[d] * C[d] + ph[c, d]) for d in range(3)) for c in range(3)]
+    per_dir = []
+    for d in range(3):
+        comps = [np.abs(A[c, d] * _symbols(m[c, d] * ks[d] * dd[d])[2] * np.cos(m[c, d] * ks[d] * C[d] + ph[c, d]))
+                 for c in range(3)]
+        per_dir.append(np.maximum(np.maximum(comps[0], comps[1]), comps[2]) * dd[d])
+    want = np.maximum(np.maximum(per_dir[0], per_dir[1]), per_dir[2])
+    got = P.ringv(*f)
+    assert np.abs(got - want).max() / np.abs(want).max() < 1e-11
+
+
+# ---------------------------------------------------------------------------
+# SYMM ends: both members of the pair are the periodic operator on the line
+# mirrored about the boundary face (data even for +1, odd for -1)
+# ---------------------------------------------------------------------------
+def _mirror(f, isym, axis):
+    return np.concatenate([f, isym * np.flip(f, axis=axis)], axis=axis)
+
+
+@pytest.mark.parametrize("op", range(5))
+@pytest.mark.parametrize("isym", [1, -1])
+@pytest.mark.parametrize("direction", [0, 1, 2])
+def test_symm_pair_is_mirrored_periodic(oracle_mod, op, isym, direction):
+    """compact.f90:78-84 builds variant +1 (iop=1) and -1 (iop=2) at a SYMM
+    end; folding the stencils about the face (stencils.f90:2390-2453) makes
+    each the periodic operator on the mirrored line of twice the length."""
+    po = oracle_mod
+    n = 23
+    shape = [3, 4, 5]
+    shape[direction] = n
+    dd = [0.3, 0.5, 0.7]
+    rng = np.random.default_rng(100 * op + 10 * direction + (isym > 0))
+    f = np.asfortranarray(rng.standard_normal(shape) + 2.0)
+    sym = [[False, False], [False, False], [False, False]]
+    sym[direction] = [True, True]
+    P = po.Oracle(tuple(shape), dd, (False, False, False), symmetric=sym)
+    got = P.apply_sym(op, direction, isym, f)
+    big = list(shape)
+    big[direction] = 2 * n
+    per = [False, False, False]
+    per[direction] = True
+    Pp = po.Oracle(tuple(big), dd, tuple(per))
+    ref = Pp.apply(op, direction, np.asfortranarray(_mirror(f, isym, direction)))
+    ref = np.take(ref, range(n), axis=direction)
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-12
+    if isym == 1:
+        assert np.array_equal(got, P.apply(op, direction, f))
+
+
+def test_divergence_ringv_symm_components(oracle_mod):
+    """With SYMM ends in every direction, div and ringV differentiate the
+    normal component of each direction with the antisymmetric member
+    (ddx(fx,..,isymX), ringx(f,..,isymX); operators.f90:38-53,661-671): the
+    same as the periodic composites on the box mirrored about every face,
+    normal components odd and the others even."""
+    po = oracle_mod
+    shape = (12, 10, 9)
+    dd = (0.3, 0.5, 0.7)
+    rng = np.random.default_rng(31)
+    f = [np.asfortranarray(rng.standard_normal(shape) + 1.0) for _ in range(3)]
+    P = po.Oracle(shape, dd, (False, False, False), symmetric=[[True, True]] * 3)
+    big = tuple(2 * n for n in shape)
+    Pp = po.Oracle(big, dd, (True, True, True))
+    fm = []
+    for c in range(3):
+        g = f[c]
+        for d in range(3):
+            g = _mirror(g, -1 if c == d else 1, d)
+        fm.append(np.asfortranarray(g))
+    crop = (slice(0, shape[0]), slice(0, shape[1]), slice(0, shape[2]))
+    for name in ("divergence", "ringv"):
+        got = getattr(P, name)(*f)
+        ref = getattr(Pp, name)(*fm)[crop]
+        assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-12, name
+    # gradients of a scalar keep the symmetric member (operators.f90:184-212)
+    g = P.grads(f[0])
+    gm = Pp.grads(np.asfortranarray(_mirror(_mirror(_mirror(f[0], 1, 0), 1, 1), 1, 2)))
+    for a, b in zip(g, gm):
+        assert np.abs(a - b[crop]).max() / np.abs(b[crop]).max() < 1e-12
+
+
+# ---------------------------------------------------------------------------
+# Reference-held values of the 1-D cases and of Taylor-Green
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [50, 100, 200, 300, 400])
+def test_advection_golden_through_oracle(oracle_mod, n):
+    """advec-1d-N (tests/cases/test1DAdvection.py:3-7): periodic ddx and the
+    RK4 driver of oracle/line1d_cpu.py, which also drives the filtered 1-D
+    workload the GPU front end is checked against."""
+    from oracle.line1d_cpu import advection_error
+    want = _golden()["advec-1d-%d" % n]["value"]
+    got = advection_error(n)
+    assert abs(got - want) / want < 1e-6, (got, want)
+
+
+def test_advection_planted_dt_mistake_is_caught(oracle_mod):
+    """A driver that skips the last (shortened) step misses the golden."""
+    from oracle import line1d_cpu as L
+    orig = L.Advection1DCPU.run
+
+    def short(self, phi, dt, until, dt_rule=None):
+        return orig(self, phi, dt, until - 0.5 * dt, dt_rule)
+
+    L.Advection1DCPU.run = short
+    try:
+        got = L.advection_error(100)
+    finally:
+        L.Advection1DCPU.run = orig
+    want = _golden()["advec-1d-100"]["value"]
+    assert abs(got - want) / want > 1e-3
+
+
+@pytest.mark.parametrize("n", [16, 32, 64, 128, 256])
+def test_heat1d_golden(oracle_mod, n):
+    """heat1D-analytic-N = 0.0 (tests/cases/testHeat1D.py:3-7; the harness
+    compares a zero baseline absolutely, tests/testObj.py:45-50, to 1e-4):
+    bounded d2 with bc.const keeps the linear steady state."""
+    from oracle.line1d_cpu import heat_error
+    assert heat_error(n) < 1e-12
+
+
+def test_tg_cpu_golden(oracle_mod):
+    """taylorgreen = 1.00106718415 (tests/cases/testTaylorGreen.py:3):
+    oracle/tg_cpu.py's hand-written Taylor-Green step at 32^3 with the loop of
+    examples/TaylorGreen.py:128-147 (CFL 0.5, t < 0.1)."""
+    from oracle.tg_cpu import TaylorGreenCPU
+    tg = TaylorGreenCPU(32, p0=100.0, threads=4)
+    enst0 = tg.V["enst"].sum()
+    dt = tg.V["dt"] * 0.5
+    time = 0.0
+    while time < 0.1:
+        time = tg.rk4(time, dt)
+        dt = tg.V["dt"] * 0.5
+    got = tg.V["enst"].sum() / enst0
+    want = _golden()["taylorgreen"]["value"]
+    assert abs(got - want) / want < 1e-10, (got, want)
