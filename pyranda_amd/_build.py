@@ -5,6 +5,9 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 SRCS = [os.path.join(HERE, "csrc", "pyr_core.hip"), os.path.join(HERE, "csrc", "pyr_mlp.hip"),
         os.path.join(HERE, "csrc", "pyr_fuse.cpp")]
+# compile units: pyr_core.hip builds as four parts (host code and C-ABI; the
+# line kernels of each solve mode), compiled in parallel
+UNITS = [(SRCS[0], ["PYR_PART=%d" % k]) for k in range(4)] + [(SRCS[1], []), (SRCS[2], [])]
 OUT = os.path.join(HERE, "libpyranda_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -21,10 +24,11 @@ def build(verbose=False, out=OUT, defines=()):
     to objects in parallel, then link."""
     tmp = out + ".tmp"
     defs = ["-D" + d for d in defines]
-    objs = [out + ".%d.o" % i for i in range(len(SRCS))]
+    objs = [out + ".%d.o" % i for i in range(len(UNITS))]
     flags = [f for f in FLAGS if f != "-shared"]
-    procs = [subprocess.Popen([HIPCC] + flags + defs + ["-c", "-o", o, s], stdout=subprocess.PIPE,
-                              stderr=subprocess.STDOUT, text=True) for s, o in zip(SRCS, objs)]
+    procs = [subprocess.Popen([HIPCC] + flags + defs + ["-D" + d for d in udefs] + ["-c", "-o", o, src],
+                              stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+             for (src, udefs), o in zip(UNITS, objs)]
     errs = []
     for p in procs:
         msg, _ = p.communicate()

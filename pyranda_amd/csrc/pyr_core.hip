@@ -31,15 +31,30 @@ using namespace pyr;
 // ---------------------------------------------------------------------------
 // error reporting
 // ---------------------------------------------------------------------------
+// The file compiles as several translation units that build in parallel:
+// PYR_PART 0 holds the host code and the C-ABI, parts 1-3 the line kernels
+// of one solve mode each (FP64, FP32-refined, FP32 fields with FP64 solves),
+// through explicit instantiations of the launch functions.
+#ifndef PYR_PART
+#define PYR_PART 0
+#endif
+#if PYR_PART == 0
 static thread_local std::string g_err;
 static int fail(const std::string& m) {
   g_err = m;
   return 1;
 }
 namespace pyr {
-// errors of the other translation units (pyr_mlp.hip) go to pyr_last_error too
+// errors of the other translation units (pyr_mlp.hip, the kernel parts) go
+// to pyr_last_error too
 int report_error(const std::string& m) { return fail(m); }
 }  // namespace pyr
+#else
+namespace pyr {
+int report_error(const std::string& m);
+}  // namespace pyr
+static int fail(const std::string& m) { return pyr::report_error(m); }
+#endif
 #define HIPCHECK(x)                                                              \
   do {                                                                           \
     hipError_t e_ = (x);                                                         \
@@ -1099,7 +1114,7 @@ static int launch_typed(pyr_ctx* c, int dir, LineArgs a, const TS* in, TS* out) 
 }
 
 template <typename TS, int MODE, bool FULL>
-static int launch_kind(pyr_ctx* c, int dir, int kind, int sv, const LineArgs& a, const TS* in, TS* out) {
+int launch_kind(pyr_ctx* c, int dir, int kind, int sv, const LineArgs& a, const TS* in, TS* out) {
   if constexpr (FULL && PYR_FULL_CHUNKS) {
     switch (kind) {
       case K_D1: return launch_typed<TS, K_D1, SOLVE_CIRC, MODE, true>(c, dir, a, in, out);
@@ -1124,6 +1139,29 @@ static int launch_kind(pyr_ctx* c, int dir, int kind, int sv, const LineArgs& a,
   }
   return fail("full-chunk kernels are not built");
 }
+
+#define PYR_LAUNCH_KIND(TS, MODE, FULL) \
+  template int launch_kind<TS, MODE, FULL>(pyr_ctx*, int, int, int, const LineArgs&, const TS*, TS*);
+#if PYR_PART == 1
+PYR_LAUNCH_KIND(double, MODE_F64, true)
+PYR_LAUNCH_KIND(double, MODE_F64, false)
+#elif PYR_PART == 2
+PYR_LAUNCH_KIND(float, MODE_R32, true)
+PYR_LAUNCH_KIND(float, MODE_R32, false)
+#elif PYR_PART == 3
+PYR_LAUNCH_KIND(float, MODE_S32, true)
+PYR_LAUNCH_KIND(float, MODE_S32, false)
+#else
+extern PYR_LAUNCH_KIND(double, MODE_F64, true)
+extern PYR_LAUNCH_KIND(double, MODE_F64, false)
+extern PYR_LAUNCH_KIND(float, MODE_R32, true)
+extern PYR_LAUNCH_KIND(float, MODE_R32, false)
+extern PYR_LAUNCH_KIND(float, MODE_S32, true)
+extern PYR_LAUNCH_KIND(float, MODE_S32, false)
+#endif
+#undef PYR_LAUNCH_KIND
+
+#if PYR_PART == 0
 
 template <typename TS, int MODE>
 static int launch_op(pyr_ctx* c, int op, int dir, const TS* in, TS* out, int epi, double ec, bool anti) {
@@ -1462,6 +1500,8 @@ static int emulate(pyr_ctx* c, int op, int dir, const TS* in, TS* out) {
   return 0;
 }
 
+#endif  // PYR_PART == 0
+
 // ---------------------------------------------------------------------------
 // groups: one operator over the fields of several simulations
 // ---------------------------------------------------------------------------
@@ -1525,6 +1565,7 @@ struct pyr_group {
   }
 };
 
+#if PYR_PART == 0
 template <typename TS, int MODE>
 static int grp_build(pyr_group* g, const GrpKey& key, GrpPlan& gp) {
   using CS = typename SolveT<MODE>::type;
@@ -1603,6 +1644,8 @@ static int grp_build(pyr_group* g, const GrpKey& key, GrpPlan& gp) {
   return 0;
 }
 
+#endif  // PYR_PART == 0
+
 template <typename TS, int KIND, int SOLVE, int MODE>
 static int grp_launch_typed(pyr_group* g, const GrpLaunch& L, const TS* in, TS* out) {
   if (L.nblocks <= 0) return 0;
@@ -1632,6 +1675,32 @@ static int grp_launch_typed(pyr_group* g, const GrpLaunch& L, const TS* in, TS* 
 }
 
 template <typename TS, int MODE>
+int grp_launch_mode(pyr_group* g, const GrpLaunch& L, const TS* in, TS* out) {
+  switch (L.kind) {
+    case K_D1: return grp_launch_typed<TS, K_D1, SOLVE_CIRC, MODE>(g, L, in, out);
+    case K_R3: return grp_launch_typed<TS, K_R3, SOLVE_CIRC, MODE>(g, L, in, out);
+    default:
+      return L.solve == SOLVE_CIRC ? grp_launch_typed<TS, K_R4, SOLVE_CIRC, MODE>(g, L, in, out)
+                                   : grp_launch_typed<TS, K_R4, SOLVE_NONE, MODE>(g, L, in, out);
+  }
+}
+
+#define PYR_GRP_MODE(TS, MODE) template int grp_launch_mode<TS, MODE>(pyr_group*, const GrpLaunch&, const TS*, TS*);
+#if PYR_PART == 1
+PYR_GRP_MODE(double, MODE_F64)
+#elif PYR_PART == 2
+PYR_GRP_MODE(float, MODE_R32)
+#elif PYR_PART == 3
+PYR_GRP_MODE(float, MODE_S32)
+#else
+extern PYR_GRP_MODE(double, MODE_F64)
+extern PYR_GRP_MODE(float, MODE_R32)
+extern PYR_GRP_MODE(float, MODE_S32)
+#endif
+#undef PYR_GRP_MODE
+
+#if PYR_PART == 0
+template <typename TS, int MODE>
 static int grp_apply(pyr_group* g, const GrpKey& key, const TS* in, TS* out) {
   auto it = g->plans.find(key);
   if (it == g->plans.end()) {
@@ -1643,17 +1712,8 @@ static int grp_apply(pyr_group* g, const GrpKey& key, const TS* in, TS* out) {
     }
     it = g->plans.emplace(key, std::move(gp)).first;
   }
-  for (const GrpLaunch& L : it->second.launches) {
-    int r;
-    switch (L.kind) {
-      case K_D1: r = grp_launch_typed<TS, K_D1, SOLVE_CIRC, MODE>(g, L, in, out); break;
-      case K_R3: r = grp_launch_typed<TS, K_R3, SOLVE_CIRC, MODE>(g, L, in, out); break;
-      default:
-        r = L.solve == SOLVE_CIRC ? grp_launch_typed<TS, K_R4, SOLVE_CIRC, MODE>(g, L, in, out)
-                                  : grp_launch_typed<TS, K_R4, SOLVE_NONE, MODE>(g, L, in, out);
-    }
-    if (r) return 1;
-  }
+  for (const GrpLaunch& L : it->second.launches)
+    if (grp_launch_mode<TS, MODE>(g, L, in, out)) return 1;
   for (const GrpNull& z : it->second.nulls) {
     const long long n = (long long)g->mem[z.member]->npts();
     const long long o = g->off[z.member];
@@ -1984,3 +2044,4 @@ int pyr_host_emulate(pyr_ctx* c, int op, int dir, int prec, const void* in, void
 }
 
 }  // extern "C"
+#endif  // PYR_PART == 0
