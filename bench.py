@@ -7,9 +7,10 @@ on the conserved variables after every stage) with Mach 0.1
 pyranda_amd.pyrandaSim with every field resident in HBM.
 
 The headline value is FP32-refined point-updates/s (FP32 fields and sweeps,
-one FP64-residual refinement per operator, FP64 RK4 state). The FP64 path is
-timed beside it on the same steps, and the FP32-refined state is compared
-with the FP64 state after the same steps (rel_max / rel_L2).
+one FP64-residual refinement in every operator, FP64 RK4 state, FP64
+storage of the offset-dominated fields). The FP64 path is timed beside it on
+the same steps, and every conserved variable of the FP32 run is compared
+with the FP64 run after the same steps (rel_max / rel_L2, bar 1e-6).
 
 Roofline: every operator launch inside the timed region is bracketed by HIP
 events on its stream; the operator kernel with the largest total time is the
@@ -52,6 +53,7 @@ MEASURED_TRAFFIC = {
     ("fbar", "z", "torch.float64+float32"): (131502 + 196608) * 1024,
 }
 OPS = ["d1", "d2", "d8", "fbar", "gbar"]
+CONSERVED = ("rho", "rhou", "rhov", "rhow", "Et")
 DIRS = "xyz"
 
 
@@ -70,8 +72,10 @@ def make_tg(n, precision, p0, solver="circulant"):
     import pyranda_amd as pr
     from pyranda_amd.workloads import TG_EOM, TG_IC
     rounded = precision == "fp32-rounded-state"
-    base = "fp32" if precision in ("fp32-rounded-state", "fp32-refined-only") else precision
-    ss = pr.pyrandaSim("TGvortex", tg_mesh(n), silent=True, precision=base, fp64_state=not rounded)
+    base = "fp32" if precision in ("fp32-rounded-state", "fp32-refined-only", "fp32-all-f32") else precision
+    wide = None if precision == "fp32-all-f32" else "auto"
+    ss = pr.pyrandaSim("TGvortex", tg_mesh(n), silent=True, precision=base, fp64_state=not rounded,
+                       wide_fields=wide)
     ss.PyMPI.ctx.set_periodic_solver(solver)
     ss.addPackage(pr.pyrandaTimestep(ss))
     ss.EOM(TG_EOM)
@@ -97,14 +101,33 @@ def rel_err(a, b):
 
 
 def cpu_baseline(n, p0, dt, threads, sample_steps=1):
+    """Point-updates/s of the oracle's FP64 Taylor-Green step (oracle/tg_cpu.py)
+    on `threads` host threads: one untimed step, then `sample_steps` timed."""
     from oracle.tg_cpu import TaylorGreenCPU
     tg = TaylorGreenCPU(n, p0=p0, threads=threads)
+    t = tg.rk4(0.0, dt)
     t0 = time.perf_counter()
-    t = 0.0
     for _ in range(sample_steps):
         t = tg.rk4(t, dt)
     el = time.perf_counter() - t0
+    tg.close()
     return n ** 3 * sample_steps / el, el
+
+
+def host_info():
+    """nproc, usable cores, OMP_NUM_THREADS and the CPU model of this host."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    usable = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable_cores": usable,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": model}
 
 
 def _symbols(theta):
@@ -404,11 +427,13 @@ def closure_bench(args, dev):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256)
-    ap.add_argument("--cpu-n", type=int, default=128,
-                    help="grid of the CPU-baseline sample (one RK4 step)")
+    ap.add_argument("--cpu-n", type=int, default=256,
+                    help="grid of the CPU-baseline sample (one timed RK4 step on every usable core)")
+    ap.add_argument("--cpu-n1", type=int, default=64,
+                    help="grid of the one-thread CPU sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--ops", action="store_true",
@@ -417,11 +442,11 @@ def main():
                     help="solver of periodic implicit lines")
     ap.add_argument("--r32-solve", default="refined", choices=["refined", "fp64"],
                     help="--ops: arithmetic of the solves on FP32 fields")
-    ap.add_argument("--precision", default="fp64,fp32,fp32-refined-only,fp32-fp64solve,fp32-rounded-state",
-                    help="comma-separated pyrandaSim precisions to time (the headline is fp32, whose "
-                         "autotuning also picks refined-FP32 or FP64 solves per operator; "
-                         "fp32-refined-only keeps the refined solves everywhere; fp32-rounded-state is "
-                         "fp32 without the FP64 state through the filter)")
+    ap.add_argument("--precision", default="fp64,fp32,fp32-fp64solve",
+                    help="comma-separated pyrandaSim precisions to time (the headline is fp32: FP32 "
+                         "sweeps plus one FP64-residual refinement in every operator; fp32-fp64solve "
+                         "solves in FP64 on the same FP32 fields; fp32-rounded-state is fp32 without "
+                         "the FP64 state through the filter; fp32-all-f32 stores every field in FP32)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (no per-operator autotuning before timing)")
     ap.add_argument("--ensemble", action="store_true",
@@ -467,6 +492,7 @@ def main():
     results = {}
     states = {}
     tuned = {}
+    wide = None
     dt = None
     precs = [p for p in args.precision.split(",") if p and not (args.no_fp64 and p == "fp64")]
     if "fp32" not in precs:
@@ -477,7 +503,9 @@ def main():
             dt = float(ss.variables["dt"].data) * 0.5  # CFL 0.5 from the initial state
         ctx = ss.PyMPI.ctx
         if not args.no_autotune:
-            rep = ss.autotune(solve_modes=precision in ("fp32", "fp32-rounded-state"))
+            # launch knobs only: they leave results unchanged bit for bit (the
+            # solve arithmetic is the precision's, not a timing decision)
+            rep = ss.autotune()
             tuned[precision] = {"_".join([OPS[key[0]], DIRS[key[1]]] + list(key[2:])):
                                 ",".join("%s=%d" % kv for kv in best)
                                 for key, (best, _) in sorted(rep.items(), key=lambda kv: str(kv[0]))}
@@ -495,7 +523,9 @@ def main():
         ctx.timing = None
         results[precision] = {"elapsed_s": el, "ms_per_step": 1e3 * el / args.steps,
                               "pups": n ** 3 * args.steps / el, "kernels": summ}
-        states[precision] = {k: ss.variables[k].data.clone() for k in ("rho", "rhou", "Et")}
+        states[precision] = {k: ss.variables[k].data.clone() for k in CONSERVED}
+        if precision == "fp32":
+            wide = sorted(k for k, v in ss._storage.items() if v == "f64")
         print("[bench] %s %.3f ms/step, %.3e point-updates/s" %
               (precision, 1e3 * el / args.steps, n ** 3 * args.steps / el), file=sys.stderr, flush=True)
         del ss
@@ -514,26 +544,34 @@ def main():
     achieved = alg_bytes / (mean_ms * 1e-3) / 1e9
     op_share = sum(tot.values()) / (head["elapsed_s"] * 1e3)
 
-    acc, acc_s = {}, {}
+    # every variant against the FP64 run after the same warmup + timed steps
+    acc = {}
     if "fp64" in states:
-        for k in states["fp32"]:
-            acc[k] = rel_err(states["fp32"][k], states["fp64"][k])
-            if "fp32-fp64solve" in states:
-                acc_s[k] = rel_err(states["fp32-fp64solve"][k], states["fp64"][k])
-    acc_r = {k: rel_err(states["fp32-rounded-state"][k], states["fp64"][k])
-             for k in states["fp32"]} if "fp64" in states and "fp32-rounded-state" in states else {}
-    acc_o = {k: rel_err(states["fp32-refined-only"][k], states["fp64"][k])
-             for k in states["fp32"]} if "fp64" in states and "fp32-refined-only" in states else {}
+        for prec in states:
+            if prec != "fp64":
+                acc[prec] = {k: dict(zip(("rel_max", "rel_l2"), rel_err(states[prec][k], states["fp64"][k])))
+                             for k in CONSERVED}
+    worst = (max(max(v["rel_max"], v["rel_l2"]) for v in acc["fp32"].values()) if "fp32" in acc else None)
 
     cpu = None
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
-        print("[bench] cpu baseline %d^3 x 2 steps, %d threads" % (args.cpu_n, threads),
+        # every core this process may use; where OMP_NUM_THREADS is set (the
+        # host's share of cores per GPU job) it bounds the count
+        info = host_info()
+        threads = int(info["usable_cores"] or 1)
+        if info["omp_num_threads"]:
+            threads = min(threads, max(1, int(info["omp_num_threads"])))
+        print("[bench] cpu baseline %d^3, %d threads; %d^3, 1 thread" % (args.cpu_n, threads, args.cpu_n1),
               file=sys.stderr, flush=True)
-        val, el = cpu_baseline(args.cpu_n, p0, dt, threads, sample_steps=2)
+        val, el = cpu_baseline(args.cpu_n, p0, dt, threads, sample_steps=1)
+        val1, el1 = cpu_baseline(args.cpu_n1, p0, dt, 1, sample_steps=1)
         cpu = {"value": val, "unit": "point-updates/s", "cores": threads, "kind": "port",
-               "sample": "two RK4 steps of the same Taylor-Green Euler+LAD EOM at %d^3 (FP64 oracle "
-                         "operators with OpenMP + numpy pointwise), %.1f s" % (args.cpu_n, el)}
+               "sample": "one RK4 step (after one untimed step) of the same Taylor-Green Euler+LAD EOM at "
+                         "%d^3 on %d threads: the oracle's FP64 operators (OpenMP over lines) and numpy "
+                         "pointwise algebra over z-slabs in %d threads, %.1f s" % (args.cpu_n, threads, threads, el),
+               "one_thread": {"value": val1, "sample": "one RK4 step at %d^3 on 1 thread, %.1f s"
+                                                       % (args.cpu_n1, el1)},
+               "host": info}
 
     line = {
         "metric": "256^3 Euler+LAD point-updates/s (FP32-refined), 3-D Taylor-Green",
@@ -546,36 +584,27 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 fields; per operator f32 sweeps with f64 residual refinement or f64 sweeps "
-                 "(autotuned); f64 RK4 state",
+        "dtype": "f32 fields (offset-dominated fields in f64); f32 sweeps plus one f64-residual "
+                 "refinement in every operator; f64 RK4 state",
         "data": "synthetic (Taylor-Green initial condition)",
         "config": {"workload": "taylor_green_euler_lad_%d^3" % n, "grid": [n, n, n], "gamma": gamma,
                    "mach": 0.1, "cfl": 0.5, "dt": dt, "rk_stages": 5, "lad": True,
-                   "periodic_solver": args.solver, "autotune": tuned.get("fp32")},
+                   "periodic_solver": args.solver, "autotune": tuned.get("fp32"),
+                   "fp64_fields": wide},
         "fp64": ({"value": results["fp64"]["pups"], "ms_per_step": results["fp64"]["ms_per_step"]}
                  if "fp64" in results else None),
         "speedup_fp32r_over_fp64": (head["pups"] / results["fp64"]["pups"]) if "fp64" in results else None,
-        # FP32 fields with FP64 solves (pyrandaSim precision "fp32-fp64solve")
-        "fp32_fp64solve": {
-            "value": results["fp32-fp64solve"]["pups"],
-            "ms_per_step": results["fp32-fp64solve"]["ms_per_step"],
-            "speedup_over_fp64": (results["fp32-fp64solve"]["pups"] / results["fp64"]["pups"])
-            if "fp64" in results else None,
-            "accuracy_vs_fp64": {k: {"rel_max": a, "rel_l2": b} for k, (a, b) in acc_s.items()},
-        } if "fp32-fp64solve" in results else None,
-        # fp32 with the refined FP32 solve for every operator (no per-operator choice)
-        "fp32_refined_only": {
-            "value": results["fp32-refined-only"]["pups"],
-            "ms_per_step": results["fp32-refined-only"]["ms_per_step"],
-            "accuracy_vs_fp64": {k: {"rel_max": a, "rel_l2": b} for k, (a, b) in acc_o.items()},
-        } if "fp32-refined-only" in results else None,
-        # fp32 whose conservative filter rounds the RK state to FP32 each stage
-        "fp32_rounded_state": {
-            "value": results["fp32-rounded-state"]["pups"],
-            "ms_per_step": results["fp32-rounded-state"]["ms_per_step"],
-            "accuracy_vs_fp64": {k: {"rel_max": a, "rel_l2": b} for k, (a, b) in acc_r.items()},
-        } if "fp32-rounded-state" in results else None,
-        "accuracy_fp32r_vs_fp64": {k: {"rel_max": a, "rel_l2": b} for k, (a, b) in acc.items()},
+        # the other variants: FP32 fields with FP64 solves ("fp32-fp64solve")
+        # and any asked for with --precision
+        "variants": {p.replace("-", "_"): {
+            "value": results[p]["pups"], "ms_per_step": results[p]["ms_per_step"],
+            "speedup_over_fp64": (results[p]["pups"] / results["fp64"]["pups"]) if "fp64" in results else None,
+            "accuracy_vs_fp64": acc.get(p)} for p in results if p not in ("fp32", "fp64")},
+        # the headline run against the FP64 run after warmup + steps RK4 steps
+        "accuracy_fp32r_vs_fp64": acc.get("fp32"),
+        "accuracy_steps": args.warmup + args.steps,
+        "accuracy_worst": worst,
+        "accuracy_bar": 1e-6,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
                      "traffic": (MEASURED_TRAFFIC.get((OPS[dom[0]], DIRS[dom[1]], dom[2])) if n == 256 else None),

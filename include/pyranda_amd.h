@@ -48,6 +48,10 @@ enum { PYR_EPI_STORE = 0, PYR_EPI_ADD = 1, PYR_EPI_ABS_SCALED = 2, PYR_EPI_MAXAB
 
 /* precision of the generic entry point */
 enum { PYR_F64 = 0, PYR_R32 = 1 };
+/* mixed precisions of the RK update (F and U of different widths): FP32
+ * right-hand side with an FP64 field, FP64 right-hand side with an FP32
+ * field (PYR_F64: both double, PYR_R32: both float) */
+enum { PYR_RK_F32_U64 = 2, PYR_RK_F64_U32 = 3 };
 
 /* --- setup / teardown -------------------------------------------------- */
 
@@ -155,7 +159,9 @@ int pyr_apply_rounded(pyr_ctx* ctx, int op, int dir, const double* in, double* o
  *   PYR_F64: U_out <- U_in + B*PHI                (double fields)
  *   PYR_R32: U64 <- U64 + B*PHI (FP64 state), U_out <- (float)U64; with
  *            U64 == NULL: U_out <- (float)((double)U_in + B*PHI)
- * F, U_in, U_out have the field precision; n = number of points. With U64
+ *   PYR_RK_F32_U64 / PYR_RK_F64_U32: as above with F float and U double /
+ *            F double and U float (an FP32 run with some fields kept in FP64)
+ * F, U_in, U_out have the precision prec names; n = number of points. With U64
  * given, U_out may be NULL: only the FP64 state advances (for a caller whose
  * next read of U is the FP64 state, e.g. the filter of the state). */
 int pyr_rk_update(pyr_ctx* ctx, int prec, long long n, const void* F, double* PHI,

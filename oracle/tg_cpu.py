@@ -3,8 +3,17 @@
 The EOM of examples/TaylorGreen.py:40-86 written out by hand in numpy, with
 the compact operators from the C oracle and the 5-stage RK4 of
 pyranda/pyranda.py:758-810. Used as the CPU baseline of bench.py and as an
-independent check of the GPU front end's EOM path. FP64 throughout.
+independent check of the GPU front end's EOM path (the reference-held
+taylorgreen value, tests/test_oracle.py). FP64 throughout.
+
+Threads: the oracle's operators run their lines on `threads` OpenMP threads;
+the pointwise algebra between them runs on the same number of Python threads,
+each over a slab of z-planes (numpy releases the GIL inside its loops). Every
+value is computed by the same operations in the same order as a serial
+evaluation, so the result does not depend on the thread count.
 """
+from concurrent.futures import ThreadPoolExecutor
+
 import numpy as np
 
 from . import pyoracle as po
@@ -33,7 +42,12 @@ class TaylorGreenCPU:
         X, Y, Z = np.meshgrid(x, x, x, indexing="ij")
         self.n = n
         self.d = d
-        self.P = po.Oracle((n, n, n), (d, d, d), (True, True, True), threads=threads)
+        self.threads = max(1, int(threads))
+        self._pool = ThreadPoolExecutor(self.threads) if self.threads > 1 else None
+        nz = n
+        cuts = np.linspace(0, nz, min(self.threads, nz) + 1).astype(int)
+        self._slabs = [slice(int(a), int(b)) for a, b in zip(cuts[:-1], cuts[1:]) if b > a]
+        self.P = po.Oracle((n, n, n), (d, d, d), (True, True, True), threads=self.threads)
         self.gamma = 1.4
         F = lambda a: np.asfortranarray(a)
         u = F(np.sin(X) * np.cos(Y) * np.cos(Z))
@@ -46,34 +60,85 @@ class TaylorGreenCPU:
         self.V = {}
         self.update_vars()
 
+    def close(self):
+        if self._pool is not None:
+            self._pool.shutdown()
+            self._pool = None
+
+    # -- pointwise algebra over z-slabs ---------------------------------------
+    def pw(self, fn, *fields, nout=1):
+        """Outputs of fn(*slabs) for every z-slab of the fields (Fortran order
+        (x, y, z) arrays), assembled into new fields; fn returns one array or
+        a tuple of nout arrays of the slab's shape."""
+        shape = fields[0].shape
+        outs = [np.empty(shape, order="F") for _ in range(nout)]
+
+        def run(sl):
+            r = fn(*[f[:, :, sl] for f in fields])
+            r = r if nout > 1 else (r,)
+            for o, a in zip(outs, r):
+                o[:, :, sl] = a
+
+        if self._pool is None:
+            run(slice(None))
+        else:
+            list(self._pool.map(run, self._slabs))
+        return tuple(outs) if nout > 1 else outs[0]
+
+    def pmax(self, fn, *fields):
+        """max over the grid of fn(*slabs) (slab maxima, then their max)."""
+        if self._pool is None:
+            return np.max(fn(*fields))
+        return max(self._pool.map(lambda sl: np.max(fn(*[f[:, :, sl] for f in fields])), self._slabs))
+
+    def pmin(self, fn, *fields):
+        if self._pool is None:
+            return np.min(fn(*fields))
+        return min(self._pool.map(lambda sl: np.min(fn(*[f[:, :, sl] for f in fields])), self._slabs))
+
     def courant(self, u, v, w, c):
         d = self.d
-        vrate = np.abs(u) / d + np.abs(v) / d + np.abs(w) / d
-        return 1.0 / np.max(vrate + np.abs(c) / d)
+        return 1.0 / self.pmax(lambda u, v, w, c: (np.abs(u) / d + np.abs(v) / d + np.abs(w) / d) + np.abs(c) / d,
+                               u, v, w, c)
 
     def diff(self, bulk, rho):
-        return np.min(rho * self.d * self.d / np.maximum(1.0e-12, bulk))
+        d = self.d
+        return self.pmin(lambda b, r: r * d * d / np.maximum(1.0e-12, b), bulk, rho)
 
     def update_vars(self):
-        P, U, V = self.P, self.U, self.V
+        P, U, V, g = self.P, self.U, self.V, self.gamma
         for k in ("rho", "rhou", "rhov", "rhow", "Et"):
             U[k] = P.sfilter(U[k])
         rho = U["rho"]
-        u, v, w = U["rhou"] / rho, U["rhov"] / rho, U["rhow"] / rho
-        p = (U["Et"] - .5 * rho * (u * u + v * v + w * w)) * (self.gamma - 1.0)
+
+        def prim(rho, ru, rv, rw, Et):
+            u, v, w = ru / rho, rv / rho, rw / rho
+            p = (Et - .5 * rho * (u * u + v * v + w * w)) * (g - 1.0)
+            return u, v, w, p
+
+        u, v, w, p = self.pw(prim, rho, U["rhou"], U["rhov"], U["rhow"], U["Et"], nout=4)
         ux, vy, wz = P.ddx(u), P.ddy(v), P.ddz(w)
-        div = ux + vy + wz
+        div = self.pw(lambda a, b, c: a + b + c, ux, vy, wz)
         uy, uz, vx, vz, wy, wx = P.ddy(u), P.ddz(u), P.ddx(v), P.ddz(v), P.ddy(w), P.ddx(w)
-        V["enst"] = np.sqrt((uy - vx) ** 2 + (uz - wx) ** 2 + (vz - wy) ** 2)
-        V["tke"] = rho * (u * u + v * v + w * w)
-        S = np.sqrt(ux * ux + vy * vy + wz * wz + .5 * ((uy + vx) ** 2 + (uz + wx) ** 2 + (vz + wy) ** 2))
-        mu = P.gfilter(np.abs(P.ring(S))) * rho * 1.0e-4
-        beta = P.gfilter(np.abs(P.ring(div)) * rho) * 7.0e-3
-        taudia = (beta - 2. / 3. * mu) * div - p
-        V.update(u=u, v=v, w=w, p=p, mu=mu, beta=beta,
-                 tauxx=2.0 * mu * ux + taudia, tauyy=2.0 * mu * vy + taudia, tauzz=2.0 * mu * wz + taudia,
-                 tauxy=mu * (uy + vx), tauxz=mu * (uz + wx), tauyz=mu * (vz + wz))
-        cs = np.sqrt(p / rho * self.gamma)
+        V["enst"] = self.pw(lambda uy, vx, uz, wx, vz, wy: np.sqrt((uy - vx) ** 2 + (uz - wx) ** 2 + (vz - wy) ** 2),
+                            uy, vx, uz, wx, vz, wy)
+        V["tke"] = self.pw(lambda r, u, v, w: r * (u * u + v * v + w * w), rho, u, v, w)
+        S = self.pw(lambda ux, vy, wz, uy, vx, uz, wx, vz, wy:
+                    np.sqrt(ux * ux + vy * vy + wz * wz + .5 * ((uy + vx) ** 2 + (uz + wx) ** 2 + (vz + wy) ** 2)),
+                    ux, vy, wz, uy, vx, uz, wx, vz, wy)
+        mu = self.pw(lambda a, r: a * r * 1.0e-4, P.gfilter(self.pw(np.abs, P.ring(S))), rho)
+        beta = self.pw(lambda a: a * 7.0e-3,
+                       P.gfilter(self.pw(lambda a, r: np.abs(a) * r, P.ring(div), rho)))
+
+        def stress_rho(beta, mu, div, p, ux, vy, wz, uy, vx, uz, wx, vz, r):
+            taudia = (beta - 2. / 3. * mu) * div - p
+            return (2.0 * mu * ux + taudia, 2.0 * mu * vy + taudia, 2.0 * mu * wz + taudia,
+                    mu * (uy + vx), mu * (uz + wx), mu * (vz + wz), np.sqrt(p / r * g))
+
+        txx, tyy, tzz, txy, txz, tyz, cs = self.pw(stress_rho, beta, mu, div, p, ux, vy, wz, uy, vx, uz, wx,
+                                                   vz, rho, nout=7)
+        V.update(u=u, v=v, w=w, p=p, mu=mu, beta=beta, tauxx=txx, tauyy=tyy, tauzz=tzz,
+                 tauxy=txy, tauxz=txz, tauyz=tyz)
         dt = self.courant(u, v, w, cs) * 1.0
         dt = np.minimum(dt, 0.2 * self.diff(beta, rho))
         dt = np.minimum(dt, 0.2 * self.diff(mu, rho))
@@ -83,22 +148,38 @@ class TaylorGreenCPU:
         P, U, V = self.P, self.U, self.V
         u, v, w = V["u"], V["v"], V["w"]
         txx, tyy, tzz, txy, txz, tyz = (V[k] for k in ("tauxx", "tauyy", "tauzz", "tauxy", "tauxz", "tauyz"))
+        pw = self.pw
+
+        def rhs(a, b, c):
+            return pw(lambda a, b, c: -a - b - c, a, b, c)
+
         return {
-            "rho": -P.ddx(U["rho"] * u) - P.ddy(U["rho"] * v) - P.ddz(U["rho"] * w),
-            "rhou": -P.ddx(U["rhou"] * u - txx) - P.ddy(U["rhou"] * v - txy) - P.ddz(U["rhou"] * w - txz),
-            "rhov": -P.ddx(U["rhov"] * u - txy) - P.ddy(U["rhov"] * v - tyy) - P.ddz(U["rhov"] * w - tyz),
-            "rhow": -P.ddx(U["rhow"] * u - txz) - P.ddy(U["rhow"] * v - tyz) - P.ddz(U["rhow"] * w - tzz),
-            "Et": -P.ddx((U["Et"] - txx) * u - txy * v - txz * w)
-                  - P.ddy((U["Et"] - tyy) * v - txy * u - tyz * w)
-                  - P.ddz((U["Et"] - tzz) * w - txz * u - tyz * v),
+            "rho": rhs(P.ddx(pw(lambda r, u: r * u, U["rho"], u)), P.ddy(pw(lambda r, v: r * v, U["rho"], v)),
+                       P.ddz(pw(lambda r, w: r * w, U["rho"], w))),
+            "rhou": rhs(P.ddx(pw(lambda a, u, t: a * u - t, U["rhou"], u, txx)),
+                        P.ddy(pw(lambda a, v, t: a * v - t, U["rhou"], v, txy)),
+                        P.ddz(pw(lambda a, w, t: a * w - t, U["rhou"], w, txz))),
+            "rhov": rhs(P.ddx(pw(lambda a, u, t: a * u - t, U["rhov"], u, txy)),
+                        P.ddy(pw(lambda a, v, t: a * v - t, U["rhov"], v, tyy)),
+                        P.ddz(pw(lambda a, w, t: a * w - t, U["rhov"], w, tyz))),
+            "rhow": rhs(P.ddx(pw(lambda a, u, t: a * u - t, U["rhow"], u, txz)),
+                        P.ddy(pw(lambda a, v, t: a * v - t, U["rhow"], v, tyz)),
+                        P.ddz(pw(lambda a, w, t: a * w - t, U["rhow"], w, tzz))),
+            "Et": rhs(P.ddx(pw(lambda E, txx, u, txy, v, txz, w: (E - txx) * u - txy * v - txz * w,
+                               U["Et"], txx, u, txy, v, txz, w)),
+                      P.ddy(pw(lambda E, tyy, v, txy, u, tyz, w: (E - tyy) * v - txy * u - tyz * w,
+                               U["Et"], tyy, v, txy, u, tyz, w)),
+                      P.ddz(pw(lambda E, tzz, w, txz, u, tyz, v: (E - tzz) * w - txz * u - tyz * v,
+                               U["Et"], tzz, w, txz, u, tyz, v))),
         }
 
     def rk4(self, time, dt):
         PHI = {k: np.zeros_like(a) for k, a in self.U.items()}
         for ii in range(5):
             F = self.fluxes()
+            A, B = RK_A[ii], RK_B[ii]
             for k in self.U:
-                PHI[k] = dt * F[k] + RK_A[ii] * PHI[k]
-                self.U[k] = self.U[k] + RK_B[ii] * PHI[k]
+                PHI[k], self.U[k] = self.pw(lambda f, ph, uu: ((lambda q: (q, uu + B * q))(dt * f + A * ph)),
+                                            F[k], PHI[k], self.U[k], nout=2)
             self.update_vars()
         return time + dt

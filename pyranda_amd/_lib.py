@@ -16,6 +16,7 @@ OP_D1, OP_D2, OP_D8, OP_SFILTER, OP_GFILTER = 0, 1, 2, 3, 4
 OP_ANTISYM = 0x10  # or-ed into an op: antisymmetric member of the SYMM pair
 EPI_STORE, EPI_ADD, EPI_ABS_SCALED, EPI_MAXABS_SCALED = 0, 1, 2, 3
 F64, R32 = 0, 1
+RK_F32_U64, RK_F64_U32 = 2, 3  # RK update with F and U of different widths
 PSOLVE = {"circulant": 0, "lu": 1}
 R32_SOLVE = {"refined": 0, "fp64": 1}
 

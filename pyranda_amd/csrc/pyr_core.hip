@@ -763,8 +763,8 @@ __global__ void k_null(long long n, int copy, int epi, double c, const TS* __res
 // clearing before the step. dtp (if not null) holds dt in device memory: one
 // value, or with mid (the member index of every point of an ensemble's
 // fields) one value per member.
-template <typename TS>
-__global__ void k_rk_update(long long n, const TS* __restrict__ F, double* __restrict__ PHI,
+template <typename TF, typename TS>
+__global__ void k_rk_update(long long n, const TF* __restrict__ F, double* __restrict__ PHI,
                             const TS* __restrict__ Uin, TS* __restrict__ Uout,
                             double* __restrict__ U64, const double* __restrict__ dtp,
                             const int* __restrict__ mid, double dt, double A, double B) {
@@ -817,7 +817,7 @@ struct pyr_ctx {
   bool has_anti[3] = {false, false, false};
   OpPlan plan_a[NOPS][3];
   DevPlan dplan_a[NOPS][3];
-  void* scratch[2] = {nullptr, nullptr};  // one field-sized temp per precision
+  void* field_tmp[2] = {nullptr, nullptr};  // one field-sized temp per precision
   int periodic_solver = PYR_PSOLVE_CIRCULANT;
   int r32_solve = PYR_R32_SOLVE_REFINED;
   int x_lds_kb = 40;                        // LDS budget of one x-line workgroup
@@ -1228,20 +1228,25 @@ static int apply_any(pyr_ctx* c, int op, int dir, int prec, const void* in, void
   return fail("bad precision");
 }
 
-static int scratch(pyr_ctx* c, int prec, void** p) {
+static int field_temp(pyr_ctx* c, int prec, void** p) {
   const size_t bytes = c->npts() * (prec == PYR_F64 ? sizeof(double) : sizeof(float));
-  if (!c->scratch[prec]) HIPCHECK(hipMalloc(&c->scratch[prec], bytes));
-  *p = c->scratch[prec];
+  if (!c->field_tmp[prec]) HIPCHECK(hipMalloc(&c->field_tmp[prec], bytes));
+  *p = c->field_tmp[prec];
   return 0;
+}
+
+// filter x -> y -> z (operators.f90:847-875) through the field temp `tmp`
+static int filter3_with(pyr_ctx* c, int op, int prec, const void* in, void* out, void* tmp) {
+  if (apply_any(c, op, 0, prec, in, out, EPI_STORE, 0.0)) return 1;
+  if (apply_any(c, op, 1, prec, out, tmp, EPI_STORE, 0.0)) return 1;
+  return apply_any(c, op, 2, prec, tmp, out, EPI_STORE, 0.0);
 }
 
 // filter x -> y -> z (operators.f90:847-875)
 static int filter3(pyr_ctx* c, int op, int prec, const void* in, void* out) {
   void* tmp;
-  if (check_ctx(c) || scratch(c, prec, &tmp)) return 1;
-  if (apply_any(c, op, 0, prec, in, out, EPI_STORE, 0.0)) return 1;
-  if (apply_any(c, op, 1, prec, out, tmp, EPI_STORE, 0.0)) return 1;
-  return apply_any(c, op, 2, prec, tmp, out, EPI_STORE, 0.0);
+  if (check_ctx(c) || field_temp(c, prec, &tmp)) return 1;
+  return filter3_with(c, op, prec, in, out, tmp);
 }
 
 // ring L=2 (operators.f90:615-643): max_d |d8_d f| * d_d^2, fused into the stores
@@ -1749,7 +1754,7 @@ int pyr_destroy(pyr_ctx* c) {
         for (DevPlan* dp : {&c->dplan[op][d], &c->dplan_a[op][d]})
           if (dp->mem) (void)hipFree(dp->mem);
     for (int i = 0; i < 2; ++i)
-      if (c->scratch[i]) (void)hipFree(c->scratch[i]);
+      if (c->field_tmp[i]) (void)hipFree(c->field_tmp[i]);
   }
   delete c;
   return 0;
@@ -1879,7 +1884,7 @@ int pyr_sfilter_f64_rounded(pyr_ctx* c, const double* v, double* o, float* o32) 
   if (check_ctx(c)) return 1;
   if (!o32) return fail("null field pointer");
   void* tmp;
-  if (scratch(c, PYR_F64, &tmp)) return 1;
+  if (field_temp(c, PYR_F64, &tmp)) return 1;
   if (apply_any(c, OP_SF, 0, PYR_F64, v, o, EPI_STORE, 0.0)) return 1;
   if (apply_any(c, OP_SF, 1, PYR_F64, o, tmp, EPI_STORE, 0.0)) return 1;
   // the z sweep also stores the FP32 rounding of the result
@@ -1925,11 +1930,17 @@ static int rk_update(hipStream_t stream, int prec, long long n, const void* F, d
   long long blocks = (n + 1023) / 1024;
   if (blocks > 8192) blocks = 8192;
   if (prec == PYR_F64)
-    hipLaunchKernelGGL((k_rk_update<double>), dim3((unsigned)blocks), dim3(256), 0, stream, n,
+    hipLaunchKernelGGL((k_rk_update<double, double>), dim3((unsigned)blocks), dim3(256), 0, stream, n,
                        (const double*)F, PHI, (const double*)Uin, (double*)Uout, U64, dtp, mid, dt, A, B);
   else if (prec == PYR_R32)
-    hipLaunchKernelGGL((k_rk_update<float>), dim3((unsigned)blocks), dim3(256), 0, stream, n,
+    hipLaunchKernelGGL((k_rk_update<float, float>), dim3((unsigned)blocks), dim3(256), 0, stream, n,
                        (const float*)F, PHI, (const float*)Uin, (float*)Uout, U64, dtp, mid, dt, A, B);
+  else if (prec == PYR_RK_F32_U64)
+    hipLaunchKernelGGL((k_rk_update<float, double>), dim3((unsigned)blocks), dim3(256), 0, stream, n,
+                       (const float*)F, PHI, (const double*)Uin, (double*)Uout, U64, dtp, mid, dt, A, B);
+  else if (prec == PYR_RK_F64_U32)
+    hipLaunchKernelGGL((k_rk_update<double, float>), dim3((unsigned)blocks), dim3(256), 0, stream, n,
+                       (const double*)F, PHI, (const float*)Uin, (float*)Uout, U64, dtp, mid, dt, A, B);
   else
     return fail("bad precision");
   HIPCHECK(hipGetLastError());

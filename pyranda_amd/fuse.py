@@ -11,7 +11,9 @@ their pointwise arguments are fused kernels of their own.
 
 Evaluation order and rounding follow Python/numpy: operands are evaluated in
 FP64 registers with the same expression tree, no FMA contraction, and the
-result is stored in FP64 if any input field is FP64, else in FP32. Subtrees
+result is stored in FP64 if any input field is FP64, else in FP32 (FP32
+runs that store fields by value decide per result instead, see
+offset_dominated). Subtrees
 whose inputs are all scalars are evaluated in Python; inputs that are not
 whole fields (slices, vectors) make that subtree evaluate with torch ops.
 """
@@ -133,6 +135,35 @@ class _Gen:
         raise AssertionError(name)
 
 
+# --- storage of FP32 runs' fields ---------------------------------------------
+# FP32 storage rounds a value to 6e-8 of its magnitude. A field whose values
+# all sit close to one large value (the pressure of a low-Mach flow, p ~ 71
+# against variations ~ 0.4; the total energy; stresses carrying -p) loses
+# most of its variation to that rounding, and a derivative of it turns the
+# rounding into noise of ~1e-4 relative. In FP32 runs such fields are stored
+# in FP64: a field is offset-dominated when its largest magnitude exceeds
+# WIDE_RATIO times its range of values (max - min).
+WIDE_RATIO = 8.0
+
+# value_storage: True while an FP32 run with storage by value evaluates its
+# equations: a pointwise kernel's unnamed result (e.g. the flux argument of a
+# derivative) is then stored in FP64 when its first evaluation (in FP64) is
+# offset-dominated, else in FP32 (the decision is kept per kernel and input
+# signature); with False a result is FP64 when an input field is
+STORAGE = {"value_storage": False}
+
+
+def offset_dominated(t):
+    """Is the field offset-dominated (max|t| > WIDE_RATIO * (max t - min t))?
+    A constant nonzero field is; a zero or non-finite field is not."""
+    lo, hi = torch.aminmax(t.detach())
+    lo, hi = float(lo), float(hi)
+    mag = max(abs(lo), abs(hi))
+    if not (mag == mag) or mag == float("inf"):
+        return False
+    return mag > 0.0 and mag > WIDE_RATIO * (hi - lo)
+
+
 def _device_scalar(v):
     return (torch.is_tensor(v) and v.dim() == 0 and v.is_cuda
             and v.dtype in (torch.float64, torch.float32))
@@ -191,6 +222,11 @@ class FusedSubtree:
                         {"numpy": _dnp})
         self._kernels = {}
         self.tag = src_tag
+        # storage of the result: out_dtype ('f64'/'f32', set by the owner of a
+        # named result) wins; else decided per input signature when
+        # STORAGE["value_storage"] is on; else by the input types
+        self.out_dtype = None
+        self._decided = {}
 
     def _source(self, sig, out_dt):
         name = "pyr_pw_" + hashlib.sha1((self.body + repr(sig) + out_dt).encode()).hexdigest()[:16]
@@ -278,16 +314,30 @@ class FusedSubtree:
             fields.append(shape.mid)
         if len(fields) + 1 > _lib.PW_MAX_PTRS or len(scalars) > _lib.PW_MAX_SCALARS:
             return self._py(*_expand_members(vals, shape))
-        out_dt = "f64" if any64 else "f32"
-        fn = self.kernel(tuple(sig), out_dt)
+        sig = tuple(sig)
+        decide = False
+        if self.out_dtype is not None:
+            out_dt = self.out_dtype
+        elif STORAGE["value_storage"]:
+            out_dt = self._decided.get(sig)
+            if out_dt is None:
+                out_dt, decide = "f64", True
+        else:
+            out_dt = "f64" if any64 else "f32"
+        fn = self.kernel(sig, out_dt)
         nx, ny, nz = shape
-        out = torch.empty((nz, ny, nx), dtype=torch.float64 if any64 else torch.float32,
+        out = torch.empty((nz, ny, nx), dtype=torch.float64 if out_dt == "f64" else torch.float32,
                           device=dev).permute(2, 1, 0)
         ptrs = (ctypes.c_void_p * (len(fields) + 1))(*[f.data_ptr() for f in fields], out.data_ptr())
         sc = (ctypes.c_double * max(1, len(scalars)))(*scalars)
         stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
         _lib.check_pw(_lib.lib().pyr_pw_launch(fn, stream, nx * ny * nz, len(fields) + 1, ptrs,
                                                len(scalars), sc))
+        if decide:
+            # first evaluation in FP64: keep FP64 storage if offset-dominated
+            self._decided[sig] = "f64" if offset_dominated(out) else "f32"
+            if self._decided[sig] == "f32":
+                out = out.to(torch.float32)
         return out
 
 
@@ -321,6 +371,7 @@ class FusedGroup:
         self.maps = [[g.leaf_src[ast.unparse(l)] for l in sub.leaves] for sub in subs]
         self._kernels = {}
         self.tag = src_tag
+        self._decided = {}  # per input signature: output types by value (FusedSubtree)
 
     def _source(self, sig, out_dts):
         key = "|".join(self.bodies) + repr(sig) + repr(out_dts)
@@ -404,6 +455,13 @@ class FusedGroup:
             if not any(sig[j][0] == "T" for j in m):
                 return self._separate(shape, vals)
             out_dts.append("f64" if any(is64[j] for j in m) else "f32")
+        decide = False
+        if STORAGE["value_storage"]:
+            known = self._decided.get(tuple(sig))
+            if known is None:
+                out_dts, decide = ["f64"] * len(self.maps), True
+            else:
+                out_dts = list(known)
         if any(k[0] == "G" for k in sig):
             fields.append(shape.mid)
         nout = len(self.subs)
@@ -419,6 +477,10 @@ class FusedGroup:
         stream = ctypes.c_void_p(torch.cuda.current_stream(dev).cuda_stream)
         _lib.check_pw(_lib.lib().pyr_pw_launch(fn, stream, nx * ny * nz, len(fields) + nout, ptrs,
                                                len(scalars), sc))
+        if decide:
+            dts = tuple("f64" if offset_dominated(o) else "f32" for o in outs)
+            self._decided[tuple(sig)] = dts
+            outs = [o if dt == "f64" else o.to(torch.float32) for o, dt in zip(outs, dts)]
         return tuple(outs)
 
 
@@ -532,6 +594,12 @@ class FusedRHS:
         body = f.visit(tree.body)
         self.subtrees = f.subtrees
         self.src = src
+        # the subtree that computes the whole right-hand side, if one does
+        # (its result is the equation's value: its storage can be set)
+        self.root = None
+        if (isinstance(body, ast.Call) and isinstance(body.func, ast.Subscript)
+                and isinstance(body.func.value, ast.Name) and body.func.value.id == "_pw"):
+            self.root = self.subtrees[body.func.slice.value]
         body, self.group = _group_siblings(body, self.subtrees, src[:40]) if GROUP_SIBLINGS else (body, None)
         lam = ast.Expression(body=ast.Lambda(
             args=ast.arguments(posonlyargs=[], args=[ast.arg(arg="self"), ast.arg(arg="_shape"),
@@ -941,6 +1009,18 @@ class PointwiseRun:
         types = self._types(sig)
         if types is None or dev is None:
             return False
+        # an FP32 run's named fields have a storage type of their own
+        # (pyrandaSim.storage_of: 'f64', 'f32', or None when not decided yet:
+        # computed in FP64, then decided from the values)
+        storage_of = getattr(sim, "storage_of", None) if getattr(sim, "value_storage", False) else None
+        undecided = []
+        if storage_of is not None:
+            for j, lhs in enumerate(self.outputs):
+                st = storage_of(lhs)
+                if st is None:
+                    undecided.append(j)
+                    st = "f64"
+                types[j] = st
         if len(fields) + len(types) > _lib.PW_MAX_PTRS or len(scalars) > _lib.PW_MAX_SCALARS:
             return False
         key = (sig, tuple(types))
@@ -963,6 +1043,8 @@ class PointwiseRun:
                                                len(scalars), sc))
         for lhs, o in zip(self.outputs, outs):
             sim.variables[lhs].data = o
+        for j in undecided:
+            sim.settle_storage(self.outputs[j])
         return True
 
 

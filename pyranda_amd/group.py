@@ -306,6 +306,10 @@ class pyrandaGroupSim(pyrandaSim):
         self.silent = silent
         self.precision = precision
         self.dtype = torch.float64 if precision == "fp64" else torch.float32
+        # every field of a group is stored in the run's precision (no FP64
+        # storage of offset-dominated fields, pyrandaSim(wide_fields=None))
+        self._storage = {}
+        self.value_storage = self.named_storage = False
         self.members = [_Member(m, dev) for m in meshes]
         if precision == "fp32-fp64solve":
             for m in self.members:

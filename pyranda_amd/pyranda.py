@@ -15,6 +15,17 @@ operators' right-hand sides and solves in FP64 registers (one rounding to
 FP32 per result): on gfx950 FP32 and FP64 vector FMAs issue at the same rate,
 so this keeps the halved memory traffic without the refinement's second solve.
 
+Storage of FP32 runs (`wide_fields`): FP32 storage rounds a value to 6e-8 of
+its magnitude, so a field whose values sit close to one large value (the
+pressure and total energy of a low-Mach flow, stresses carrying -p, the
+momentum fluxes) loses most of its variation, and a derivative turns that
+rounding into noise. With wide_fields="auto" (the default of the fp32 modes)
+every field is stored in FP32 unless it is offset-dominated when first set
+(fuse.offset_dominated): such fields, named or the unnamed results of
+pointwise kernels (flux arguments), are stored in FP64, and the decision is
+kept for the run. wide_fields=None stores every field in FP32; a collection
+of names stores exactly those variables in FP64.
+
 `device_scalars=True` keeps the grid reductions (dt.courant, dt.diff, sum,
 max, min) on the GPU as 0-d FP64 tensors instead of Python floats, and rk4
 accepts such a tensor as dt: a time step then makes no host round trip and
@@ -49,7 +60,7 @@ RK_ETA = [494393426753. / 4806282396855., 4702696611523. / 9636871101405.,
 class pyrandaSim:
 
     def __init__(self, name, meshOptions, silent=False, precision="fp64", device="cuda",
-                 device_scalars=False, fp64_state=True):
+                 device_scalars=False, fp64_state=True, wide_fields="auto"):
         self.name = name
         # fp32 modes: conserved variables rewritten only by the conservative
         # filter keep an FP64 state through it (False: the FP32 field is the state)
@@ -104,6 +115,16 @@ class pyrandaSim:
         self._shadow = {}  # FP64 RK state of conserved variables (fp32 mode)
         self._state64 = {}  # persistent FP64 state of self-filtered conserved variables
         self._state_src = {}  # the FP32 field last derived from it
+        # storage of named fields in fp32 modes: name -> "f64"/"f32"
+        self._storage = {}
+        self.value_storage = False   # undecided fields are decided by value
+        self.named_storage = False   # named fields have a storage of their own
+        if precision != "fp64":
+            if wide_fields == "auto":
+                self.value_storage = self.named_storage = True
+            elif wide_fields is not None:
+                self._storage = {n: "f64" for n in wide_fields}
+                self.named_storage = True
 
     def autotune(self, **kw):
         """Tune the line-operator launches of this grid and precision on the
@@ -281,22 +302,79 @@ class pyrandaSim:
             if name.startswith("mesh"):
                 continue
             d = var.data
-            # fields take the run's dtype; device scalars (reductions kept on
-            # the GPU, one value or one per ensemble member) stay FP64 like
-            # the host floats they stand for
-            if (torch.is_tensor(d) and tuple(d.shape) == tuple(self.mesh.shape) and d.is_floating_point()
-                    and d.dtype != self.dtype):
-                var.data = d.to(self.dtype)
+            # fields take the run's dtype (fp32 modes: their storage, see
+            # storage_of); device scalars (reductions kept on the GPU, one
+            # value or one per ensemble member) stay FP64 like the host
+            # floats they stand for
+            if torch.is_tensor(d) and tuple(d.shape) == tuple(self.mesh.shape) and d.is_floating_point():
+                want = self._storage_dtype(name, d)
+                if name in self._state64 and want == torch.float64:
+                    # an FP64 field with an FP64 state: the state is the field
+                    var.data = self._state64[name]
+                elif d.dtype != want:
+                    var.data = d.to(want)
+
+    # -- storage of fp32 runs' fields (see the module docstring) -----------------
+    def _is_field(self, t):
+        return torch.is_tensor(t) and tuple(t.shape) == tuple(self.mesh.shape) and t.is_floating_point()
+
+    def storage_of(self, name):
+        """Storage of a named field: "f64", "f32", or None while an fp32 run
+        with storage by value has not seen the field yet."""
+        if self.dtype == torch.float64:
+            return "f64"
+        st = self._storage.get(name)
+        if st is None and not self.value_storage:
+            return "f32"
+        return st
+
+    def _storage_dtype(self, name, t):
+        """Storage dtype of `name`, deciding it from `t` if undecided."""
+        st = self.storage_of(name)
+        if st is None:
+            from .fuse import offset_dominated
+            st = self._storage[name] = "f64" if offset_dominated(t) else "f32"
+        return torch.float64 if st == "f64" else torch.float32
+
+    def settle_storage(self, name):
+        """Decide (once) and apply the storage of a named field from its current value."""
+        var = self.variables[name]
+        d = var.data
+        if self.dtype == torch.float64 or not self._is_field(d):
+            return
+        want = self._storage_dtype(name, d)
+        if d.dtype != want:
+            var.data = d.to(want)
+
+    def _wide(self, name):
+        return self.dtype != torch.float64 and self._storage.get(name) == "f64"
 
     def emptyScalar(self, val=0.0):
         return self.PyMPI.emptyScalar() + val
 
     # -- time integration (pyranda.py:376-427, 758-810) --------------------------
+    def _storage_mode(self):
+        """Context of an evaluation: unnamed pointwise results are stored by
+        value while this run stores its fields by value."""
+        from . import fuse
+        sim = self
+
+        class _Mode:
+            def __enter__(self):
+                self.old = fuse.STORAGE["value_storage"]
+                fuse.STORAGE["value_storage"] = sim.value_storage
+
+            def __exit__(self, *exc):
+                fuse.STORAGE["value_storage"] = self.old
+
+        return _Mode()
+
     def updateFlux(self):
         flux = {}
-        for eqo in self.equations:
-            if eqo.kind == "PDE":
-                flux[eqo.LHS[0]] = eqo.RHS(self)
+        with self._storage_mode():
+            for eqo in self.equations:
+                if eqo.kind == "PDE":
+                    flux[eqo.LHS[0]] = eqo.RHS(self)
         return flux
 
     def _alg_plan(self):
@@ -316,11 +394,14 @@ class pyrandaSim:
         U = eq.LHS[0] if eq.LHS and len(eq.LHS) == 1 else None
         if U is not None and U in self._state64 and U in self._self_filtered():
             # the conservative filter of an FP32 run acts on the FP64 state
-            # (FP64 kernels) and the field is its FP32 rounding
+            # (FP64 kernels) and the field is its FP32 rounding (or, for a
+            # field stored in FP64, the state itself)
             if not self._state_current(U):
                 self._state64[U] = self.variables[U].data.to(torch.float64)
             rounded = getattr(self.PyMPI.fil, "filter_rounded", None)
-            if rounded is not None and self.dtype == torch.float32:
+            if self._wide(U):
+                out = out32 = self.filter(self._state64[U])
+            elif rounded is not None and self.dtype == torch.float32:
                 # the last sweep stores the rounding as well
                 out, out32 = rounded(self._state64[U])
             else:
@@ -330,23 +411,33 @@ class pyrandaSim:
             self.variables[U].data = out32
             self._mark_state(U)
             return
+        root = getattr(eq.RHS, "root", None)
+        if self.named_storage and root is not None and eq.rank == 1 and root.out_dtype is None:
+            st = self.storage_of(eq.LHS[0])
+            if st is not None:
+                root.out_dtype = st
         rhs = eq.RHS(self)
         if not eq.LHS:
             return
         if eq.rank == 1:
             self.variables[eq.LHS[0]].data = rhs
+            if self.named_storage:
+                self.settle_storage(eq.LHS[0])
         else:
             for ii in range(len(rhs)):
                 self.variables[eq.LHS[ii]].data = rhs[ii]
+                if self.named_storage:
+                    self.settle_storage(eq.LHS[ii])
 
     def updateVars(self, verbose=False):
         # pointwise ALG lines run as fused multi-output kernels where possible
-        for item in self._alg_plan():
-            if item[0] == "eq":
-                self._eval_alg(item[1], verbose)
-            elif verbose or not item[1](self):
-                for eq in item[2]:
-                    self._eval_alg(eq, verbose)
+        with self._storage_mode():
+            for item in self._alg_plan():
+                if item[0] == "eq":
+                    self._eval_alg(item[1], verbose)
+                elif verbose or not item[1](self):
+                    for eq in item[2]:
+                        self._eval_alg(eq, verbose)
 
     def rk4(self, time, dt):
         """5-stage 2N-storage RK4 (pyranda.py:758-810).
@@ -375,10 +466,14 @@ class pyrandaSim:
             if fp32 and U in sf:
                 if not self._state_current(U):
                     self._state64[U] = self.variables[U].data.to(torch.float64)
+                    if self._wide(U):
+                        self.variables[U].data = self._state64[U]
                     self._mark_state(U)
                 self._shadow[U] = self._state64[U]
             elif fp32:
-                self._shadow[U] = None if U in rewritten else self.variables[U].data.to(torch.float64)
+                # a field stored in FP64 needs no FP64 shadow
+                self._shadow[U] = None if (U in rewritten or self._wide(U)) else \
+                    self.variables[U].data.to(torch.float64)
         time_i = time
         self.deltat = dt
         for ii in range(5):
@@ -399,7 +494,8 @@ class pyrandaSim:
                 for U in self.conserved:
                     if U in sf:
                         self._shadow[U] = self._state64[U]
-                    elif U not in rewritten and self.variables[U].data is not before[U]:
+                    elif (U not in rewritten and not self._wide(U)
+                          and self.variables[U].data is not before[U]):
                         self._shadow[U] = self.variables[U].data.to(torch.float64)
         self.cycle += 1
         return time
@@ -475,35 +571,49 @@ class pyrandaSim:
         from .parcop import _contig_equiv
         u = self.variables[U].data
         nn = tuple(self.mesh.shape)
+        floats = (torch.float32, torch.float64)
+        # an FP64 field of an fp32 run whose FP64 state is the field itself
+        state_is_field = fp32 and torch.is_tensor(u) and u.dtype == torch.float64 and self._shadow.get(U) is u
         ok = (torch.is_tensor(F) and tuple(F.shape) == nn and _contig_equiv(F)
-              and F.dtype == self.dtype and torch.is_tensor(u) and tuple(u.shape) == nn
-              and _contig_equiv(u) and u.dtype == self.dtype)
+              and F.dtype in floats and torch.is_tensor(u) and tuple(u.shape) == nn
+              and _contig_equiv(u) and u.dtype in floats)
         if not ok:
             # scalar fluxes (e.g. ODE right-hand sides) and odd layouts: torch ops
             PHI[U] = dt * F + A * (PHI[U] if A != 0.0 else torch.zeros_like(PHI[U]))
-            if fp32:
+            if state_is_field:
+                self._shadow[U] = u + B * PHI[U]
+                return self._shadow[U]
+            if fp32 and u.dtype != torch.float64:
                 base = self._shadow[U] if self._shadow[U] is not None else u.to(torch.float64)
                 self._shadow[U] = base + B * PHI[U]
                 return self._shadow[U].to(self.dtype)
             return u + B * PHI[U]
-        shadow = self._shadow[U].data_ptr() if fp32 and self._shadow[U] is not None else None
-        if shadow is not None and not store:
-            out, optr = u, None
+        if state_is_field:
+            # the kernel advances the state in place; nothing else to write
+            shadow, out, uin, optr = u.data_ptr(), u, None, None
         else:
-            out = torch.empty((self.nz, self.ny, self.nx), dtype=self.dtype, device=u.device).permute(2, 1, 0)
-            optr = out.data_ptr()
+            shadow = (self._shadow[U].data_ptr() if fp32 and u.dtype != torch.float64
+                      and self._shadow[U] is not None else None)
+            uin = u.data_ptr()
+            if shadow is not None and not store:
+                out, optr = u, None
+            else:
+                out = torch.empty((self.nz, self.ny, self.nx), dtype=u.dtype, device=u.device).permute(2, 1, 0)
+                optr = out.data_ptr()
         ctx = self.PyMPI.ctx
         ctx._stream()
-        prec = _lib.R32 if fp32 else _lib.F64
+        f64, u64 = F.dtype == torch.float64, u.dtype == torch.float64
+        prec = {(True, True): _lib.F64, (False, False): _lib.R32,
+                (False, True): _lib.RK_F32_U64, (True, False): _lib.RK_F64_U32}[(f64, u64)]
         if torch.is_tensor(dt):
             # dt in device memory (device_scalars): read by the kernel
             dt64 = dt.to(device=u.device, dtype=torch.float64).reshape(())
             _lib.check(ctx._L.pyr_rk_update_dt(ctx._h, prec, self.npts, F.data_ptr(), PHI[U].data_ptr(),
-                                               u.data_ptr(), optr, shadow, dt64.data_ptr(),
+                                               uin, optr, shadow, dt64.data_ptr(),
                                                float(A), float(B)))
         else:
             _lib.check(ctx._L.pyr_rk_update(ctx._h, prec, self.npts, F.data_ptr(), PHI[U].data_ptr(),
-                                            u.data_ptr(), optr, shadow, float(dt), float(A),
+                                            uin, optr, shadow, float(dt), float(A),
                                             float(B)))
         return out
 

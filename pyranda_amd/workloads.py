@@ -188,7 +188,7 @@ rhoM = (rho1-rho2)/2.0
 
 
 def kh_setup(pr, nx, ny=None, vmean=0.0, precision="fp64", name="KelvinHelmholtz", device=None,
-             device_scalars=False):
+             device_scalars=False, wide_fields="auto"):
     """A KH.py simulation on an nx x ny grid (2-D, doubly periodic, the
     script's unit box with spacing 1/n), returned after setIC. The script
     uses a square grid (KH.py:52-55); ny != nx gives the uneven grids of an
@@ -202,7 +202,8 @@ def kh_setup(pr, nx, ny=None, vmean=0.0, precision="fp64", name="KelvinHelmholtz
     ydom = (0.0, LY,  NY, periodic=True)
     """.replace("LX", str(1.0 * (nx - 1.0) / nx)).replace("LY", str(1.0 * (ny - 1.0) / ny)) \
        .replace("NX", str(nx)).replace("NY", str(ny))
-    kw = {"silent": True, "precision": precision, "device_scalars": device_scalars}
+    kw = {"silent": True, "precision": precision, "device_scalars": device_scalars,
+          "wide_fields": wide_fields}
     if device is not None:
         kw["device"] = device
     ss = pr.pyrandaSim(name, mesh, **kw)
@@ -253,11 +254,12 @@ def ensemble_sizes(n_members=64, lo=128, hi=512, seed=2024):
 
 def kh_ensemble(pr, sizes, precision="fp32", device=None):
     """KH members on the given grids, each with its own seeded-free IC (the
-    script's), plus their initial driver state {time, dt}."""
+    script's), plus their initial driver state {time, dt}. Members store every
+    field in the run's precision, as a group does (wide_fields=None)."""
     sims, state = [], []
     for i, (nx, ny) in enumerate(sizes):
         ss = kh_setup(pr, nx, ny, precision=precision, name="KH-member-%d" % i, device=device,
-                      device_scalars=True)
+                      device_scalars=True, wide_fields=None)
         dt = ss.var("dt").data
         dt = dt if torch.is_tensor(dt) else torch.tensor(float(dt), dtype=torch.float64,
                                                          device=ss.PyMPI.device)

@@ -257,3 +257,24 @@ def test_group_operands_and_segmented_reductions_compile(amd):
     assert blk[0] == 0 and seg.nblocks == blk[-1] and all(b > a for a, b in zip(blk, blk[1:]))
     shape = fuse.GroupShape(75136, torch.zeros(75136, dtype=torch.int32), 3)
     assert tuple(shape) == (75136, 1, 1) and shape.nmem == 3
+
+
+def test_offset_dominated_rule():
+    """FP32 runs store a field in FP64 when max|v| > 8 (max v - min v)."""
+    import torch
+    from pyranda_amd.fuse import offset_dominated, WIDE_RATIO
+    x = torch.linspace(-1.0, 1.0, 1000, dtype=torch.float64)
+    p = 100.0 + 0.05 * torch.sin(7 * x)          # a low-Mach pressure
+    assert offset_dominated(p)
+    assert offset_dominated(torch.full((10,), 1.0))          # constant nonzero
+    assert not offset_dominated(torch.zeros(10))              # zero field
+    assert not offset_dominated(torch.sin(7 * x))             # sign-changing
+    assert not offset_dominated(1.0 + x)                      # range [0, 2]
+    assert not offset_dominated(torch.tensor([1.0, float("nan")]))
+    assert not offset_dominated(torch.tensor([1.0, float("inf")]))
+    # the threshold itself: max|v| = 9, range 1 is in; 8 and 1 is out
+    assert offset_dominated(torch.tensor([8.0, 9.0])) == (9.0 > WIDE_RATIO * 1.0)
+    assert not offset_dominated(torch.tensor([7.0, 8.0]))
+    # FP32 and permuted (Fortran-order) views decide the same way
+    f = (100.0 + torch.rand(4, 5, 6)).permute(2, 1, 0)
+    assert offset_dominated(f) and offset_dominated(f.float())
