@@ -143,6 +143,19 @@ int pyr_divergence_r32(pyr_ctx* ctx, const float* fx, const float* fy, const flo
 int pyr_apply(pyr_ctx* ctx, int op, int dir, int prec, const void* in, void* out, int epi,
               double epi_scale);
 
+/* --- fbar that allocates nothing -----------------------------------------
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc. The filter of pyr_sfilter_f64 (prec PYR_F64) or
+ * pyr_sfilter_r32 (prec PYR_R32), results identical, with the field
+ * temporary of its x -> y -> z sweeps in a file-scope device array of 64 MiB
+ * (one per GPU, held until the process ends; its address is taken at
+ * pyr_setup): fields of at most 64 MiB (8,388,608 FP64 or 16,777,216 FP32
+ * points, e.g. 256^3 in FP32); larger fields return an error and launch
+ * nothing. Every call on a GPU uses that GPU's one array: calls that overlap
+ * on one GPU (other streams, other threads) race. */
+int pyr_sfilter_noalloc(pyr_ctx* ctx, int prec, const void* val, void* dval);
+
 /* --- FP64 operators that also store the FP32 rounding of their result ----
  * For FP32 runs that keep an FP64 state through the conservative filter
  * (the state is filtered in FP64, the field is its rounding). No counterpart

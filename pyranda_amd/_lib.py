@@ -66,6 +66,7 @@ SIGNATURES = {
     "pyr_set_solve_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pyr_sfilter_f64_rounded": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "pyr_apply_rounded": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]),
+    "pyr_sfilter_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
 }
 PW_MAX_PTRS, PW_MAX_SCALARS = 24, 16
 MLP_PARAM_FLOAT_OFFSET, MLP_PARAM_BYTES, MLP_GRAD_FLOATS = 14336, 14960, 7313

@@ -838,6 +838,7 @@ struct pyr_ctx {
       }
   }
   int ncu = 256;                             // compute units of the device
+  void* noalloc_tmp = nullptr;               // this device's copy of `scratch` (pyr_sfilter_noalloc)
   float* out32 = nullptr;                    // FP32 copy of the next FP64 y/z sweep's output
   size_t npts() const { return (size_t)n[0] * n[1] * n[2]; }
 };
@@ -1270,6 +1271,34 @@ static int sum3(pyr_ctx* c, int op, int prec, const void* f0, const void* f1, co
 }
 
 // ---------------------------------------------------------------------------
+// device memory of the entry point that allocates nothing
+// ---------------------------------------------------------------------------
+// pyr_sfilter_noalloc keeps the field temporary of the x -> y -> z filter in
+// this file-scope array (one copy per GPU, resident until the process ends)
+// instead of allocating it. Host code never passes `scratch` itself to a
+// kernel: its device address on each GPU comes from hipGetSymbolAddress at
+// the first pyr_setup on that GPU (noalloc_init) and is kept per device.
+constexpr size_t PYR_NOALLOC_BYTES = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[PYR_NOALLOC_BYTES];
+constexpr int PYR_MAX_DEVICES = 64;
+static std::mutex g_noalloc_mu;
+static void* g_noalloc_ptr[PYR_MAX_DEVICES] = {};
+
+static int noalloc_init(int dev, void** ptr) {
+  if (dev < 0 || dev >= PYR_MAX_DEVICES) return fail("device ordinal out of range for the filter scratch");
+  std::lock_guard<std::mutex> lock(g_noalloc_mu);
+  if (!g_noalloc_ptr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    g_noalloc_ptr[dev] = p;
+  }
+  *ptr = g_noalloc_ptr[dev];
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
 // setup
 // ---------------------------------------------------------------------------
 static int setup_common(pyr_ctx** out, int nx, int ny, int nz, double x1, double xn, double y1,
@@ -1321,10 +1350,12 @@ static int setup_common(pyr_ctx** out, int nx, int ny, int nz, double x1, double
   }
   if (device) {
     int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
     hipDeviceProp_t prop;
-    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess &&
-        prop.multiProcessorCount > 0)
+    if (hipGetDeviceProperties(&prop, dev) == hipSuccess && prop.multiProcessorCount > 0)
       c->ncu = prop.multiProcessorCount;
+    // once per GPU, with that GPU current: its copy of the scratch array
+    if (noalloc_init(dev, &c->noalloc_tmp)) return 1;
     for (int op = 0; op < NOPS; ++op)
       for (int d = 0; d < 3; ++d) {
         if (upload_plan(c->plan[op][d], c->dplan[op][d])) return 1;
@@ -1894,6 +1925,19 @@ int pyr_sfilter_f64_rounded(pyr_ctx* c, const double* v, double* o, float* o32) 
   return r;
 }
 int pyr_sfilter_r32(pyr_ctx* c, const float* v, float* o) { return filter3(c, OP_SF, PYR_R32, v, o); }
+int pyr_sfilter_noalloc(pyr_ctx* c, int prec, const void* v, void* o) {
+  // pyr_sfilter_f64/_r32 with the field temporary in `scratch` (this device's
+  // copy, from pyr_setup): no allocation; overlapping calls on one GPU race
+  if (check_ctx(c)) return 1;
+  if (prec != PYR_F64 && prec != PYR_R32) return fail("bad precision");
+  if (!v || !o) return fail("null field pointer");
+  const size_t bytes = c->npts() * (prec == PYR_F64 ? sizeof(double) : sizeof(float));
+  if (bytes > PYR_NOALLOC_BYTES)
+    return fail("pyr_sfilter_noalloc: the field needs " + std::to_string(bytes) +
+                " bytes of temporary, above the 64 MiB scratch (use pyr_sfilter_f64/_r32)");
+  if (!c->noalloc_tmp) return fail("pyr_sfilter_noalloc: no scratch address for this context's device");
+  return filter3_with(c, OP_SF, prec, v, o, c->noalloc_tmp);
+}
 int pyr_gfilter_f64(pyr_ctx* c, const double* v, double* o) { return filter3(c, OP_GF, PYR_F64, v, o); }
 int pyr_gfilter_r32(pyr_ctx* c, const float* v, float* o) { return filter3(c, OP_GF, PYR_R32, v, o); }
 int pyr_gfilterdir_f64(pyr_ctx* c, const double* v, double* o, int dir) {
