@@ -903,7 +903,10 @@ class PointwiseRun:
     """Consecutive pointwise equations `lhs_i = rhs_i` as one kernel: reads
     each outside variable once, writes every lhs. A value produced inside the
     run is rounded to its field type before later lines use it, so the results
-    are those of evaluating the lines one after another."""
+    are those of evaluating the lines one after another; in an FP32 run a
+    line whose result is stored in FP64 reads earlier lines' values unrounded
+    (and conserved variables through their FP64 state), as an FP64 run
+    would."""
 
     def __init__(self, lines, tag=""):
         self.lines = lines  # [(lhs, src)]
@@ -974,9 +977,16 @@ class PointwiseRun:
         for i, t in enumerate(sig):
             if t in ("T64", "T32"):
                 L.append("    const double x%d = (double)p%d[i];" % (i, i))
+        import re as _re
         for j, t in enumerate(types):
             ot = "double" if t == "f64" else "float"
-            L.append("    const %s r%d = (%s)(%s);" % (ot, j, ot, self.bodies[j]))
+            body = self.bodies[j]
+            if t == "f64":
+                # an FP64 result reads earlier lines' values unrounded (u<k>);
+                # FP32 results read them rounded to their field type (l<k>)
+                body = _re.sub(r"\bl(\d+)\b", r"u\1", body)
+            L.append("    const double u%d = (%s);" % (j, body))
+            L.append("    const %s r%d = (%s)u%d;" % (ot, j, ot, j))
             L.append("    o%d[i] = r%d;" % (j, j))
             L.append("    const double l%d = (double)r%d;" % (j, j))
         L.append("  }")
@@ -989,8 +999,17 @@ class PointwiseRun:
         shape = tuple(sim.mesh.shape)
         sig, fields, scalars = [], [], []
         dev = None
+        # a run with an FP64-stored result reads conserved variables through
+        # their FP64 state (pyrandaSim.fp64_state_of) rather than the FP32 field
+        named = getattr(sim, "named_storage", False)
+        wide_out = named and any(sim.storage_of(l) != "f32" for l in self.outputs)
+        state_of = getattr(sim, "fp64_state_of", None) if wide_out else None
         for nm in self.inputs:
             v = sim.variables[nm].data
+            if state_of is not None:
+                v64 = state_of(nm)
+                if v64 is not None:
+                    v = v64
             if torch.is_tensor(v) and v.dim() == 0 and not _device_scalar(v):
                 v = v.item()
             if isinstance(v, (int, float)) and not isinstance(v, bool):

@@ -316,7 +316,10 @@ class ParcopContext:
         for d in range(3):
             for comp, f in enumerate((vx, vy, vz)):
                 first = d == 0 and comp == 0
-                self.apply(_lib.OP_D8, d, f, out=out,
+                # the normal component takes the antisymmetric member at SYMM
+                # ends (ringx(f,..,isymX) etc., operators.f90:661-671)
+                op = _lib.OP_D8 | (_lib.OP_ANTISYM if comp == d else 0)
+                self.apply(op, d, f, out=out,
                            epi=_lib.EPI_ABS_SCALED if first else _lib.EPI_MAXABS_SCALED,
                            epi_scale=self.dmesh[d])
         return out

@@ -45,6 +45,8 @@ from .pyrandaMesh import pyrandaMesh, defaultMeshOptions
 from .pyrandaMPI import pyrandaMPI
 from .pyrandaUtils import fortran3d, findVar, splitLines  # noqa: F401
 from .pyrandaVar import pyrandaVar
+from .pyrandaIO import pyrandaIO
+from . import pyrandaIO as _io
 
 # 5-stage, 2N-storage fourth-order Runge-Kutta coefficients (Carpenter &
 # Kennedy 1994), as pyranda.py:760-779 uses them.
@@ -112,6 +114,10 @@ class pyrandaSim:
         self.time = 0.0
         self.cycle = 0
         self.deltat = 0.0
+        self.vizDumpHistory = []
+        self.PyIO = pyrandaIO(self.name)
+        self.eom = ""
+        self.ics = ""
         self._shadow = {}  # FP64 RK state of conserved variables (fp32 mode)
         self._state64 = {}  # persistent FP64 state of self-filtered conserved variables
         self._state_src = {}  # the FP32 field last derived from it
@@ -286,7 +292,9 @@ class pyrandaSim:
                 d = self.variables[U].data
                 if torch.is_tensor(d) and tuple(d.shape) == tuple(self.mesh.shape):
                     self._state64[U] = d.to(torch.float64)
-        self._cast_fields()
+        # storage decided by value only for fields the ICs set (the others
+        # hold their allocation's zeros until an ALG line writes them)
+        self._cast_fields(decide=set(names))
         for U in self._state64:
             self._mark_state(U)
         snans = self.checkForNan(sorted(set(names)))
@@ -297,17 +305,20 @@ class pyrandaSim:
         if snans:
             raise FloatingPointError("Found some nans in Update after init: %s" % snans)
 
-    def _cast_fields(self):
+    def _cast_fields(self, decide=None):
+        """Fields take the run's dtype (fp32 modes: their storage, see
+        storage_of, decided by value for the names in `decide`, default all);
+        device scalars (reductions kept on the GPU, one value or one per
+        ensemble member) stay FP64 like the host floats they stand for."""
         for name, var in self.variables.items():
             if name.startswith("mesh"):
                 continue
             d = var.data
-            # fields take the run's dtype (fp32 modes: their storage, see
-            # storage_of); device scalars (reductions kept on the GPU, one
-            # value or one per ensemble member) stay FP64 like the host
-            # floats they stand for
             if torch.is_tensor(d) and tuple(d.shape) == tuple(self.mesh.shape) and d.is_floating_point():
-                want = self._storage_dtype(name, d)
+                if decide is None or name in decide or self.storage_of(name) is not None:
+                    want = self._storage_dtype(name, d)
+                else:
+                    want = self.dtype
                 if name in self._state64 and want == torch.float64:
                     # an FP64 field with an FP64 state: the state is the field
                     var.data = self._state64[name]
@@ -345,6 +356,13 @@ class pyrandaSim:
         want = self._storage_dtype(name, d)
         if d.dtype != want:
             var.data = d.to(want)
+
+    def fp64_state_of(self, name):
+        """The FP64 state of a conserved variable of an fp32 run whose field is
+        its FP32 rounding (None otherwise)."""
+        if self.dtype == torch.float64 or name not in self._state64 or self._wide(name):
+            return None
+        return self._state64[name] if self._state_current(name) else None
 
     def _wide(self, name):
         return self.dtype != torch.float64 and self._storage.get(name) == "f64"
@@ -617,6 +635,85 @@ class pyrandaSim:
                                             float(B)))
         return out
 
+    # -- restart and viz files (pyranda.py:431-588) -------------------------------
+    def _value_of(self, name):
+        """FP64 value of a variable as a restart stores it: the FP64 state of
+        a conserved variable of an fp32 run when it is current, else the
+        field (upcast)."""
+        if name in self._state64 and self._state_current(name):
+            return self._state64[name]
+        return self.variables[name].data
+
+    def writeRestart(self, ivars=None, suffix=None, wVars=[]):
+        """Write the simulation state to <rootname>/restart<suffix> (suffix
+        default: _<cycle:07d>) as pyranda.py:475-588 does: serial.dat and
+        proc-00000.bin with every variable (sorted by name) in FP64. ivars:
+        local values to keep (strings, numbers, lists, dicts, booleans, None;
+        numpy arrays go to <name>.npy)."""
+        import os
+        import numpy as np
+        if not wVars:
+            wVars = list(self.variables.keys())
+        if not suffix:
+            suffix = "_" + str(self.cycle).zfill(_io.RESTART_INT)
+        self.PyIO.ensure_root()
+        dump = os.path.join(self.PyIO.rootname, "restart" + suffix)
+        os.makedirs(dump, exist_ok=True)
+        serial = {}
+        if ivars:
+            serial["local_vars"] = {"numpyArrays": {}}
+            for vv, val in ivars.items():
+                if isinstance(val, np.ndarray):
+                    fn = os.path.join(dump, "%s" % vv)
+                    serial["local_vars"]["numpyArrays"][vv] = fn
+                    np.save(fn, val, allow_pickle=False)
+                elif val is None or isinstance(val, (str, int, float, bool, list, dict, np.floating)):
+                    serial["local_vars"][vv] = val
+        mesh = dict(self.meshOptions)
+        if mesh.get("function"):
+            mesh["function"] = None
+        if "comm" in mesh:
+            mesh["comm"] = "user specified"
+        n = (self.nx, self.ny, self.nz)
+        serial.update({
+            "mesh": mesh, "EOM": self.eom, "ICs": self.ics, "decomp": [1, 1, 1],
+            "procMap": {"0-g1": np.array([0, 0, 0], dtype=np.int32),
+                        "0-gn": np.array([n[0] - 1, n[1] - 1, n[2] - 1], dtype=np.int32)},
+            "packages": list(self.packagesRestart), "time": float(self.time),
+            "deltat": float(self.deltat) if not torch.is_tensor(self.deltat) else float(self.deltat.item()),
+            "cycle": int(self.cycle), "vizDumpHistory": list(self.vizDumpHistory), "sMap": dict(self.sMap),
+        })
+        names = sorted(wVars)
+        serial["vars"] = {v: i for i, v in enumerate(names)}
+        # this build's own keys (other readers ignore them): the storage of an
+        # fp32 run's fields
+        serial["storage"] = dict(self._storage)
+        _io.write_serial(os.path.join(dump, "serial.dat"), serial)
+        cols = [_io._host_field(self, self._value_of(v), n) for v in names]
+        _io.write_proc_bin(os.path.join(dump, "proc-%s.bin" % str(0).zfill(5)), cols, n)
+        return dump
+
+    def write(self, wVars=[]):
+        """Viz dump of the conserved variables (or wVars): the legacy binary VTK
+        file and pyranda.visit of pyranda.py:431-468 / pyrandaIO.py:115-217."""
+        import os
+        if not wVars:
+            wVars = list(self.conserved)
+        self.PyIO.ensure_root()
+        dname = "vis" + str(self.cycle).zfill(_io.VIS_INT)
+        os.makedirs(os.path.join(self.PyIO.rootname, dname), exist_ok=True)
+        fname = os.path.join(self.PyIO.rootname, dname,
+                             "proc-%s.%s.vtk" % (str(0).zfill(_io.PROC_INT), str(self.cycle).zfill(_io.VIS_INT)))
+        n = (self.nx, self.ny, self.nz)
+        coords = [_io._host_field(self, c.data, n) for c in self.mesh.coords]
+        fields = [_io._host_field(self, self.variables[v].data, n) for v in wVars]
+        fields = [f if not isinstance(f, float) else __import__("numpy").full(n, f) for f in fields]
+        t = float(self.time.item()) if torch.is_tensor(self.time) else float(self.time)
+        _io.write_vtk_binary(fname, coords, fields, list(wVars), int(self.cycle), t)
+        self.vizDumpHistory.append([self.cycle, t])
+        _io.write_visit(os.path.join(self.PyIO.rootname, "pyranda.visit"), self.vizDumpHistory)
+        return fname
+
     def step(self, nsteps=1):
         for _ in range(nsteps):
             self.rk4(self.time, self.deltat)
@@ -761,3 +858,90 @@ class pyrandaSim:
         sMap["meshj"] = "self.mesh.indices[1].data"
         sMap["meshk"] = "self.mesh.indices[2].data"
         self.sMap = sMap
+
+
+def pyrandaRestart(rootname, suffix=None, comm=None, **sim_options):
+    """A pyrandaSim rebuilt from a restart directory (pyranda.py:979-1080):
+    the latest restart* under rootname, or restart_<suffix>. Returns
+    [pysim, local_vars]. sim_options (precision, device, ...) go to
+    pyrandaSim; the fields are read back as they were written (FP64) and
+    take the run's storage."""
+    import glob
+    import os
+    import numpy as np
+    if not suffix:
+        dumps = sorted(glob.glob(os.path.join(rootname, "restart*")))
+        if not dumps:
+            raise FileNotFoundError("no restart directory under %s" % rootname)
+        dump = dumps[-1]
+    else:
+        dump = os.path.join(rootname, "restart_%s" % suffix)
+    if not os.path.isdir(dump):
+        raise FileNotFoundError("Error: Cant read restart file %s" % dump)
+    serial = _io.read_serial(os.path.join(dump, "serial.dat"))
+    local_vars = {}
+    if "local_vars" in serial:
+        lv = serial.pop("local_vars")
+        arrays = lv.pop("numpyArrays", {})
+        local_vars.update(lv)
+        for na, fn in arrays.items():
+            local_vars[na] = np.load(fn + ".npy", allow_pickle=False)
+    mesh = dict(serial["mesh"])
+    if comm is not None:
+        mesh["comm"] = comm
+    elif mesh.get("comm") == "user specified":
+        mesh.pop("comm")
+    pysim = pyrandaSim(rootname, mesh, **sim_options)
+    from .pyrandaBC import pyrandaBC as _BC
+    from .pyrandaTimestep import pyrandaTimestep as _TS
+    known = {"pyrandaTimestep": _TS, "pyrandaBC": _BC}
+    for pack in serial["packages"]:
+        name = pack.split(".")[1] if "." in pack else pack
+        if name not in known:
+            raise ValueError("restart names package %s, which this build does not have" % pack)
+        pysim.addPackage(known[name](pysim))
+    pysim.sMap = dict(serial["sMap"])
+    if "storage" in serial and pysim.dtype != torch.float64:
+        pysim._storage.update(serial["storage"])
+    pysim.EOM(serial["EOM"])
+    pysim.setIC(serial["ICs"], addOnly=True)
+    pysim.time = serial["time"]
+    pysim.deltat = serial["deltat"]
+    pysim.cycle = serial["cycle"]
+    pysim.vizDumpHistory = list(serial["vizDumpHistory"])
+    names = sorted(serial["vars"], key=lambda v: serial["vars"][v])
+    n = (pysim.nx, pysim.ny, pysim.nz)
+    decomp = [int(v) for v in serial["decomp"]]
+    if decomp != [1, 1, 1]:
+        raise ValueError("restart written by %s ranks; this build reads single-rank restarts" % decomp)
+    mm = _io.open_proc_bin(os.path.join(dump, "proc-%s.bin" % str(0).zfill(5)), n, len(names))
+    dev = pysim.PyMPI.device
+    for v in names:
+        col = serial["vars"][v]
+        if v not in pysim.variables:
+            pysim.addVar(v, kind="conserved")
+            pysim.variables[v].__allocate__(pysim.PyMPI)
+        var = pysim.variables[v]
+        if v.startswith("mesh"):
+            continue  # the mesh comes from the mesh options (Cartesian)
+        if torch.is_tensor(var.data) and var.data.dim() == 3:
+            host = np.ascontiguousarray(np.asarray(mm[:, :, :, col]).transpose(2, 1, 0))
+            var.data = torch.from_numpy(host).to(dev).permute(2, 1, 0)
+        else:
+            var.data = float(mm[0, 0, 0, col])
+    del mm
+    if pysim.dtype != torch.float64:
+        # the FP64 state of self-filtered conserved variables is the stored
+        # value; fields take their storage
+        pysim._state64 = {}
+        for U in pysim._self_filtered():
+            d = pysim.variables[U].data
+            if torch.is_tensor(d) and tuple(d.shape) == tuple(pysim.mesh.shape):
+                pysim._state64[U] = d
+        pysim._cast_fields()
+        for U in pysim._state64:
+            pysim._mark_state(U)
+    for pack in pysim.packages.values():
+        pack.__init__(pysim)
+    pysim.iprint("Successful read of restart file: %s " % dump)
+    return [pysim, local_vars]

@@ -340,9 +340,10 @@ def test_fp32_fp64_state_through_filter(pr):
     in place is taken as the new state."""
     import torch
     from pyranda_amd.workloads import kh_setup
+    # every field in FP32 storage (wide_fields=None): the state mechanism alone
     runs = {"fp64": kh_setup(pr, 64, 48, precision="fp64"),
-            "state": kh_setup(pr, 64, 48, precision="fp32"),
-            "rounded": kh_setup(pr, 64, 48, precision="fp32")}
+            "state": kh_setup(pr, 64, 48, precision="fp32", wide_fields=None),
+            "rounded": kh_setup(pr, 64, 48, precision="fp32", wide_fields=None)}
     runs["rounded"].fp64_state = False
     st = runs["state"]
     assert sorted(st._self_filtered()) == ["Et", "rho", "rhou", "rhov"]
@@ -366,3 +367,31 @@ def test_fp32_fp64_state_through_filter(pr):
     st.rk4(t, dt)
     assert st._state_current("rho")
     assert torch.equal(st.var("rho").data, st._state64["rho"].float())
+
+
+def test_fp32_offset_dominated_fields_stored_fp64(pr):
+    """FP32 KH run with storage by value (the default): the fields set close
+    to one large value (pressure 2.5, total energy) are stored in FP64, the
+    FP64 state of a filtered FP64 field is the field itself, and the run ends
+    closer to the FP64 run than with every field in FP32."""
+    import torch
+    from pyranda_amd.workloads import kh_setup
+    runs = {"fp64": kh_setup(pr, 64, 48, precision="fp64"),
+            "wide": kh_setup(pr, 64, 48, precision="fp32"),
+            "f32": kh_setup(pr, 64, 48, precision="fp32", wide_fields=None)}
+    w = runs["wide"]
+    wide = sorted(k for k, v in w._storage.items() if v == "f64")
+    _say("FP64 storage:", wide)
+    assert "p" in wide and "Et" in wide and "u" not in wide
+    assert w.var("p").data.dtype == torch.float64 and w.var("u").data.dtype == torch.float32
+    dt = float(runs["fp64"].var("dt").data) * 0.5
+    for s in runs.values():
+        t = 0.0
+        for _ in range(16):
+            t = s.rk4(t, dt)
+    torch.cuda.synchronize()
+    assert w.var("Et").data is w._state64["Et"]
+    ref = runs["fp64"].var("rhou").data
+    err = {n: float((runs[n].var("rhou").data.double() - ref).norm() / ref.norm()) for n in ("wide", "f32")}
+    _say("rhou rel l2 vs fp64", err)
+    assert err["wide"] < err["f32"]
