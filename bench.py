@@ -72,8 +72,8 @@ def make_tg(n, precision, p0, solver="circulant"):
     import pyranda_amd as pr
     from pyranda_amd.workloads import TG_EOM, TG_IC
     rounded = precision == "fp32-rounded-state"
-    base = "fp32" if precision in ("fp32-rounded-state", "fp32-refined-only", "fp32-all-f32") else precision
-    wide = None if precision == "fp32-all-f32" else "auto"
+    base = "fp32" if precision in ("fp32-rounded-state", "fp32-offset", "fp32-all-f32") else precision
+    wide = {"fp32-all-f32": None, "fp32-offset": "offset"}.get(precision, "auto")
     ss = pr.pyrandaSim("TGvortex", tg_mesh(n), silent=True, precision=base, fp64_state=not rounded,
                        wide_fields=wide)
     ss.PyMPI.ctx.set_periodic_solver(solver)
@@ -442,11 +442,13 @@ def main():
                     help="solver of periodic implicit lines")
     ap.add_argument("--r32-solve", default="refined", choices=["refined", "fp64"],
                     help="--ops: arithmetic of the solves on FP32 fields")
-    ap.add_argument("--precision", default="fp64,fp32,fp32-fp64solve",
+    ap.add_argument("--precision", default="fp64,fp32,fp32-offset",
                     help="comma-separated pyrandaSim precisions to time (the headline is fp32: FP32 "
-                         "sweeps plus one FP64-residual refinement in every operator; fp32-fp64solve "
-                         "solves in FP64 on the same FP32 fields; fp32-rounded-state is fp32 without "
-                         "the FP64 state through the filter; fp32-all-f32 stores every field in FP32)")
+                         "storage where it keeps 1e-6 (pyrandaSim wide_fields='auto'), FP32 sweeps plus "
+                         "one FP64-residual refinement in every FP32 operator; fp32-offset stores only the "
+                         "offset-dominated fields in FP64 (faster, less accurate); fp32-fp64solve solves "
+                         "in FP64 on FP32 fields; fp32-rounded-state is fp32 without the FP64 state through "
+                         "the filter; fp32-all-f32 stores every field in FP32)")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch shapes (no per-operator autotuning before timing)")
     ap.add_argument("--ensemble", action="store_true",
@@ -525,7 +527,8 @@ def main():
                               "pups": n ** 3 * args.steps / el, "kernels": summ}
         states[precision] = {k: ss.variables[k].data.clone() for k in CONSERVED}
         if precision == "fp32":
-            wide = sorted(k for k, v in ss._storage.items() if v == "f64")
+            wide = {"f64": sorted(k for k, v in ss._storage.items() if v == "f64"),
+                    "f32": sorted(k for k, v in ss._storage.items() if v == "f32")}
         print("[bench] %s %.3f ms/step, %.3e point-updates/s" %
               (precision, 1e3 * el / args.steps, n ** 3 * args.steps / el), file=sys.stderr, flush=True)
         del ss
@@ -584,13 +587,16 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32 fields (offset-dominated fields in f64); f32 sweeps plus one f64-residual "
-                 "refinement in every operator; f64 RK4 state",
+        "dtype": "f32 storage for operator results (derivatives, LAD ring and filters) and what is "
+                 "computed from them, f64 storage for offset-dominated fields, for values computed from "
+                 "f64 values only and for flux arguments with f64 inputs; f32 sweeps plus one "
+                 "f64-residual refinement in every operator on f32 fields, f64 sweeps on f64 fields; "
+                 "f64 RK4 state",
         "data": "synthetic (Taylor-Green initial condition)",
         "config": {"workload": "taylor_green_euler_lad_%d^3" % n, "grid": [n, n, n], "gamma": gamma,
                    "mach": 0.1, "cfl": 0.5, "dt": dt, "rk_stages": 5, "lad": True,
                    "periodic_solver": args.solver, "autotune": tuned.get("fp32"),
-                   "fp64_fields": wide},
+                   "storage": wide},
         "fp64": ({"value": results["fp64"]["pups"], "ms_per_step": results["fp64"]["ms_per_step"]}
                  if "fp64" in results else None),
         "speedup_fp32r_over_fp64": (head["pups"] / results["fp64"]["pups"]) if "fp64" in results else None,

@@ -112,56 +112,93 @@ __device__ __forceinline__ Tabs stage_tables(const LineArgs& a, unsigned char* d
   return t;
 }
 
+// The tables as the solve reads them from global memory (lines whose tables
+// exceed the LDS budget, grouped launches).
+template <int MODE>
+__device__ __forceinline__ Tabs global_tables(const LineArgs& a) {
+  Tabs t;
+  t.lu = a.lu;
+  t.rc = a.rc;
+  t.gsel = a.gsel;
+  t.lu32 = a.lu32;
+  t.rc32 = a.rc32;
+  t.gsel32 = a.gsel32;
+  t.band = a.band;
+  return t;
+}
+
+template <int MODE>
+__device__ __forceinline__ Tabs get_tables(const LineArgs& a, unsigned char* dst, int tid, int nthr) {
+  return a.gtab ? global_tables<MODE>(a) : stage_tables<MODE>(a, dst, tid, nthr);
+}
+
+// staged tables above this size are read from global memory instead
+constexpr size_t STAGE_MAX_BYTES = 64 * 1024;
+
 enum Solve { SOLVE_NONE = 0, SOLVE_LU = 1, SOLVE_CIRC = 2 };
 
 // Solve one chunk given its RHS with the chunked LU; the LDS exchange of edge
 // values goes through Y (4P x ly entries of the solve type), column `col`
 // belongs to this line. The solution replaces the RHS in `rhs`.
+// act: the thread's chunk exists (a grouped launch shapes its workgroups for
+// the member with the most chunks; other threads only take the barriers).
 template <int MODE, typename CS>
 __device__ __forceinline__ void solve_chunk_lu(const LineArgs& a, const Tabs& t, int p, int s, int m,
-                                               CS* Y, int ly, int col, double* rhs) {
+                                               CS* Y, int ly, int col, double* rhs, bool act = true) {
   auto yr = [&](int c) { return Y[c * ly + col]; };
   if (MODE != MODE_R32) {
     double* x = rhs;  // solve in place
-    local_solve<double>(t.lu, s, m, x);
     double e4[4];
-    edges(x, m, e4);
+    if (act) {
+      local_solve<double>(t.lu, s, m, x);
+      edges(x, m, e4);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) Y[(4 * p + q) * ly + col] = (CS)e4[q];
+      for (int q = 0; q < 4; ++q) Y[(4 * p + q) * ly + col] = (CS)e4[q];
+    }
     __syncthreads();
-    double z[4];
-    neighbours<double>(t.gsel, p, a.P, yr, z);
-    spike_correct(t.rc, s, m, z, x);
+    if (act) {
+      double z[4];
+      neighbours<double>(t.gsel, p, a.P, yr, z);
+      spike_correct(t.rc, s, m, z, x);
+    }
   } else {
     float x[CHUNK];
+    float z[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
     for (int k = 0; k < CHUNK; ++k) x[k] = (float)rhs[k];
-    local_solve<float>(t.lu32, s, m, x);
-    float e4[4];
-    edges(x, m, e4);
+    if (act) {
+      local_solve<float>(t.lu32, s, m, x);
+      float e4[4];
+      edges(x, m, e4);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) Y[(4 * p + q) * ly + col] = (CS)e4[q];
+      for (int q = 0; q < 4; ++q) Y[(4 * p + q) * ly + col] = (CS)e4[q];
+    }
     __syncthreads();
-    float z[4];
-    neighbours<float>(t.gsel32, p, a.P, yr, z);
-    spike_correct(t.rc32, s, m, z, x);
-    // one refinement step: FP64 residual against the same banded operator
-    residual64([&](int k) { return t.band + 5 * (s + k); }, m, rhs, x, z, rhs);
     float d[CHUNK];
+    if (act) {
+      neighbours<float>(t.gsel32, p, a.P, yr, z);
+      spike_correct(t.rc32, s, m, z, x);
+      // one refinement step: FP64 residual against the same banded operator
+      residual64([&](int k) { return t.band + 5 * (s + k); }, m, rhs, x, z, rhs);
 #pragma unroll
-    for (int k = 0; k < CHUNK; ++k) d[k] = (float)rhs[k];
-    local_solve<float>(t.lu32, s, m, d);
-    float f4[4];
-    edges(d, m, f4);
+      for (int k = 0; k < CHUNK; ++k) d[k] = (float)rhs[k];
+      local_solve<float>(t.lu32, s, m, d);
+    }
     __syncthreads();
+    if (act) {
+      float f4[4];
+      edges(d, m, f4);
 #pragma unroll
-    for (int q = 0; q < 4; ++q) Y[(4 * p + q) * ly + col] = (CS)f4[q];
+      for (int q = 0; q < 4; ++q) Y[(4 * p + q) * ly + col] = (CS)f4[q];
+    }
     __syncthreads();
-    float zd[4];
-    neighbours<float>(t.gsel32, p, a.P, yr, zd);
-    spike_correct(t.rc32, s, m, zd, d);
+    if (act) {
+      float zd[4];
+      neighbours<float>(t.gsel32, p, a.P, yr, zd);
+      spike_correct(t.rc32, s, m, zd, d);
 #pragma unroll
-    for (int k = 0; k < CHUNK; ++k) rhs[k] = (double)x[k] + (double)d[k];
+      for (int k = 0; k < CHUNK; ++k) rhs[k] = (double)x[k] + (double)d[k];
+    }
   }
 }
 
@@ -328,7 +365,7 @@ __device__ __forceinline__ void line_yz_body(const LineArgs& a, long long blk, c
   const int nthr = LY * blockDim.y;
   CS* Y = reinterpret_cast<CS*>(smem);
   Tabs t;
-  if (SOLVE == SOLVE_LU) t = stage_tables<MODE>(a, smem + align16((size_t)4 * a.P * LY * sizeof(CS)), tid, nthr);
+  if (SOLVE == SOLVE_LU) t = get_tables<MODE>(a, smem + align16((size_t)4 * a.P * LY * sizeof(CS)), tid, nthr);
   const long long L = blk * LY + lane;
   const bool valid = L < a.nlines;
   const long long Lc = valid ? L : (long long)a.nlines - 1;
@@ -412,7 +449,7 @@ __device__ __forceinline__ void line_yz_body(const LineArgs& a, long long blk, c
   } else {
     if (SOLVE == SOLVE_LU) {
       __syncthreads();  // staged tables visible
-      solve_chunk_lu<MODE, CS>(a, t, p, s, m, Y, LY, lane, rhs);
+      solve_chunk_lu<MODE, CS>(a, t, p, s, m, Y, LY, lane, rhs, act);
     }
 #pragma unroll
     for (int k = 0; k < CHUNK; ++k)
@@ -636,9 +673,9 @@ __device__ __forceinline__ void line_x_body(const LineArgs& a, long long blk, co
   if (SOLVE == SOLVE_LU) {
     __syncthreads();  // the tile is consumed; reuse its LDS
     CS* Y = reinterpret_cast<CS*>(smem);
-    const Tabs t = stage_tables<MODE>(a, smem + align16((size_t)4 * a.P * LX * sizeof(CS)), tid, nthr);
+    const Tabs t = get_tables<MODE>(a, smem + align16((size_t)4 * a.P * LX * sizeof(CS)), tid, nthr);
     __syncthreads();
-    solve_chunk_lu<MODE, CS>(a, t, p, s, m, Y, LX, l, rhs);
+    solve_chunk_lu<MODE, CS>(a, t, p, s, m, Y, LX, l, rhs, act);
     __syncthreads();  // tables and edge values no longer read
   } else if (SOLVE == SOLVE_NONE) {
     __syncthreads();  // the tile is consumed
@@ -700,6 +737,104 @@ template <typename TS, int KIND, int SOLVE, int MODE, bool FULL>
 __global__ void __launch_bounds__(WG_MAX) __attribute__((amdgpu_waves_per_eu(PYR_X_WAVES_PER_EU, 8)))
 k_line_x(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
   line_x_body<TS, KIND, SOLVE, MODE, FULL>(a, blockIdx.x, in, out);
+}
+
+// x lines, software-pipelined (periodic lines of whole 32-row chunks whose
+// length is a whole number of KiB, circulant solve or explicit filter). A
+// persistent workgroup walks groups of LX lines; the next group's lines are
+// copied HBM -> LDS by global->LDS DMA into the second of two tiles while the
+// current group is solved from the first, and the results leave through the
+// current tile with coalesced vector stores. Tile row pitch: N + one 16-byte
+// vector, so the windows (lanes over lines) read conflict-free.
+template <typename TS, int KIND, int SOLVE, int MODE>
+__global__ void __launch_bounds__(WG_MAX) __attribute__((amdgpu_waves_per_eu(PYR_X_WAVES_PER_EU, 8)))
+k_line_x_pipe(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
+  constexpr int R = Halo<KIND>::R;
+  using CS = typename SolveT<MODE>::type;
+  using V = typename Vec16<TS>::type;
+  constexpr int VW = Vec16<TS>::W;
+  constexpr int EPK = 1024 / (int)sizeof(TS);  // elements of one wave's 1 KiB copy
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = a.N, LX = a.lx, pitch = a.pitch;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int l = tid % LX, p = tid / LX;
+  const int lane64 = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int nwaves = nthr >> 6;
+  const int tile_elems = LX * pitch;
+  TS* tiles = reinterpret_cast<TS*>(smem);
+  CS* X0 = reinterpret_cast<CS*>(smem + align16(2 * (size_t)tile_elems * sizeof(TS)));
+  CS* X1 = X0 + 4 * a.P * LX;
+  const int ngroups = (a.nlines + LX - 1) / LX;
+  const int ppl = N / EPK;  // 1 KiB pieces per line
+  auto issue = [&](int g, int t) {
+    const long long L0 = (long long)g * LX;
+    const long long rem = (long long)a.nlines - L0;
+    const int nval = (int)(rem < LX ? rem : LX);
+    const TS* src = in + L0 * N;
+    TS* dst = tiles + t * tile_elems;
+    for (int j = wave; j < nval * ppl; j += nwaves) {
+      const int ll = j / ppl, q = j - ll * ppl;
+      lds_copy16(src + (long long)ll * N + q * EPK + lane64 * VW, dst + ll * pitch + q * EPK);
+    }
+  };
+  const int s = p * CHUNK;
+  constexpr int m = CHUNK;
+  const bool need_old = (a.epi == EPI_ADD || a.epi == EPI_MAXABS);
+  int t = 0;
+  if ((int)blockIdx.x < ngroups) issue(blockIdx.x, 0);
+  for (int g = blockIdx.x; g < ngroups; g += gridDim.x, t ^= 1) {
+    // this wave's copies and last group's stores are done; the barrier makes
+    // every wave's copies visible and retires all reads of the other tile
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    __syncthreads();
+    const int gn = g + gridDim.x;
+    if (gn < ngroups) issue(gn, t ^ 1);
+    const long long L0 = (long long)g * LX;
+    const long long rem = (long long)a.nlines - L0;
+    const int nval = (int)(rem < LX ? rem : LX);
+    TS* tile = tiles + t * tile_elems;
+    const int lr = l < nval ? l : 0;
+    const TS* row0 = tile + lr * pitch;
+    TS win[CHUNK + 2 * R];
+#pragma unroll
+    for (int w = 0; w < CHUNK + 2 * R; ++w) {
+      int row = s - R + w;
+      row = row < 0 ? row + N : (row >= N ? row - N : row);
+      win[w] = row0[row];
+    }
+    double rhs[CHUNK];
+    rhs_interior<KIND>(a, win, rhs, m);
+    TS* rowo = tile + l * pitch;
+    auto emit = [&](int k, double y) {
+      if (l < nval) {
+        double val = y * a.scale;
+        if (a.addv) val += (double)rowo[s + k];
+        rowo[s + k] = (TS)val;
+      }
+    };
+    if constexpr (SOLVE == SOLVE_CIRC) {
+      // the first carry barrier also retires every window read of the tile
+      solve_chunk_circ<MODE, CS>(a, kmat<MODE>(a, 0), kmat<MODE>(a, 1), p, m, X0, X1, LX, l, rhs, emit);
+    } else {
+      __syncthreads();  // every window read before any result lands in the tile
+#pragma unroll
+      for (int k = 0; k < CHUNK; ++k) emit(k, rhs[k]);
+    }
+    __syncthreads();
+    const int vpl = N / VW, nvec = nval * vpl;
+    V* dv = reinterpret_cast<V*>(out + L0 * N);
+#pragma unroll XU
+    for (int e = tid; e < nvec; e += nthr) {
+      const int ll = e / vpl, i = e - ll * vpl;
+      V r = *reinterpret_cast<const V*>(tile + ll * pitch + i * VW);
+      if (a.epi != EPI_STORE) {
+        const V o = need_old ? dv[e] : V{};
+        vzip(r, o, [&](TS y, TS old) { return epilogue<TS>(a.epi, a.epi_c, (double)y, old); });
+      }
+      dv[e] = r;
+    }
+  }
 }
 
 // Grouped launches: one launch applies the same operator to the fields of
@@ -1032,6 +1167,7 @@ static int launch_yz_pipe(pyr_ctx* c, int pre, const LineArgs& a, const TS* in, 
       return 0;
     } else {
     if (pre == PRE_OLD) return 0;
+    if (LG * a.P > WG_MAX) return 0;
     // whole waves (the copy loop strides by the wave count)
     if (!c->yz_pipe || !a.periodic || a.N % CHUNK != 0 || (LG * a.P) % 64 != 0) return 0;
     if (a.A % LG != 0 || a.nlines % LG != 0) return 0;
@@ -1063,7 +1199,11 @@ template <typename TS, int KIND, int SOLVE, int MODE, bool FULL>
 static int launch_typed(pyr_ctx* c, int dir, LineArgs a, const TS* in, TS* out) {
   using CS = typename SolveT<MODE>::type;
   const int P = a.P;
-  const size_t stage = SOLVE == SOLVE_LU ? stage_bytes<MODE>(a.ntab, a.N) : 0;
+  // LU tables go to LDS unless they are too large (long lines): then the
+  // solve reads them from global memory (L2-resident, shared by all lines)
+  const size_t stage_full = SOLVE == SOLVE_LU ? stage_bytes<MODE>(a.ntab, a.N) : 0;
+  a.gtab = stage_full > STAGE_MAX_BYTES;
+  const size_t stage = a.gtab ? 0 : stage_full;
   if (dir == 0) {
     constexpr int VW = Vec16<TS>::W;
     a.vec = (a.N % VW == 0) && ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 16 == 0);
@@ -1077,6 +1217,37 @@ static int launch_typed(pyr_ctx* c, int dir, LineArgs a, const TS* in, TS* out) 
       }
       return tile;
     };
+    if constexpr (FULL && (SOLVE == SOLVE_CIRC || SOLVE == SOLVE_NONE)) {
+      // pipelined persistent kernel: two tiles of LX lines (16-byte aligned
+      // rows, whole KiB per line), LX * P threads in whole waves
+      const bool ok = c->yz_pipe && a.periodic && (a.N * (int)sizeof(TS)) % 1024 == 0 &&
+                      ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 16 == 0);
+      if (ok) {
+        const int pitch = a.N + VW;
+        auto lds2 = [&](int lx) {
+          size_t t = align16(2 * (size_t)lx * pitch * sizeof(TS));
+          if (SOLVE == SOLVE_CIRC) t += 2 * align16((size_t)4 * P * lx * sizeof(CS));
+          return t;
+        };
+        int lx = WG_MAX / P;
+        if (lx > 64) lx = 64;
+        while (lx > 1 && lds2(lx) > (size_t)c->x_lds_kb * 1024) lx /= 2;
+        if ((lx * P) % 64 == 0) {
+          a.lx = lx;
+          a.pitch = pitch;
+          a.vec = 1;
+          const size_t shm = lds2(lx);
+          const int ngroups = (a.nlines + lx - 1) / lx;
+          const int per_cu = std::max(1, (int)((160 * 1024) / shm));
+          const int nblocks = std::min(ngroups, c->ncu * per_cu);
+          auto k = k_line_x_pipe<TS, KIND, SOLVE, MODE>;
+          if (set_lds(k, shm)) return 1;
+          hipLaunchKernelGGL(k, dim3(nblocks), dim3(lx * P), shm, c->stream, a, in, out);
+          HIPCHECK(hipGetLastError());
+          return 0;
+        }
+      }
+    }
     int lx = WG_MAX / P;
     if (lx > 64) lx = 64;
     // keep the workgroup's LDS within the budget so several workgroups share a CU
@@ -1088,7 +1259,9 @@ static int launch_typed(pyr_ctx* c, int dir, LineArgs a, const TS* in, TS* out) 
     if (set_lds(k, shm)) return 1;
     hipLaunchKernelGGL(k, grid, block, shm, c->stream, a, in, out);
   } else {
-    const int ly = (P <= 8 && c->yz_lines != 32) ? 64 : 32;
+    // lines per workgroup: 64 x P chunk threads within WG_MAX (32 or 16 lines
+    // for longer lines)
+    const int ly = (P <= 8 && c->yz_lines != 32) ? 64 : (P <= WG_MAX / 32 ? 32 : 16);
     const size_t ex = align16((size_t)4 * P * ly * sizeof(CS));
     const size_t shm = SOLVE == SOLVE_CIRC ? 2 * ex : (SOLVE == SOLVE_LU ? ex + stage : 0);
     const dim3 grid((a.nlines + ly - 1) / ly), block(ly, P);
@@ -1106,8 +1279,10 @@ static int launch_typed(pyr_ctx* c, int dir, LineArgs a, const TS* in, TS* out) 
     }
     if (ly == 64) {
       if (launch_yz<TS, KIND, SOLVE, MODE, 64, FULL>(c, pre, grid, block, shm, a, in, out)) return 1;
-    } else {
+    } else if (ly == 32) {
       if (launch_yz<TS, KIND, SOLVE, MODE, 32, FULL>(c, pre, grid, block, shm, a, in, out)) return 1;
+    } else {
+      if (launch_yz<TS, KIND, SOLVE, MODE, 16, FULL>(c, pre, grid, block, shm, a, in, out)) return 1;
     }
   }
   HIPCHECK(hipGetLastError());
@@ -1618,37 +1793,46 @@ static int grp_build(pyr_group* g, const GrpKey& key, GrpPlan& gp) {
   if (ms.empty()) return 0;
   const OpPlan& pl0 = g->mem[ms[0]]->plan[key.op][key.dir];
   const int sv = plan_solve(g->mem[ms[0]], pl0);
-  if (sv == SOLVE_LU) return fail("grouped operators need periodic lines (circulant solve) or explicit filters");
-  GrpLaunch L;
-  L.kind = pl0.kind;
-  L.solve = sv;
-  L.x = key.dir == 0;
-  std::vector<GrpEntry> host(ms.size());
-  int Pmax = 1;
+  std::vector<GrpEntry> all(ms.size());
   for (size_t j = 0; j < ms.size(); ++j) {
     pyr_ctx* c = g->mem[ms[j]];
     const OpPlan& pl = c->plan[key.op][key.dir];
-    if (plan_solve(c, pl) != sv || pl.kind != L.kind)
+    if (plan_solve(c, pl) != sv || pl.kind != pl0.kind)
       return fail("group members differ in boundary conditions along a direction");
     const double ec = key.ec * std::pow(c->dmesh[key.dir], (double)key.power);
-    host[j].a = make_args(c, pl, c->dplan[key.op][key.dir], key.dir, key.epi, ec, key.prec);
-    host[j].off = g->off[ms[j]];
-    Pmax = std::max(Pmax, pl.P);
+    all[j].a = make_args(c, pl, c->dplan[key.op][key.dir], key.dir, key.epi, ec, key.prec);
+    // bounded lines: every member's LU tables are read from global memory
+    // (members differ in length; the workgroup's LDS holds the exchange only)
+    all[j].a.gtab = sv == SOLVE_LU;
+    all[j].off = g->off[ms[j]];
   }
-  long long blk = 0;
-  if (L.x) {
+  auto upload = [&](GrpLaunch& L, std::vector<GrpEntry>& host) -> int {
+    L.kind = pl0.kind;
+    L.solve = sv;
+    L.M = (int)host.size();
+    HIPCHECK(hipMalloc(&L.dev, host.size() * sizeof(GrpEntry)));
+    HIPCHECK(hipMemcpy(L.dev, host.data(), host.size() * sizeof(GrpEntry), hipMemcpyHostToDevice));
+    gp.launches.push_back(L);
+    return 0;
+  };
+  if (key.dir == 0) {
     // each member keeps its own lines per workgroup (its LDS within the
-    // budget); the launch takes the most threads and LDS of any member
+    // budget), so lx * P stays near WG_MAX for every member; the launch
+    // takes the most threads and LDS of any member
+    GrpLaunch L;
+    L.x = true;
     int nthr = 0;
     size_t shm = 0;
-    for (auto& e : host) {
+    long long blk = 0;
+    for (auto& e : all) {
       const int P = e.a.P;
       e.a.vec = key.aligned && (e.a.N % VW == 0) && ((e.off * (long long)sizeof(TS)) % 16 == 0);
       e.a.pitch = e.a.vec ? e.a.N + VW : e.a.N + 1;
       auto lds = [&](int lx) {
-        size_t t = align16((size_t)lx * e.a.pitch * sizeof(TS));
-        if (sv == SOLVE_CIRC) t += 2 * align16((size_t)4 * P * lx * sizeof(CS));
-        return t;
+        const size_t tile = align16((size_t)lx * e.a.pitch * sizeof(TS));
+        if (sv == SOLVE_CIRC) return tile + 2 * align16((size_t)4 * P * lx * sizeof(CS));
+        if (sv == SOLVE_LU) return std::max(tile, align16((size_t)4 * P * lx * sizeof(CS)));
+        return tile;
       };
       int lx = WG_MAX / P;
       if (lx > 64) lx = 64;
@@ -1661,22 +1845,37 @@ static int grp_build(pyr_group* g, const GrpKey& key, GrpPlan& gp) {
     }
     L.block = dim3(nthr);
     L.shm = shm;
-  } else {
-    const int ly = Pmax <= 8 ? 64 : 32;
+    L.nblocks = blk;
+    return upload(L, all);
+  }
+  // y/z: workgroups are (ly lanes, P chunks) shaped for the longest member of
+  // the launch; members are split into launches by chunk count (<= 8, <= 16,
+  // <= 32) so that short lines do not run in workgroups shaped for 1024 points
+  const int pre = all[0].a.addv ? PRE_V : ((key.epi == EPI_ADD || key.epi == EPI_MAXABS) ? PRE_OLD : PRE_NONE);
+  const int bounds[3] = {8, 16, 32};
+  for (int cl = 0; cl < 3; ++cl) {
+    std::vector<GrpEntry> host;
+    for (auto& e : all)
+      if (e.a.P <= bounds[cl] && (cl == 0 || e.a.P > bounds[cl - 1])) host.push_back(e);
+    if (host.empty()) continue;
+    int Pmax = 1;
+    for (auto& e : host) Pmax = std::max(Pmax, e.a.P);
+    const int ly = Pmax <= 8 ? 64 : (Pmax <= WG_MAX / 32 ? 32 : 16);
+    long long blk = 0;
     for (auto& e : host) {
       e.blk0 = blk;
       blk += (e.a.nlines + ly - 1) / ly;
     }
+    GrpLaunch L;
+    L.x = false;
     L.ly = ly;
     L.block = dim3(ly, Pmax);
-    L.shm = sv == SOLVE_CIRC ? 2 * align16((size_t)4 * Pmax * ly * sizeof(CS)) : 0;
-    L.pre = host[0].a.addv ? PRE_V : ((key.epi == EPI_ADD || key.epi == EPI_MAXABS) ? PRE_OLD : PRE_NONE);
+    const size_t ex = align16((size_t)4 * Pmax * ly * sizeof(CS));
+    L.shm = sv == SOLVE_CIRC ? 2 * ex : (sv == SOLVE_LU ? ex : 0);
+    L.pre = pre;
+    L.nblocks = blk;
+    if (upload(L, host)) return 1;
   }
-  L.nblocks = blk;
-  L.M = (int)host.size();
-  HIPCHECK(hipMalloc(&L.dev, host.size() * sizeof(GrpEntry)));
-  HIPCHECK(hipMemcpy(L.dev, host.data(), host.size() * sizeof(GrpEntry), hipMemcpyHostToDevice));
-  gp.launches.push_back(L);
   return 0;
 }
 
@@ -1691,15 +1890,18 @@ static int grp_launch_typed(pyr_group* g, const GrpLaunch& L, const TS* in, TS* 
     if (set_lds(k, L.shm)) return 1;
     hipLaunchKernelGGL(k, grid, L.block, L.shm, g->stream, L.dev, L.M, in, out);
   } else {
-    auto pick = [&](auto k64, auto k32) {
-      return L.ly == 64 ? k64 : k32;
+    auto pick = [&](auto k64, auto k32, auto k16) {
+      return L.ly == 64 ? k64 : (L.ly == 32 ? k32 : k16);
     };
-    auto k = pick(k_line_yz_grp<TS, KIND, SOLVE, MODE, 64, PRE_NONE>, k_line_yz_grp<TS, KIND, SOLVE, MODE, 32, PRE_NONE>);
+    auto k = pick(k_line_yz_grp<TS, KIND, SOLVE, MODE, 64, PRE_NONE>, k_line_yz_grp<TS, KIND, SOLVE, MODE, 32, PRE_NONE>,
+                  k_line_yz_grp<TS, KIND, SOLVE, MODE, 16, PRE_NONE>);
     if (L.pre == PRE_OLD)
-      k = pick(k_line_yz_grp<TS, KIND, SOLVE, MODE, 64, PRE_OLD>, k_line_yz_grp<TS, KIND, SOLVE, MODE, 32, PRE_OLD>);
+      k = pick(k_line_yz_grp<TS, KIND, SOLVE, MODE, 64, PRE_OLD>, k_line_yz_grp<TS, KIND, SOLVE, MODE, 32, PRE_OLD>,
+               k_line_yz_grp<TS, KIND, SOLVE, MODE, 16, PRE_OLD>);
     if (L.pre == PRE_V) {
       if constexpr (KIND == K_R4)
-        k = pick(k_line_yz_grp<TS, KIND, SOLVE, MODE, 64, PRE_V>, k_line_yz_grp<TS, KIND, SOLVE, MODE, 32, PRE_V>);
+        k = pick(k_line_yz_grp<TS, KIND, SOLVE, MODE, 64, PRE_V>, k_line_yz_grp<TS, KIND, SOLVE, MODE, 32, PRE_V>,
+                 k_line_yz_grp<TS, KIND, SOLVE, MODE, 16, PRE_V>);
       else
         return fail("filter epilogue on a non-filter kernel");
     }
@@ -1713,11 +1915,16 @@ static int grp_launch_typed(pyr_group* g, const GrpLaunch& L, const TS* in, TS* 
 template <typename TS, int MODE>
 int grp_launch_mode(pyr_group* g, const GrpLaunch& L, const TS* in, TS* out) {
   switch (L.kind) {
-    case K_D1: return grp_launch_typed<TS, K_D1, SOLVE_CIRC, MODE>(g, L, in, out);
-    case K_R3: return grp_launch_typed<TS, K_R3, SOLVE_CIRC, MODE>(g, L, in, out);
+    case K_D1:
+      return L.solve == SOLVE_CIRC ? grp_launch_typed<TS, K_D1, SOLVE_CIRC, MODE>(g, L, in, out)
+                                   : grp_launch_typed<TS, K_D1, SOLVE_LU, MODE>(g, L, in, out);
+    case K_R3:
+      return L.solve == SOLVE_CIRC ? grp_launch_typed<TS, K_R3, SOLVE_CIRC, MODE>(g, L, in, out)
+                                   : grp_launch_typed<TS, K_R3, SOLVE_LU, MODE>(g, L, in, out);
     default:
-      return L.solve == SOLVE_CIRC ? grp_launch_typed<TS, K_R4, SOLVE_CIRC, MODE>(g, L, in, out)
-                                   : grp_launch_typed<TS, K_R4, SOLVE_NONE, MODE>(g, L, in, out);
+      if (L.solve == SOLVE_CIRC) return grp_launch_typed<TS, K_R4, SOLVE_CIRC, MODE>(g, L, in, out);
+      if (L.solve == SOLVE_LU) return grp_launch_typed<TS, K_R4, SOLVE_LU, MODE>(g, L, in, out);
+      return grp_launch_typed<TS, K_R4, SOLVE_NONE, MODE>(g, L, in, out);
   }
 }
 

@@ -51,6 +51,8 @@ struct LineArgs {
   const double* tab64;
   const float* tab32;
   int ntab;                // entries of one block: 9N + 16P^2
+  int gtab;                // 1: the solve reads the tables from global memory
+                           //    (too large for LDS: long lines, grouped launches)
   const double* bcoef;
   const short* bidx;
   const int* bnt;

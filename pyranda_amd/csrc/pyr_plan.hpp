@@ -39,7 +39,7 @@
 namespace pyr {
 
 constexpr int CHUNK = 32;   // rows per thread (register-resident)
-constexpr int PMAX = 16;    // chunks per line -> N <= 512
+constexpr int PMAX = 32;    // chunks per line -> N <= 1024
 constexpr int NB = 4;       // boundary rows per side with their own RHS forms
 constexpr int MAXT = 10;    // terms per boundary row
 
@@ -292,8 +292,9 @@ inline OpPlan build_plan(int op, int n, int periodic, int v1, int v2, double met
   if (pl.null_op) return pl;
   // the reference's boundary forms read up to row 8 of a line (compact_r4.f90:
   // 117-120); shorter lines are outside what the schemes define
-  if (n < 8) throw std::runtime_error("line length " + std::to_string(n) + " is below 8 (supported: 1-3 or 8-512)");
-  if (n > CHUNK * PMAX) throw std::runtime_error("line length " + std::to_string(n) + " exceeds 512");
+  if (n < 8) throw std::runtime_error("line length " + std::to_string(n) + " is below 8 (supported: 1-3 or 8-1024)");
+  if (n > CHUNK * PMAX)
+    throw std::runtime_error("line length " + std::to_string(n) + " exceeds " + std::to_string(CHUNK * PMAX));
 
   // assembled rows (compact_basetype.f90:124-133), 0-based
   std::vector<std::vector<double>> gal(n, std::vector<double>(5, 0.0));

@@ -150,7 +150,7 @@ WIDE_RATIO = 8.0
 # derivative) is then stored in FP64 when its first evaluation (in FP64) is
 # offset-dominated, else in FP32 (the decision is kept per kernel and input
 # signature); with False a result is FP64 when an input field is
-STORAGE = {"value_storage": False}
+STORAGE = {"value_storage": False, "wide_if_wide_input": False}
 
 
 def offset_dominated(t):
@@ -227,6 +227,10 @@ class FusedSubtree:
         # STORAGE["value_storage"] is on; else by the input types
         self.out_dtype = None
         self._decided = {}
+        # the result is a named field of a run storing by value: kept in FP64
+        # when every field input is FP64 (last_all64 tells the owner)
+        self.named_all64 = False
+        self.last_all64 = False
 
     def _source(self, sig, out_dt):
         name = "pyr_pw_" + hashlib.sha1((self.body + repr(sig) + out_dt).encode()).hexdigest()[:16]
@@ -316,8 +320,14 @@ class FusedSubtree:
             return self._py(*_expand_members(vals, shape))
         sig = tuple(sig)
         decide = False
+        all64 = all(k != "T32" for k in sig if k[0] == "T")
+        self.last_all64 = all64
         if self.out_dtype is not None:
             out_dt = self.out_dtype
+        elif STORAGE["value_storage"] and self.named_all64 and STORAGE["wide_if_wide_input"] and all64:
+            out_dt = "f64"
+        elif STORAGE["value_storage"] and not self.named_all64 and STORAGE["wide_if_wide_input"] and any64:
+            out_dt = "f64"
         elif STORAGE["value_storage"]:
             out_dt = self._decided.get(sig)
             if out_dt is None:
@@ -462,6 +472,8 @@ class FusedGroup:
                 out_dts, decide = ["f64"] * len(self.maps), True
             else:
                 out_dts = list(known)
+            if STORAGE["wide_if_wide_input"]:
+                out_dts = ["f64" if any(is64[j] for j in m) else dt for m, dt in zip(self.maps, out_dts)]
         if any(k[0] == "G" for k in sig):
             fields.append(shape.mid)
         nout = len(self.subs)
@@ -1004,12 +1016,14 @@ class PointwiseRun:
         named = getattr(sim, "named_storage", False)
         wide_out = named and any(sim.storage_of(l) != "f32" for l in self.outputs)
         state_of = getattr(sim, "fp64_state_of", None) if wide_out else None
+        in64 = []   # per input: read as an FP64 field (or scalar)
         for nm in self.inputs:
             v = sim.variables[nm].data
             if state_of is not None:
                 v64 = state_of(nm)
                 if v64 is not None:
                     v = v64
+            in64.append(not (torch.is_tensor(v) and v.dtype == torch.float32))
             if torch.is_tensor(v) and v.dim() == 0 and not _device_scalar(v):
                 v = v.item()
             if isinstance(v, (int, float)) and not isinstance(v, bool):
@@ -1031,14 +1045,22 @@ class PointwiseRun:
         # an FP32 run's named fields have a storage type of their own
         # (pyrandaSim.storage_of: 'f64', 'f32', or None when not decided yet:
         # computed in FP64, then decided from the values)
-        storage_of = getattr(sim, "storage_of", None) if getattr(sim, "value_storage", False) else None
+        storage_of = getattr(sim, "storage_of", None) if getattr(sim, "named_storage", False) else None
         undecided = []
         if storage_of is not None:
+            inputs_rule = STORAGE["wide_if_wide_input"]
             for j, lhs in enumerate(self.outputs):
                 st = storage_of(lhs)
                 if st is None:
-                    undecided.append(j)
-                    st = "f64"
+                    ins, ints = self.deps[j]
+                    # a line computed from FP64 values only keeps FP64 storage
+                    if inputs_rule and all(sig[i] != "T32" or in64[i] for i in ins) \
+                            and all(types[k] == "f64" for k in ints):
+                        st = "f64"
+                        sim.decide_storage(lhs, "f64")
+                    else:
+                        undecided.append(j)
+                        st = "f64"
                 types[j] = st
         if len(fields) + len(types) > _lib.PW_MAX_PTRS or len(scalars) > _lib.PW_MAX_SCALARS:
             return False

@@ -32,6 +32,7 @@ from .pyrandaMesh import pyrandaMesh, defaultMeshOptions
 from .pyrandaPackage import pyrandaPackage
 from .pyrandaVar import pyrandaVar
 from .pyrandaTimestep import pyrandaTimestep
+from .pyrandaBC import pyrandaBC
 
 ALIGN = 16
 
@@ -238,14 +239,67 @@ class _Member:
         self.opts = opts
         self.nn = tuple(int(n) for n in opts["nn"])
         self.periodic = [bool(p) for p in opts["periodic"]]
-        if not all(self.periodic[d] or self.nn[d] == 1 for d in range(3)):
-            raise NotImplementedError("group members need periodic (or single-point) directions")
+        sym = opts.get("symmetric") or [[False, False]] * 3
+        if any(any(s) for s in sym):
+            raise NotImplementedError("group members take periodic or bounded (NONE) directions, not SYMM")
         x1, xn = opts["x1"], opts["xn"]
         bcs = []
         for d in range(3):
             bcs += ["PERI" if self.periodic[d] else "NONE"] * 2
         self.ctx = ParcopContext(self.nn, (x1[0], x1[1], x1[2]), (xn[0], xn[1], xn[2]), bcs, device=device)
         self.npts = self.nn[0] * self.nn[1] * self.nn[2]
+
+
+class _GroupBC(pyrandaBC):
+    """pyrandaBC's extrap/const (pyrandaBC.py:58-160) on every member's
+    boundary planes at once: the planes' points (offsets into the shared
+    allocation) are gathered per direction once, and each update is one
+    gather/scatter over all members, with the arithmetic of the single
+    simulation (bit for bit its result)."""
+
+    def __init__(self, pysim):
+        pyrandaBC.__init__(self, pysim)
+        self._idx = {}
+
+    def _planes(self, direction):
+        if direction not in self._idx:
+            g = self.pyranda
+            axis = {"x": 0, "y": 1, "z": 2}[direction[0]]
+            hi = direction[1:] == "n"
+            sets = ([], [], [])
+            for mem, off in zip(g.members, g.offsets):
+                nx, ny, nz = mem.nn
+                n = mem.nn[axis]
+                if mem.periodic[axis] or n < 3:
+                    raise ValueError("bc on %s: member %s is periodic or too short along it" % (direction, mem.nn))
+                ii, jj, kk = torch.arange(nx), torch.arange(ny), torch.arange(nz)
+                for r, pos in enumerate([n - 1, n - 2, n - 3] if hi else [0, 1, 2]):
+                    # the plane's points, in the same order for the three planes
+                    if axis == 0:
+                        flat = pos + nx * jj[None, :] + nx * ny * kk[:, None]
+                    elif axis == 1:
+                        flat = ii[None, :] + nx * pos + nx * ny * kk[:, None]
+                    else:
+                        flat = ii[None, :] + nx * jj[:, None] + nx * ny * pos
+                    sets[r].append(off + flat.reshape(-1))
+            dev = g.PyMPI.device
+            self._idx[direction] = tuple(torch.cat(s_).to(dev) for s_ in sets)
+        return self._idx[direction]
+
+    def extrapolate(self, var, direction, order):
+        i0, i1, i2 = self._planes(direction)
+        d = self.pyranda.variables[var].data.view(-1)
+        if order == 2:
+            d[i0] = 2 * d[i1] - d[i2]
+        else:
+            d[i0] = d[i1]
+
+    def constant(self, var, direction, val):
+        i0, _, _ = self._planes(direction)
+        self.pyranda.variables[var].data.view(-1)[i0] = val
+
+    def field(self, var, direction, field):
+        raise NotImplementedError("bc.field over a group")
 
 
 class _GroupTimestep(pyrandaTimestep):
@@ -309,7 +363,7 @@ class pyrandaGroupSim(pyrandaSim):
         # every field of a group is stored in the run's precision (no FP64
         # storage of offset-dominated fields, pyrandaSim(wide_fields=None))
         self._storage = {}
-        self.value_storage = self.named_storage = False
+        self.value_storage = self.named_storage = self.wide_if_wide_input = False
         self.members = [_Member(m, dev) for m in meshes]
         if precision == "fp32-fp64solve":
             for m in self.members:
@@ -425,6 +479,8 @@ class pyrandaGroupSim(pyrandaSim):
     def addPackage(self, package):
         if isinstance(package, pyrandaTimestep) and not isinstance(package, _GroupTimestep):
             package = _GroupTimestep(self)
+        if isinstance(package, pyrandaBC) and not isinstance(package, _GroupBC):
+            package = _GroupBC(self)
         pyrandaSim.addPackage(self, package)
 
     # -- operators (null directions of every member give 0) ---------------------

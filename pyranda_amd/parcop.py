@@ -122,11 +122,12 @@ class ParcopContext:
         _lib.check(self._L.pyr_set_tuning(self._h, op, direction, prec, int(x_lds_kb), int(yz_pipe),
                                           int(yz_lines)))
 
-    def autotune(self, dtype, ops=None, reps=5, x_lds_kb=(8, 12, 16, 24, 40, 64), seed=7, rounded=False,
+    def autotune(self, dtype, ops=None, reps=5, x_lds_kb=(8, 12, 16, 24, 40, 64, 80), seed=7, rounded=False,
                  solve_modes=False):
         """Time each operator and direction at this precision with every
-        candidate (x: LDS budget, hence lines per workgroup; y/z: pipelined
-        kernel or not) on random fields and keep the fastest. The timed
+        candidate (x: LDS budget, hence lines per workgroup, and the pipelined
+        kernel or not; y/z: pipelined kernel or not) on random fields and keep
+        the fastest. The timed
         launches cycle through enough fields to exceed the device's 256 MB
         last-level cache, so every launch reads its input from HBM as the
         operators of a time step do. rounded: FP64
@@ -153,7 +154,10 @@ class ParcopContext:
                     continue
                 kw = {"out32": out32} if out32 is not None and d == 2 else {}
                 if d == 0:
-                    cands = [(("x_lds_kb", kb),) for kb in x_lds_kb]
+                    # the LDS budget (lines per workgroup) with the pipelined
+                    # persistent x kernel (yz_pipe=1, where the line shape
+                    # allows it) or the one-pass kernel
+                    cands = [(("x_lds_kb", kb), ("yz_pipe", pp)) for pp in (0, 1) for kb in x_lds_kb]
                 else:
                     cands = [(("yz_pipe", 1),), (("yz_pipe", 0), ("yz_lines", 64)),
                              (("yz_pipe", 0), ("yz_lines", 32))]

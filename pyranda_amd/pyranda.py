@@ -19,12 +19,18 @@ Storage of FP32 runs (`wide_fields`): FP32 storage rounds a value to 6e-8 of
 its magnitude, so a field whose values sit close to one large value (the
 pressure and total energy of a low-Mach flow, stresses carrying -p, the
 momentum fluxes) loses most of its variation, and a derivative turns that
-rounding into noise. With wide_fields="auto" (the default of the fp32 modes)
-every field is stored in FP32 unless it is offset-dominated when first set
-(fuse.offset_dominated): such fields, named or the unnamed results of
-pointwise kernels (flux arguments), are stored in FP64, and the decision is
-kept for the run. wide_fields=None stores every field in FP32; a collection
-of names stores exactly those variables in FP64.
+rounding into noise. In a low-Mach flow the fluxes' derivatives also cancel
+to O(M^2) of their size, so FP32 rounding of velocities and fluxes reaches
+the right-hand sides amplified. wide_fields="auto" (the default of the fp32
+modes) therefore stores in FP64: every field offset-dominated when first set
+(fuse.offset_dominated), every field a pointwise line computes from FP64
+values only (primitives from the FP64 conserved states), and every unnamed
+pointwise result (flux argument) with an FP64 input; FP32 storage stays for
+operator results (derivatives, the LAD ring and filters) and what is computed
+from them. wide_fields="offset" keeps only the offset-dominated fields in
+FP64 (faster, less accurate); None stores every field in FP32; a collection
+of names stores exactly those variables in FP64. Decisions are kept for the
+run.
 
 `device_scalars=True` keeps the grid reductions (dt.courant, dt.diff, sum,
 max, min) on the GPU as 0-d FP64 tensors instead of Python floats, and rk4
@@ -125,9 +131,14 @@ class pyrandaSim:
         self._storage = {}
         self.value_storage = False   # undecided fields are decided by value
         self.named_storage = False   # named fields have a storage of their own
+        self.wide_if_wide_input = False
+        if wide_fields not in ("auto", "offset", None) and isinstance(wide_fields, str):
+            raise ValueError("wide_fields: 'auto', 'offset', None or a collection of names")
         if precision != "fp64":
-            if wide_fields == "auto":
+            if wide_fields in ("auto", "offset"):
                 self.value_storage = self.named_storage = True
+                # auto: FP64 values stay FP64 through pointwise algebra
+                self.wide_if_wide_input = wide_fields == "auto"
             elif wide_fields is not None:
                 self._storage = {n: "f64" for n in wide_fields}
                 self.named_storage = True
@@ -293,8 +304,11 @@ class pyrandaSim:
                 if torch.is_tensor(d) and tuple(d.shape) == tuple(self.mesh.shape):
                     self._state64[U] = d.to(torch.float64)
         # storage decided by value only for fields the ICs set (the others
-        # hold their allocation's zeros until an ALG line writes them)
-        self._cast_fields(decide=set(names))
+        # hold their allocation's zeros until an ALG line writes them); with
+        # wide_fields="auto" the fields pointwise ALG lines rewrite are
+        # decided at their first evaluation, from its inputs
+        deferred = self._pointwise_alg_lhs() if self.wide_if_wide_input else set()
+        self._cast_fields(decide=set(names) - deferred)
         for U in self._state64:
             self._mark_state(U)
         snans = self.checkForNan(sorted(set(names)))
@@ -347,6 +361,23 @@ class pyrandaSim:
             st = self._storage[name] = "f64" if offset_dominated(t) else "f32"
         return torch.float64 if st == "f64" else torch.float32
 
+    def decide_storage(self, name, st):
+        """Record the storage ("f64"/"f32") of a field not decided yet."""
+        if self.dtype != torch.float64 and self._storage.get(name) is None:
+            self._storage[name] = st
+
+    def _pointwise_alg_lhs(self):
+        """Fields written by pointwise ALG lines (their storage is decided at
+        their first evaluation from its inputs, not from an IC value)."""
+        from .fuse import pure_pointwise
+        out = set()
+        for eq in self.equations:
+            if eq.kind == "ALG" and eq.LHS and eq.rank == 1:
+                pw = pure_pointwise(eq.sRHS)
+                if pw is not None and pw is not False:
+                    out.add(eq.LHS[0])
+        return out
+
     def settle_storage(self, name):
         """Decide (once) and apply the storage of a named field from its current value."""
         var = self.variables[name]
@@ -379,11 +410,12 @@ class pyrandaSim:
 
         class _Mode:
             def __enter__(self):
-                self.old = fuse.STORAGE["value_storage"]
+                self.old = dict(fuse.STORAGE)
                 fuse.STORAGE["value_storage"] = sim.value_storage
+                fuse.STORAGE["wide_if_wide_input"] = getattr(sim, "wide_if_wide_input", False)
 
             def __exit__(self, *exc):
-                fuse.STORAGE["value_storage"] = self.old
+                fuse.STORAGE.update(self.old)
 
         return _Mode()
 
@@ -434,12 +466,17 @@ class pyrandaSim:
             st = self.storage_of(eq.LHS[0])
             if st is not None:
                 root.out_dtype = st
+            else:
+                root.named_all64 = True
         rhs = eq.RHS(self)
         if not eq.LHS:
             return
         if eq.rank == 1:
             self.variables[eq.LHS[0]].data = rhs
             if self.named_storage:
+                if (self.wide_if_wide_input and root is not None and root.named_all64 and root.last_all64
+                        and torch.is_tensor(rhs) and rhs.dtype == torch.float64):
+                    self.decide_storage(eq.LHS[0], "f64")
                 self.settle_storage(eq.LHS[0])
         else:
             for ii in range(len(rhs)):

@@ -467,3 +467,98 @@ def kh_group(pr, sizes, precision="fp32", device=None):
                     "rho2": 2.0, "L": 0.025, "Vmean": 0.0})
     dt = g.var("dt").data.to(torch.float64).clone()
     return g, {"time": torch.zeros_like(dt), "dt": dt}
+
+
+# --- Kelvin-Helmholtz shear layers between walls (ensemble workload) ---------
+# One shear layer at y = 1/2 in a unit box, periodic in x with walls at y = 0
+# and 1: velocity jump 1 (u = +-1/2), density ratio 2 (rho 1 below, 2 above),
+# Mach 0.3 on the velocity jump against the light side's sound speed
+# (p0 = rho1 (1/0.3)^2 / gamma), a seeded multimode v perturbation of
+# amplitude 1e-2 localised at the interface. The EOM is KH.py's Euler + LAD
+# set (KH_EOM) followed by the wall conditions of examples/euler.py:78-82
+# (extrapolated rho and Et, u = v = 0 on the wall planes).
+KH_WALL_BC = """
+# Walls at y = 0 and y = 1
+bc.extrap(['rho','Et'],['y1','yn'])
+bc.const(['u','v'],['y1','yn'],0.0)
+"""
+
+
+def kh_wall_ic(seed=2024, modes=6, amp=1e-2, delta=0.025, mach=0.3):
+    """IC string of the wall-bounded KH layer with a multimode perturbation
+    drawn from `seed` (integer wavenumbers 1..8 in x, normalised so that the
+    sum's largest value is `amp`)."""
+    rng = np.random.default_rng(seed)
+    ks = rng.choice(np.arange(1, 9), size=modes, replace=False)
+    a = rng.uniform(0.5, 1.0, modes)
+    ph = rng.uniform(0.0, 2.0 * math.pi, modes)
+    xs = np.linspace(0.0, 1.0, 4097)
+    norm = np.abs(sum(a[m] * np.sin(2 * math.pi * ks[m] * xs + ph[m]) for m in range(modes))).max()
+    terms = " + ".join("%r*sin(%r*meshx + %r)" % (float(a[m] / norm), float(2 * math.pi * ks[m]), float(ph[m]))
+                       for m in range(modes))
+    gamma = 5. / 3.
+    p0 = 1.0 * (1.0 / mach) ** 2 / gamma
+    return """
+:u:   = 0.5*tanh( (meshy - 0.5)/DELTA )
+:rho: = 1.5 + 0.5*tanh( (meshy - 0.5)/DELTA )
+:v:   = AMP*( TERMS )*exp( -((meshy - 0.5)/(4.0*DELTA))**2 )
+:p:   = P0 + 0.0*meshx
+:Et: = :p:/( gamma - 1.0 ) + .5*:rho:*(:u:*:u: + :v:*:v:)
+:rhou: = :rho:*:u:
+:rhov: = :rho:*:v:
+:cs:  = sqrt( :p: / :rho: * gamma )
+:dt: = dt.courant(:u:,:v:,:w:,:cs:)*.1
+""".replace("DELTA", repr(delta)).replace("AMP", repr(amp)).replace("TERMS", terms).replace("P0", repr(p0)) \
+        .replace("gamma", repr(gamma))
+
+
+def _kh_wall_mesh(nx, ny):
+    return """
+    xdom = (0.0, LX,  NX, periodic=True)
+    ydom = (0.0, 1.0, NY)
+    """.replace("LX", str(1.0 * (nx - 1.0) / nx)).replace("NX", str(nx)).replace("NY", str(ny))
+
+
+def kh_wall_setup(pr, nx, ny, precision="fp32", device=None, device_scalars=True, wide_fields=None, seed=2024):
+    """One wall-bounded KH layer on an nx x ny grid (after setIC), stored like
+    a group member (wide_fields=None: every field in the run's precision)."""
+    gamma, R = 5. / 3., 1.0
+    cp = R / (1.0 - 1.0 / gamma)
+    kw = {"silent": True, "precision": precision, "device_scalars": device_scalars, "wide_fields": wide_fields}
+    if device is not None:
+        kw["device"] = device
+    ss = pr.pyrandaSim("KH-wall-%dx%d" % (nx, ny), _kh_wall_mesh(nx, ny), **kw)
+    ss.addPackage(pr.pyrandaTimestep(ss))
+    ss.addPackage(pr.pyrandaBC(ss))
+    ss.EOM(KH_EOM + KH_WALL_BC, {"gamma": gamma, "R0": R, "cv": cp - R})
+    ss.setIC(kh_wall_ic(seed))
+    return ss
+
+
+def kh_wall_ensemble(pr, sizes, precision="fp32", device=None, seed=2024):
+    """Wall-bounded KH members run one by one, plus their driver state."""
+    sims, state = [], []
+    for nx, ny in sizes:
+        ss = kh_wall_setup(pr, nx, ny, precision=precision, device=device, seed=seed)
+        dt = ss.var("dt").data
+        dt = dt if torch.is_tensor(dt) else torch.tensor(float(dt), dtype=torch.float64, device=ss.PyMPI.device)
+        sims.append(ss)
+        state.append({"time": torch.zeros((), dtype=torch.float64, device=ss.PyMPI.device), "dt": dt.clone()})
+    return sims, state
+
+
+def kh_wall_group(pr, sizes, precision="fp32", device=None, seed=2024):
+    """The wall-bounded KH members as ONE pyrandaGroupSim plus its driver
+    state {time, dt} ((M,) FP64 device tensors)."""
+    gamma, R = 5. / 3., 1.0
+    cp = R / (1.0 - 1.0 / gamma)
+    kw = {"precision": precision}
+    if device is not None:
+        kw["device"] = device
+    g = pr.pyrandaGroupSim("KH-wall-group", [_kh_wall_mesh(nx, ny) for nx, ny in sizes], **kw)
+    g.addPackage(pr.pyrandaTimestep(g))
+    g.addPackage(pr.pyrandaBC(g))
+    g.EOM(KH_EOM + KH_WALL_BC, {"gamma": gamma, "R0": R, "cv": cp - R})
+    g.setIC(kh_wall_ic(seed))
+    dt = g.var("dt").data
+    return g, {"time": torch.zeros(len(sizes), dtype=torch.float64, device=dt.device), "dt": dt.clone()}

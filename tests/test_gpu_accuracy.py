@@ -2,9 +2,11 @@
 
 The 3-D Taylor-Green Euler+LAD workload at 256^3 (gamma 1.4, Mach 0.1,
 CFL 0.5 from the initial state, LAD on, the EOM of
-examples/TaylorGreen.py:40-86), 100 RK4 steps (pyranda.py:758-810) with FP32
-fields, FP32 sweeps plus one FP64-residual refinement in every operator, and
-FP64 storage of the offset-dominated fields (pyrandaSim wide_fields="auto"),
+examples/TaylorGreen.py:40-86), 100 RK4 steps (pyranda.py:758-810) in the FP32
+mode (pyrandaSim wide_fields="auto": FP32 storage for operator results and the
+LAD chain, FP64 for offset-dominated fields, values computed from FP64 values
+only and flux arguments with FP64 inputs; FP32 sweeps plus one FP64-residual
+refinement in every operator on FP32 fields),
 against the same steps with FP64 fields: every conserved variable within
 1e-6 of the FP64 result in both the max norm and the L2 norm (relative to
 the FP64 field's max and L2 norms).
@@ -61,8 +63,12 @@ def test_taylor_green_256_fp32_within_1e6_of_fp64_after_100_steps(pr):
     torch.cuda.empty_cache()
     ss = _tg(pr, n, "fp32", p0)
     _say("FP64 storage decided for", sorted(k for k, v in ss._storage.items() if v == "f64"))
+    # offset-dominated (p, Et, the normal stresses) and computed from FP64
+    # values only (the velocities) in FP64; operator results (the velocity
+    # gradients) and the LAD chain built on them in FP32
     assert ss._storage.get("p") == "f64" and ss._storage.get("Et") == "f64"
-    assert ss._storage.get("u") == "f32"
+    assert ss._storage.get("u") == "f64" and ss._storage.get("tauxx") == "f64"
+    assert ss._storage.get("ux") == "f32" and ss._storage.get("mu") == "f32" and ss._storage.get("S") == "f32"
     t = 0.0
     for _ in range(steps):
         t = ss.rk4(t, dt)

@@ -115,3 +115,62 @@ def test_group_sim_matches_members_alone(pr, precision):
             assert _same(gst["dt"][m], solo_state[m]["dt"]), (step, m, "dt")
             assert _same(gst["time"][m], solo_state[m]["time"]), (step, m, "time")
         _say("step", step, "ok")
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_wall_group_matches_members_alone(pr, precision):
+    """Wall-bounded KH layers (periodic x, bounded y with bc.extrap/bc.const)
+    on uneven grids up to 1024 points per line, as ONE pyrandaGroupSim
+    (grouped launches split by chunk count, grouped bounded LU, one
+    gather/scatter per boundary update): bit for bit each member alone."""
+    from pyranda_amd.workloads import kh_advance, kh_wall_ensemble, kh_wall_group
+    sizes = [(128, 96), (1024, 72), (64, 1000), (200, 130), (40, 48)]
+    _say("wall group vs members alone", precision, sizes)
+    g, gst = kh_wall_group(pr, sizes, precision=precision)
+    solo, solo_state = kh_wall_ensemble(pr, sizes, precision=precision)
+    for m in range(len(sizes)):
+        assert _same(g.var("dt").data[m], solo[m].var("dt").data), m
+        for k in ("rho", "u", "v", "Et"):
+            assert _same(g.member_field(k, m), solo[m].var(k).data), ("ic", m, k)
+    for step in range(3):
+        kh_advance(g, gst)
+        for s, st in zip(solo, solo_state):
+            kh_advance(s, st)
+        torch.cuda.synchronize()
+        for m, s in enumerate(solo):
+            for k in ("rho", "rhou", "rhov", "Et", "u", "v", "kappa", "mu", "beta"):
+                assert _same(g.member_field(k, m), s.var(k).data), (step, m, k)
+            assert _same(gst["dt"][m], solo_state[m]["dt"]), (step, m, "dt")
+        _say("step", step, "ok")
+    # the walls hold: v = 0 on both wall planes of every member
+    for m in range(len(sizes)):
+        v = g.member_field("v", m)
+        assert float(v[:, 0, :].abs().max()) == 0.0 and float(v[:, -1, :].abs().max()) == 0.0
+
+
+def test_group_mode_switch_builds_new_plans(pr):
+    """A grouped FP32 operator applied under the refined solve, then with the
+    members switched to FP64 solves: each result equals every member's own
+    apply in that mode (the group's plans are keyed by the solve mode)."""
+    from pyranda_amd import _lib
+    from pyranda_amd.workloads import kh_group
+    sizes = [(64, 48), (40, 72)]
+    g, _ = kh_group(pr, sizes, precision="fp32")
+    grp = g.PyMPI.group
+    f = g.var("rho").data
+    for mode in ("refined", "fp64", "refined"):
+        for mem in g.members:
+            mem.ctx.set_r32_solve(mode)
+        out = grp.apply(_lib.OP_D1, 0, f)
+        torch.cuda.synchronize()
+        for m, mem in enumerate(g.members):
+            want = mem.ctx.apply(_lib.OP_D1, 0, g.member_field("rho", m).contiguous())
+            assert _same(_member(g, out, m), want), (mode, m)
+        _say("mode", mode, "ok")
+
+
+def _member(g, field, m):
+    mem = g.members[m]
+    nx, ny, nz = mem.nn
+    off = g.offsets[m]
+    return field.reshape(-1)[off:off + mem.npts].reshape(nz, ny, nx).permute(2, 1, 0)

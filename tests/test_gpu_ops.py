@@ -52,6 +52,13 @@ CASES = [
     # periodic y/z lines of whole 32-row chunks in whole line groups: the
     # pipelined y/z kernels (global->LDS copies of the next line group)
     ((64, 128, 128), ("PERI", "PERI", "PERI")),
+    # periodic x lines of whole KiB: the pipelined persistent x kernel
+    ((256, 24, 10), ("PERI", "NONE", "PERI")),
+    # the longest lines (32 chunks; 16-line y/z workgroups; bounded LU tables
+    # read from global memory) and an uneven 1000-point partition
+    ((1024, 12, 8), ("PERI", "NONE", "NONE")),
+    ((12, 1000, 8), ("NONE", "NONE", "PERI")),
+    ((8, 9, 1024), ("NONE", "PERI", "NONE")),
 ]
 
 

@@ -92,8 +92,8 @@ def test_bad_inputs_return_errors(amd):
     bc6 = (ctypes.c_int * 6)(1, 1, 1, 1, 1, 1)
     assert L.pyr_setup_host(ctypes.byref(h), 0, 4, 4, 0, 1, 0, 1, 0, 1, bc6) != 0
     assert b"grid" in L.pyr_last_error()
-    assert L.pyr_setup_host(ctypes.byref(h), 600, 4, 4, 0, 1, 0, 1, 0, 1, bc6) != 0
-    assert b"512" in L.pyr_last_error()
+    assert L.pyr_setup_host(ctypes.byref(h), 1100, 16, 16, 0, 1, 0, 1, 0, 1, bc6) != 0
+    assert b"exceeds 1024" in L.pyr_last_error()
     assert L.pyr_setup_host(ctypes.byref(h), 6, 4, 4, 0, 1, 0, 1, 0, 1, bc6) != 0
     assert b"below 8" in L.pyr_last_error()
     bad = (ctypes.c_int * 6)(1, 1, 7, 1, 1, 1)
@@ -116,6 +116,10 @@ CASES = [
     ((8, 8, 3), ("PERI", "NONE", "NONE")),
     # P = 16 chunks per periodic line
     ((512, 9, 8), ("PERI", "PERI", "NONE")),
+    # the longest lines, 32 chunks (periodic and bounded; LU tables read from
+    # global memory on the device), and an uneven partition of 1000 points
+    ((1024, 8, 3), ("PERI", "NONE", "NONE")),
+    ((8, 1000, 3), ("NONE", "NONE", "NONE")),
 ]
 
 
