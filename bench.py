@@ -74,8 +74,10 @@ def make_tg(n, precision, p0, solver="circulant"):
     rounded = precision == "fp32-rounded-state"
     base = "fp32" if precision in ("fp32-rounded-state", "fp32-offset", "fp32-all-f32") else precision
     wide = {"fp32-all-f32": None, "fp32-offset": "offset"}.get(precision, "auto")
+    # grid reductions (dt.courant, dt.diff) stay on the device: no host round
+    # trip inside a step (bit for bit the host-scalar values)
     ss = pr.pyrandaSim("TGvortex", tg_mesh(n), silent=True, precision=base, fp64_state=not rounded,
-                       wide_fields=wide)
+                       wide_fields=wide, device_scalars=True)
     ss.PyMPI.ctx.set_periodic_solver(solver)
     ss.addPackage(pr.pyrandaTimestep(ss))
     ss.EOM(TG_EOM)
@@ -276,17 +278,25 @@ def ops_bench(n, steps, warmup, dev, only=None, solver="circulant", r32_solve="r
 
 
 def ensemble_bench(args, dev):
-    """KH ensemble: independent 2-D shear layers (examples/KH.py) on uneven
-    grids, FP32-refined. Timed four ways on the same members: member by member
-    with eager launches; one captured HIP graph per step over all members;
-    the ensemble as one pyrandaGroupSim (shared allocation, one launch per
-    operator and direction for all members); and that group step
-    captured in a HIP graph."""
+    """KH ensemble: independent 2-D shear layers on uneven grids. Default
+    workload: 64 wall-bounded layers (periodic x, walls in y with
+    bc.extrap/bc.const, density ratio 2, Mach 0.3 on the velocity jump,
+    seeded six-mode perturbation of amplitude 1e-2), each axis log-uniform in
+    [128, 1024]; --ens-workload periodic runs the doubly periodic
+    examples/KH.py layer on [128, 512] instead. Timed four ways on the same
+    members: member by member with eager launches; one captured HIP graph per
+    step over all members; the ensemble as one pyrandaGroupSim (shared
+    allocation, one launch per operator and direction for all members); and
+    that group step captured in a HIP graph."""
     import torch
     import pyranda_amd as pr
     from pyranda_amd.ensemble import pyrandaEnsemble
-    from pyranda_amd.workloads import ensemble_sizes, kh_advance, kh_ensemble, kh_group
-    sizes = ensemble_sizes(args.members, 128, 512)
+    from pyranda_amd.workloads import (ensemble_sizes, kh_advance, kh_ensemble, kh_group, kh_wall_ensemble,
+                                       kh_wall_group)
+    wall = args.ens_workload == "wall"
+    lo, hi = (128, 1024) if wall else (128, 512)
+    sizes = ensemble_sizes(args.members, lo, hi)
+    make_members, make_group = (kh_wall_ensemble, kh_wall_group) if wall else (kh_ensemble, kh_group)
     npts = sum(nx * ny for nx, ny in sizes)
     print("[ensemble] %d members, %d points, sizes %s" % (len(sizes), npts, sizes), file=sys.stderr, flush=True)
     out = {}
@@ -303,30 +313,33 @@ def ensemble_bench(args, dev):
 
     modes = args.ens_modes.split(",")
     if "members" in modes:
-        sims, state = kh_ensemble(pr, sizes, precision=args.ens_precision, device=dev)
+        sims, state = make_members(pr, sizes, precision=args.ens_precision, device=dev)
         ens = pyrandaEnsemble(sims, kh_advance, state)
         timed(ens.step_eager, "members_eager")
         ens.capture(concurrent=True)
         timed(ens.step, "members_graph")
         ens.graph = None
         del ens, sims, state
-    g, gst = kh_group(pr, sizes, precision=args.ens_precision, device=dev)
+    g, gst = make_group(pr, sizes, precision=args.ens_precision, device=dev)
     gens = pyrandaEnsemble([g], kh_advance, [gst])
     timed(gens.step_eager, "group_eager")
     gens.capture(concurrent=False)
     timed(gens.step, "group_graph")
     best = max(out, key=lambda k: out[k]["value"])
     line = {
-        "metric": "KH ensemble point-updates/s (%d independent 2-D Euler+LAD layers, uneven grids)" % len(sizes),
+        "metric": "KH ensemble point-updates/s (%d independent 2-D Euler+LAD layers, uneven grids%s)"
+                  % (len(sizes), ", wall-bounded in y" if wall else ""),
         "value": out[best]["value"], "unit": "point-updates/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": out[best]["ms_per_step"], "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
         "dtype": {"fp32": "f32 fields and sweeps, f64 residual refinement, f64 RK4 state",
                   "fp32-fp64solve": "f32 fields, f64 sweeps, f64 RK4 state",
                   "fp64": "f64"}[args.ens_precision],
-        "data": "synthetic (KH.py initial condition on each member grid)",
-        "config": {"workload": "kh_ensemble", "members": len(sizes), "points": npts,
-                   "sizes": "log-uniform per axis in [128, 512], multiples of 8, seed 2024",
+        "data": ("synthetic (wall-bounded KH layer, seeded multimode perturbation, on each member grid)" if wall
+                 else "synthetic (KH.py initial condition on each member grid)"),
+        "config": {"workload": "kh_wall_ensemble" if wall else "kh_ensemble", "members": len(sizes),
+                   "points": npts,
+                   "sizes": "log-uniform per axis in [%d, %d], multiples of 8, seed 2024" % (lo, hi),
                    "mode": best}, "modes": out,
         "speedup_over_members_eager": (out[best]["value"] / out["members_eager"]["value"]
                                        if "members_eager" in out else None),
@@ -346,7 +359,8 @@ def closure_bench(args, dev):
     from pyranda_amd.closure import MLPClosure, pyrandaLearnedLAD
     from pyranda_amd.workloads import rm_run, rm_setup
     n, prec = args.closure_n, args.closure_precision
-    plain = rm_setup(pr, n, precision=prec, device=dev)
+    nxr = args.closure_nx or None
+    plain = rm_setup(pr, n, precision=prec, device=dev, nx=nxr)
     nx, ny = plain.nx, plain.ny
     print("[closure] RM 2-D %dx%d %s" % (nx, ny, prec), file=sys.stderr, flush=True)
     helper = pyrandaLearnedLAD(plain, None)
@@ -389,7 +403,7 @@ def closure_bench(args, dev):
     # steps of the RM run with either evaluation, from the same initial state
     runs = {}
     for mode in ("plain", "closure"):
-        ss = rm_setup(pr, n, precision=prec, device=dev, closure=model if mode == "closure" else None)
+        ss = rm_setup(pr, n, precision=prec, device=dev, nx=nxr, closure=model if mode == "closure" else None)
         stt = rm_run(ss, 1)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -415,7 +429,9 @@ def closure_bench(args, dev):
                      "unit": "TFLOP/s", "frac": mfma_flops / (ms_mlp * 1e-3) / 1e12 / 2500.0,
                      "useful_tflops": useful_flops / (ms_mlp * 1e-3) / 1e12, "traffic": None},
         "step_ms": {"plain": 1e3 * runs["plain"][1], "closure": 1e3 * runs["closure"][1]},
-        "closure_calls": lad.calls, "fallbacks": lad.fallbacks,
+        "step_speedup": runs["plain"][1] / runs["closure"][1],
+        "closure_calls": lad.calls, "fallbacks": lad.fallbacks, "partial_fallbacks": lad.partial,
+        "points_flagged": lad.points_flagged,
         "judge": {k: {"rel_l2": a_, "rel_max": b_} for k, (a_, b_) in judge.items()},
         "drift_rho_rel_l2": drift, "train_s": t_train,
         "train_mse": [float(losses[:10].mean()), float(losses[-10:].mean())],
@@ -454,14 +470,17 @@ def main():
     ap.add_argument("--ensemble", action="store_true",
                     help="KH ensemble (independent 2-D layers on uneven grids, one HIP graph per step)")
     ap.add_argument("--members", type=int, default=64, help="--ensemble: number of members")
-    ap.add_argument("--ens-precision", default="fp32-fp64solve", choices=["fp64", "fp32", "fp32-fp64solve"],
-                    help="--ensemble: member precision (default: FP32 fields with FP64 solves, the fastest "
-                         "FP32-field mode of the grouped kernels)")
+    ap.add_argument("--ens-workload", default="wall", choices=["wall", "periodic"],
+                    help="--ensemble: wall-bounded layers on [128,1024]^2 or doubly periodic KH.py on [128,512]^2")
+    ap.add_argument("--ens-precision", default="fp32", choices=["fp64", "fp32", "fp32-fp64solve"],
+                    help="--ensemble: member precision (default: FP32 fields with the refined FP32 solve)")
     ap.add_argument("--ens-modes", default="members,group",
                     help="--ensemble: 'members' (member by member, eager and graph) and/or 'group'")
     ap.add_argument("--closure", action="store_true",
                     help="learned LAD closure on the 2-D RM case (train, time, judge)")
-    ap.add_argument("--closure-n", type=int, default=256, help="--closure: RM grid is 1.4n x n")
+    ap.add_argument("--closure-n", type=int, default=512, help="--closure: RM grid is nx x n (y periodic)")
+    ap.add_argument("--closure-nx", type=int, default=1024,
+                    help="--closure: points along the shock (x); 0: int(1.4 n) as examples/RM3D.py")
     ap.add_argument("--closure-precision", default="fp64", choices=["fp64", "fp32"])
     ap.add_argument("--closure-snapshots", type=int, default=16)
     ap.add_argument("--closure-every", type=int, default=4)

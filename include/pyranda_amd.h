@@ -156,16 +156,21 @@ int pyr_apply(pyr_ctx* ctx, int op, int dir, int prec, const void* in, void* out
  * on one GPU (other streams, other threads) race. */
 int pyr_sfilter_noalloc(pyr_ctx* ctx, int prec, const void* val, void* dval);
 
-/* --- FP64 operators that also store the FP32 rounding of their result ----
- * For FP32 runs that keep an FP64 state through the conservative filter
- * (the state is filtered in FP64, the field is its rounding). No counterpart
- * in the reference (FP64 only); the operators are those of
+/* --- FP64 operators that store the FP32 rounding of their result --------
+ * For FP32 runs that keep some fields in FP64: the conservative filter of an
+ * FP64 state (the field is its rounding), and operators of FP64 fields whose
+ * result is stored in FP32 (e.g. a velocity derivative). No counterpart in
+ * the reference (FP64 only); the operators are those of
  *   pyr_sfilter_f64_rounded: parcop.f90:337 sFilter (fbar), out32[i] = (float)dval[i]
- *   pyr_apply_rounded:       compact_operators.f90:13-497, one y (dir 1) or
- *                            z (dir 2) sweep of FP64 fields with PYR_EPI_STORE
+ *   pyr_apply_rounded:       compact_operators.f90:13-497, one sweep of an
+ *                            FP64 field with PYR_EPI_STORE, FP64 result in out
+ *                            and its rounding in out32
+ *   pyr_apply_to_r32:        the same sweep storing only the rounding (no FP64
+ *                            result is written: 8 + 4 bytes per point)
  * out32 holds as many floats as the field has points. */
 int pyr_sfilter_f64_rounded(pyr_ctx* ctx, const double* val, double* dval, float* out32);
 int pyr_apply_rounded(pyr_ctx* ctx, int op, int dir, const double* in, double* out, float* out32);
+int pyr_apply_to_r32(pyr_ctx* ctx, int op, int dir, const double* in, float* out32);
 
 /* --- one RK4 stage update of a conserved variable (pyranda.py:800-804) ----
  * PHI <- dt*F + A*PHI (in place, FP64; with A == 0 PHI is only written), then
@@ -273,6 +278,14 @@ const char* pyr_pw_last_error(void);
 #define PYR_MLP_GRAD_FLOATS 7313
 int pyr_mlp_infer(void* stream, int prec, int nx, int ny, int periodic_x, int periodic_y,
                   const void* const* features, const void* params, void* const* out);
+/* pyr_mlp_infer plus a per-point sensor: with sensor > 0 a point is flagged
+ * when the largest |input| of its 27 normalised inputs exceeds `sensor`;
+ * mask (one byte per point, when not NULL) receives 1 for a flagged point
+ * and 0 otherwise, and the number of flagged points is added to *count (when
+ * not NULL; the caller clears it). The outputs are written at every point. */
+int pyr_mlp_infer_sensed(void* stream, int prec, int nx, int ny, int periodic_x, int periodic_y,
+                         const void* const* features, const void* params, void* const* out, float sensor,
+                         unsigned char* mask, int* count);
 int pyr_mlp_train_grad(void* stream, int prec, int nx, int ny, int periodic_x, int periodic_y,
                        const void* const* features, const void* const* targets, const void* params,
                        float* partial, int npartial, float* grad);

@@ -52,6 +52,9 @@ SIGNATURES = {
                                                _vp, _vp, ctypes.c_double, ctypes.c_double]),
     "pyr_mlp_infer": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                      ctypes.c_int, ctypes.POINTER(_vp), _vp, ctypes.POINTER(_vp)]),
+    "pyr_mlp_infer_sensed": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                            ctypes.c_int, ctypes.POINTER(_vp), _vp, ctypes.POINTER(_vp),
+                                            ctypes.c_float, _vp, _vp]),
     "pyr_mlp_train_grad": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp,
                                           _vp, ctypes.c_int, _vp]),
@@ -66,6 +69,7 @@ SIGNATURES = {
     "pyr_set_solve_tuning": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]),
     "pyr_sfilter_f64_rounded": (ctypes.c_int, [_vp, _vp, _vp, _vp]),
     "pyr_apply_rounded": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp, _vp]),
+    "pyr_apply_to_r32": (ctypes.c_int, [_vp, ctypes.c_int, ctypes.c_int, _vp, _vp]),
     "pyr_sfilter_noalloc": (ctypes.c_int, [_vp, ctypes.c_int, _vp, _vp]),
 }
 PW_MAX_PTRS, PW_MAX_SCALARS = 24, 16

@@ -14,11 +14,19 @@ squared error on the matrix cores; Adam on the FP32 master weights).
 
 `pyrandaLearnedLAD` exposes it to EOM strings as
     [:beta:,:adiff:] = lad.closure(:div:,:vx:,:uy:,:rho:,:Yh:,:dt:)
-and falls back to the plain chain for a whole evaluation when a cheap
-sensor flags inputs outside the range seen in training (the largest
-normalised |feature| over the grid above `sensor_factor` times the
-training maximum). `judge()` reports the closure error against the plain
-chain on the current state.
+with the plain chain as fallback, at two levels:
+  * per point: the inference kernel flags every point whose normalised
+    inputs reach beyond `sensor_factor` times the largest seen in training
+    (inputs the network was never fitted on); when any point is flagged the
+    plain chain is evaluated and its values replace the closure's at the
+    flagged points (one host read of the flagged count per evaluation);
+  * per run: every `judge_every` evaluations the judge compares the closure
+    with the plain chain on the current state; a relative L2 error above
+    `judge_tol` in beta or adiff switches the closure off for the rest of
+    the run (every later evaluation is the plain chain).
+An untrained closure (no training range) falls back on every evaluation.
+`judge()` reports the closure error against the plain chain on the current
+state.
 
 No counterpart in the reference.
 """
@@ -105,18 +113,27 @@ class MLPClosure:
             out.append(t if t.permute(2, 1, 0).is_contiguous() else t.permute(2, 1, 0).contiguous().permute(2, 1, 0))
         return out
 
-    def infer(self, feats, periodic):
-        """(out0, out1) fields from three feature fields of shape (nx, ny, 1)."""
+    def infer(self, feats, periodic, sensor=0.0):
+        """(out0, out1) fields from three feature fields of shape (nx, ny, 1).
+        With sensor > 0 also (mask, count): mask (uint8, the fields' shape)
+        is 1 at points whose largest normalised |input| exceeds `sensor`,
+        count (a 1-element int32 device tensor) the number of such points."""
         feats = self._contig(feats)
         nx, ny, nz = feats[0].shape
         if nz != 1:
             raise ValueError("the learned closure is built for 2-D grids (nz == 1)")
         outs = [torch.empty_like(feats[0]) for _ in range(2)]
         stream = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
-        _lib.check(_lib.lib().pyr_mlp_infer(stream, self._prec(feats[0]), nx, ny, int(periodic[0]),
-                                            int(periodic[1]), self._ptrs(feats), ctypes.c_void_p(self.blob.data_ptr()),
-                                            self._ptrs(outs)))
-        return outs
+        args = (stream, self._prec(feats[0]), nx, ny, int(periodic[0]), int(periodic[1]), self._ptrs(feats),
+                ctypes.c_void_p(self.blob.data_ptr()), self._ptrs(outs))
+        if sensor <= 0.0:
+            _lib.check(_lib.lib().pyr_mlp_infer(*args))
+            return outs
+        mask = torch.empty((1, ny, nx), dtype=torch.uint8, device=feats[0].device).permute(2, 1, 0)
+        count = torch.zeros(1, dtype=torch.int32, device=feats[0].device)
+        _lib.check(_lib.lib().pyr_mlp_infer_sensed(*args, ctypes.c_float(sensor), ctypes.c_void_p(mask.data_ptr()),
+                                                   ctypes.c_void_p(count.data_ptr())))
+        return outs, mask, count
 
     def gradient(self, feats, targets, periodic):
         """(gradient dict, mean squared error) of one sample on the MFMA kernels."""
@@ -180,16 +197,21 @@ class pyrandaLearnedLAD(pyrandaPackage):
     arguments are the simulation's own fields; the plain chain reads them
     through its EOM lines (`plain_lines`, written to the same variables)."""
 
-    def __init__(self, pysim, model, sensor_factor=1.5, plain_lines=RM_PLAIN_LAD):
+    def __init__(self, pysim, model, sensor_factor=1.5, plain_lines=RM_PLAIN_LAD, judge_every=0, judge_tol=1.0):
         pyrandaPackage.__init__(self, "LearnedLAD", pysim)
         self.pysim = pysim
         self.model = model
         self.sensor_factor = sensor_factor
         self.plain_lines = plain_lines
+        self.judge_every = int(judge_every)
+        self.judge_tol = float(judge_tol)
         self._plain_eqs = None
         self.enabled = True
         self.calls = 0
-        self.fallbacks = 0
+        self.fallbacks = 0          # evaluations that were the plain chain throughout
+        self.partial = 0            # evaluations with flagged points
+        self.points_flagged = 0
+        self.judged = []            # (call, {name: (rel_l2, rel_max)})
         p = pysim.PyMPI.periodic
         self.periodic = (bool(p[0]), bool(p[1]))
 
@@ -219,27 +241,36 @@ class pyrandaLearnedLAD(pyrandaPackage):
                 s.variables[name].data = v
         return out["beta"], out["adiff"]
 
-    def flagged(self, feats):
-        """Cheap sensor: normalised inputs beyond the training range."""
-        m = self.model
-        if m.fmax is None:
-            return True
-        big = max(float(((f - m.fshift[i]) * m.fscale[i]).abs().max()) for i, f in enumerate(feats))
-        return big > self.sensor_factor * m.fmax
-
     def closure(self, div, vx, uy, rho, Yh, dt):
         self.calls += 1
-        feats = self.features(div, vx, uy, rho)
-        if not self.enabled or self.flagged(feats):
+        m = self.model
+        if not self.enabled or m is None or m.fmax is None:
             self.fallbacks += 1
             return list(self.plain())
-        beta, adiff = self.model.infer(feats, self.periodic)
+        feats = self.features(div, vx, uy, rho)
+        if self.judge_every and self.calls % self.judge_every == 0:
+            rep = self.judge(div, vx, uy, rho, Yh, dt, feats=feats)
+            self.judged.append((self.calls, rep))
+            if max(rep["beta"][0], rep["adiff"][0]) > self.judge_tol:
+                self.enabled = False
+                self.fallbacks += 1
+                return list(self.plain())
+        (beta, adiff), mask, count = m.infer(feats, self.periodic, sensor=self.sensor_factor * m.fmax)
+        n = int(count.item())
+        if n:
+            self.partial += 1
+            self.points_flagged += n
+            pb, pa = self.plain()
+            sel = mask.bool()
+            beta = torch.where(sel, pb.to(beta.dtype), beta)
+            adiff = torch.where(sel, pa.to(adiff.dtype), adiff)
         return [beta, adiff]
 
-    def judge(self, div, vx, uy, rho, Yh, dt):
+    def judge(self, div, vx, uy, rho, Yh, dt, feats=None):
         """Closure error against the plain chain on the same state:
         {name: (rel_l2, rel_max)} for beta and adiff."""
-        feats = self.features(div, vx, uy, rho)
+        if feats is None:
+            feats = self.features(div, vx, uy, rho)
         got = self.model.infer(feats, self.periodic)
         ref = self.plain()
         out = {}

@@ -435,7 +435,7 @@ __device__ __forceinline__ void line_yz_body(const LineArgs& a, long long blk, c
     // the output in place
     if constexpr (PRE == PRE_V) old = (a.epi == EPI_ADD || a.epi == EPI_MAXABS) ? ld(rout, s + k) : (TS)0;
     const TS r = epilogue<TS>(a.epi, a.epi_c, val, old);
-    st(r, s + k);
+    if (sizeof(TS) != 8 || GRP || !a.only32) st(r, s + k);
     // (store epilogues only: the accumulating variants keep their registers)
     if constexpr (sizeof(TS) == 8 && PRE != PRE_OLD && !GRP)
       if (a.out32) {
@@ -705,6 +705,23 @@ __device__ __forceinline__ void line_x_body(const LineArgs& a, long long blk, co
   __syncthreads();
   TS* dst = out + L0 * N;
   const bool need_old = (a.epi == EPI_ADD || a.epi == EPI_MAXABS);
+  if constexpr (sizeof(TS) == 8 && !GRP) {
+    if (a.out32) {
+      // FP64 result rounded to FP32 (store epilogue), with or without the
+      // FP64 store
+      float* d32 = a.out32 + L0 * N;
+      const int ne = nval * N;
+#pragma unroll XU
+      for (int e = tid; e < ne; e += nthr) {
+        const int ll = e / N, i = e - ll * N;
+        double val = (double)tile[ll * pitch + i];
+        if (!VTILE && a.addv) val = (double)(TS)(val + (double)src[e]);
+        if (!a.only32) dst[e] = (TS)val;
+        d32[e] = (float)val;
+      }
+      return;
+    }
+  }
   if (a.vec) {
     const int vpl = N / VW, nvec = nval * vpl;
     const V* sv = reinterpret_cast<const V*>(src);
@@ -889,7 +906,7 @@ __global__ void k_null(long long n, int copy, int epi, double c, const TS* __res
   const double val = copy ? (double)in[i] : 0.0;
   const TS old = (epi == EPI_ADD || epi == EPI_MAXABS) ? out[i] : (TS)0;
   const TS r = epilogue<TS>(epi, c, val, old);
-  out[i] = r;
+  if (out) out[i] = r;
   if (out32) out32[i] = (float)r;
 }
 
@@ -974,7 +991,8 @@ struct pyr_ctx {
   }
   int ncu = 256;                             // compute units of the device
   void* noalloc_tmp = nullptr;               // this device's copy of `scratch` (pyr_sfilter_noalloc)
-  float* out32 = nullptr;                    // FP32 copy of the next FP64 y/z sweep's output
+  float* out32 = nullptr;                    // FP32 copy of the next FP64 sweep's output
+  int only32 = 0;                            // ... and no FP64 store
   size_t npts() const { return (size_t)n[0] * n[1] * n[2]; }
 };
 
@@ -1220,7 +1238,7 @@ static int launch_typed(pyr_ctx* c, int dir, LineArgs a, const TS* in, TS* out) 
     if constexpr (FULL && (SOLVE == SOLVE_CIRC || SOLVE == SOLVE_NONE)) {
       // pipelined persistent kernel: two tiles of LX lines (16-byte aligned
       // rows, whole KiB per line), LX * P threads in whole waves
-      const bool ok = c->yz_pipe && a.periodic && (a.N * (int)sizeof(TS)) % 1024 == 0 &&
+      const bool ok = c->yz_pipe && !a.out32 && a.periodic && (a.N * (int)sizeof(TS)) % 1024 == 0 &&
                       ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 16 == 0);
       if (ok) {
         const int pitch = a.N + VW;
@@ -1347,17 +1365,20 @@ static int launch_op(pyr_ctx* c, int op, int dir, const TS* in, TS* out, int epi
   const OpPlan& pl = ua ? c->plan_a[op][dir] : c->plan[op][dir];
   if (pl.null_op) {
     const long long n = (long long)c->npts();
+    float* o32 = sizeof(TS) == 8 ? c->out32 : nullptr;
     hipLaunchKernelGGL((k_null<TS>), dim3((n + 255) / 256), dim3(256), 0, c->stream, n,
-                       pl.null_option == 1 ? 1 : 0, epi, ec, in, out, sizeof(TS) == 8 ? c->out32 : nullptr);
+                       pl.null_option == 1 ? 1 : 0, epi, ec, in, (o32 && c->only32) ? nullptr : out, o32);
     HIPCHECK(hipGetLastError());
     return 0;
   }
   LineArgs a = make_args(c, pl, ua ? c->dplan_a[op][dir] : c->dplan[op][dir], dir, epi, ec,
                          MODE == MODE_F64 ? PYR_F64 : PYR_R32);
   if (c->out32) {
-    // only the y/z kernels of FP64 fields write the rounded copy
-    if (sizeof(TS) != 8 || dir == 0) return fail("an FP32 copy of the output needs an FP64 y or z sweep");
+    // the plain kernels of FP64 fields with the store epilogue write the
+    // rounded copy
+    if (sizeof(TS) != 8 || epi != EPI_STORE) return fail("an FP32 copy of the output needs an FP64 sweep that stores");
     a.out32 = c->out32;
+    a.only32 = c->only32;
   }
   const int sv = plan_solve(c, pl);
   // the balanced partition gives every chunk CHUNK rows exactly when CHUNK | N;
@@ -1366,6 +1387,7 @@ static int launch_op(pyr_ctx* c, int op, int dir, const TS* in, TS* out, int epi
   const int sx = c->x_lds_kb, sp = c->yz_pipe, sl = c->yz_lines;
   if (c->tune_xlds[op][dir][MODE] >= 0) c->x_lds_kb = c->tune_xlds[op][dir][MODE];
   if (c->tune_pipe[op][dir][MODE] >= 0) c->yz_pipe = c->tune_pipe[op][dir][MODE];
+  else if (dir == 0) c->yz_pipe = 0;  // the pipelined x kernel only where tuning chose it
   if (c->tune_ly[op][dir][MODE] >= 0) c->yz_lines = c->tune_ly[op][dir][MODE];
   const int r = (PYR_FULL_CHUNKS && pl.N % CHUNK == 0 && sv != SOLVE_LU)
                     ? launch_kind<TS, MODE, true>(c, dir, pl.kind, sv, a, in, out)
@@ -1392,8 +1414,11 @@ static int apply_any(pyr_ctx* c, int op, int dir, int prec, const void* in, void
                      double ec, bool anti = false) {
   if (check_ctx(c)) return 1;
   if (op < 0 || op >= NOPS || dir < 0 || dir > 2) return fail("bad operator or direction");
-  if (!in || !out) return fail("null field pointer");
-  if (in == out) return fail("input and output must not alias");
+  // (an FP64 sweep that stores only the FP32 rounding has no FP64 output)
+  const bool only32 = c->out32 && c->only32;
+  if (!in || (!out && !only32)) return fail("null field pointer");
+  if (in == out || (void*)in == (void*)c->out32) return fail("input and output must not alias");
+  if (only32 && prec != PYR_F64) return fail("an FP32-only result needs an FP64 sweep");
   if (prec == PYR_F64)
     return launch_op<double, MODE_F64>(c, op, dir, (const double*)in, (double*)out, epi, ec, anti);
   if (prec == PYR_R32) {
@@ -2111,11 +2136,20 @@ int pyr_pringv_f64(pyr_ctx* c, const double* vx, const double* vy, const double*
 int pyr_sfilter_f64(pyr_ctx* c, const double* v, double* o) { return filter3(c, OP_SF, PYR_F64, v, o); }
 int pyr_apply_rounded(pyr_ctx* c, int op, int dir, const double* in, double* out, float* o32) {
   if (check_ctx(c)) return 1;
-  if (!o32) return fail("null field pointer");
-  if (dir != 1 && dir != 2) return fail("the FP32 copy is written by y and z sweeps only");
+  if (!o32 || !out) return fail("null field pointer");
   c->out32 = o32;
   const int r = apply_any(c, op, dir, PYR_F64, in, out, EPI_STORE, 0.0);
   c->out32 = nullptr;
+  return r;
+}
+int pyr_apply_to_r32(pyr_ctx* c, int op, int dir, const double* in, float* o32) {
+  if (check_ctx(c)) return 1;
+  if (!o32 || !in) return fail("null field pointer");
+  c->out32 = o32;
+  c->only32 = 1;
+  const int r = apply_any(c, op, dir, PYR_F64, in, nullptr, EPI_STORE, 0.0);
+  c->out32 = nullptr;
+  c->only32 = 0;
   return r;
 }
 int pyr_sfilter_f64_rounded(pyr_ctx* c, const double* v, double* o, float* o32) {

@@ -66,9 +66,11 @@ struct LineArgs {
   const double* kb;
   const float* kf32;
   const float* kb32;
-  // FP64 y/z kernels: when set, every stored value is also written here
-  // rounded to FP32 (same element index as the output)
+  // FP64 sweeps with the store epilogue: when set, every stored value is also
+  // written here rounded to FP32 (same element index as the output); with
+  // only32 the FP64 result itself is not stored
   float* out32;
+  int only32;
 };
 
 template <int KIND> struct Halo { static constexpr int R = (KIND == K_R4) ? 4 : 3; };

@@ -5,15 +5,15 @@
 // outputs (bulk viscosity beta and species diffusivity adiff):
 //     h1 = relu(W1^T x + b1), h2 = relu(W2^T h1 + b2), o = W3^T h2 + b3
 // with 64 hidden units per layer, BF16 weights and activations and FP32
-// accumulation on the matrix cores (v_mfma_f32_16x16x16_bf16).
+// accumulation on the matrix cores.
 //
-// Every layer is computed transposed, H^T = W^T X^T, on 16-point tiles: the
-// 16x16 accumulator tile of one MFMA (lane l holds rows 4(l/16)..+3 of column
-// l%16) has the layout of the B operand of the next MFMA, and tiles pass
-// between layers through a per-wave LDS slice in BF16. Training runs the
-// same forward pass, the backward pass (dH2 = W3 dO, dH1 = W2 dZ2) and the
+// Inference (k_mlp_infer32) runs on v_mfma_f32_16x16x32_bf16 with the weights
+// and the activations in registers (see the comment above it); it also
+// flags, per point, inputs beyond a sensor threshold. Training (k_mlp_train)
+// runs the forward pass, the backward pass (dH2 = W3 dO, dH1 = W2 dZ2) and the
 // weight gradients (dW = A^T dZ summed over points, points as the MFMA K
-// dimension) in one kernel; each workgroup writes one partial gradient and a
+// dimension) in one kernel on v_mfma_f32_16x16x16_bf16, tiles passing through
+// a per-wave LDS slice; each workgroup writes one partial gradient and a
 // second kernel sums the partials in a fixed order (deterministic).
 //
 // No counterpart in the reference: pyranda evaluates the LAD chain with
@@ -85,10 +85,13 @@ struct MlpArgs {
   float inv_n;       // training: 1 / number of points
 };
 
-template <typename TS, bool TRAIN>
-__global__ void __launch_bounds__(64 * WAVES) k_mlp(MlpArgs<TS> a) {
-  constexpr int ACT = TRAIN ? 16 * (NIN + 4 * NH + NO) : 16 * (NIN + 2 * NH);
-  static_assert(!TRAIN || WAVES * ACT * 2 >= GRAD * 4, "reduction buffer");
+// Training: forward and backward of 16-point tiles on 16x16x16 BF16 tiles
+// (the weight gradients take the 16 points of a tile as the K dimension),
+// activations in a per-wave LDS slice, one partial gradient per workgroup.
+template <typename TS>
+__global__ void __launch_bounds__(64 * WAVES) k_mlp_train(MlpArgs<TS> a) {
+  constexpr int ACT = 16 * (NIN + 4 * NH + NO);
+  static_assert(WAVES * ACT * 2 >= GRAD * 4, "reduction buffer");
   __shared__ unsigned short W1s[NIN * NH], W2s[NH * NH], W3s[NH * NO];
   __shared__ float B1s[NH], B2s[NH], B3s[NO], NRM[12];
   __shared__ __align__(16) unsigned short act[WAVES * ACT];
@@ -108,22 +111,19 @@ __global__ void __launch_bounds__(64 * WAVES) k_mlp(MlpArgs<TS> a) {
   unsigned short* Xs = act + w * ACT;             // [16 points][NIN]
   unsigned short* H1s = Xs + 16 * NIN;            // [16][NH]
   unsigned short* H2s = H1s + 16 * NH;            // [16][NH]
-  unsigned short* dOs = H2s + 16 * NH;            // [16][NO]   (training)
+  unsigned short* dOs = H2s + 16 * NH;            // [16][NO]
   unsigned short* dZ2s = dOs + 16 * NO;           // [16][NH]
   unsigned short* dZ1s = dZ2s + 16 * NH;          // [16][NH]
-  if (TRAIN)
-    for (int i = l; i < 16 * NO; i += 64) dOs[i] = 0;
+  for (int i = l; i < 16 * NO; i += 64) dOs[i] = 0;
   __syncthreads();
 
   f32x4 acc1[2][4], acc2[4][4], acc3[4];
   float gb1 = 0.f, gb2 = 0.f, gb3 = 0.f, loss = 0.f;
-  if (TRAIN) {
-    for (int i = 0; i < 2; ++i)
-      for (int j = 0; j < 4; ++j) acc1[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int i = 0; i < 4; ++i) {
-      acc3[i] = f32x4{0.f, 0.f, 0.f, 0.f};
-      for (int j = 0; j < 4; ++j) acc2[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    }
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 4; ++j) acc1[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int i = 0; i < 4; ++i) {
+    acc3[i] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < 4; ++j) acc2[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
   }
 
   const long long npts = (long long)a.nx * a.ny;
@@ -183,12 +183,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_mlp(MlpArgs<TS> a) {
 #pragma unroll
     for (int kb = 0; kb < NH / 16; ++kb)
       o = mma(frag(&W3s[(16 * kb + 4 * g) * NO + li], NO), frag(&H2s[li * NH + 16 * kb + 4 * g], 1), o);
-    if constexpr (!TRAIN) {
-      if (g == 0 && valid) {
-        a.o[0][p] = (TS)(fmaxf(o[0] + B3s[0], 0.f) * NRM[8]);
-        a.o[1][p] = (TS)(fmaxf(o[1] + B3s[1], 0.f) * NRM[9]);
-      }
-    } else {
+    {
       if (g == 0) {
 #pragma unroll
         for (int j = 0; j < 2; ++j) {
@@ -250,7 +245,7 @@ __global__ void __launch_bounds__(64 * WAVES) k_mlp(MlpArgs<TS> a) {
     __syncthreads();
   }
 
-  if constexpr (TRAIN) {
+  {
     for (int off = 32; off > 0; off >>= 1) loss += __shfl_xor(loss, off, 64);
     float* red = reinterpret_cast<float*>(act);  // the activation slices are free now
     for (int turn = 0; turn < WAVES; ++turn) {
@@ -281,6 +276,146 @@ __global__ void __launch_bounds__(64 * WAVES) k_mlp(MlpArgs<TS> a) {
     }
     float* dst = a.partial + (size_t)blockIdx.x * GRAD;
     for (int i = tid; i < GRAD; i += 64 * WAVES) dst[i] = red[i];
+  }
+}
+
+// --- inference on gfx950's 16x16x32 BF16 matrix cores ----------------------
+// v_mfma_f32_16x16x32_bf16: A is 16 x 32 (lane l holds row l%16, eight K
+// values of K group l/16), B is 32 x 16 (lane l holds column l%16, eight K
+// values of K group l/16), D is 16 x 16 (lane l holds rows 4(l/16)..+3 of
+// column l%16). A and B take their K values from the same (lane group,
+// element) slots, so the order of the 32 K values inside a block is free as
+// long as both operands use it. Every layer is computed transposed (rows =
+// units, columns = the 16 points of a tile); the units of a 32-wide K block
+// are ordered so that lane group g's eight B values are rows 4g..4g+3 of two
+// D tiles of the layer below: slot (g, e) of block kb is unit
+//     32 kb + 16 (e / 4) + 4 g + e % 4.
+// A layer's output therefore feeds the next layer straight from registers
+// (no LDS, no barrier), and each lane keeps its A fragments of all three
+// weight matrices in registers for the whole persistent loop.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef short s16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ f32x4 mma32(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ int kunit(int kb, int g, int e) { return 32 * kb + 16 * (e >> 2) + 4 * g + (e & 3); }
+
+// relu(d + b) of two D tiles (rows 4g..4g+3 each), as one B operand
+__device__ __forceinline__ bf16x8 pack_relu(f32x4 d0, const float* b0, f32x4 d1, const float* b1) {
+  s16x8 r;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    r[i] = (short)f2bf(fmaxf(d0[i] + b0[i], 0.f));
+    r[4 + i] = (short)f2bf(fmaxf(d1[i] + b1[i], 0.f));
+  }
+  return __builtin_bit_cast(bf16x8, r);
+}
+
+template <typename TS>
+struct InferArgs {
+  int nx, ny, px, py;
+  const TS* f[3];
+  TS* o[2];
+  const unsigned char* params;
+  float sensor;            // > 0: flag points whose largest normalised |input| exceeds it
+  unsigned char* mask;     // per point 1 (flagged) or 0, when not null
+  int* count;              // flagged points are added here, when not null
+};
+
+template <typename TS>
+__global__ void __launch_bounds__(64 * WAVES) k_mlp_infer32(InferArgs<TS> a) {
+  const int l = threadIdx.x & 63, li = l & 15, g = l >> 4;
+  const unsigned short* pw = reinterpret_cast<const unsigned short*>(a.params);
+  const float* pf = reinterpret_cast<const float*>(a.params + PYR_MLP_PARAM_FLOAT_OFFSET);
+  // A fragments: W^T rows (output units) of this lane, K slots of its group
+  bf16x8 A1[NH / 16], A2[NH / 16][2], A3[2];
+#pragma unroll
+  for (int mt = 0; mt < NH / 16; ++mt) {
+    s16x8 r;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) r[e] = (short)pw[(8 * g + e) * NH + 16 * mt + li];  // input k = 8g + e
+    A1[mt] = __builtin_bit_cast(bf16x8, r);
+#pragma unroll
+    for (int kb = 0; kb < 2; ++kb) {
+      s16x8 q;
+#pragma unroll
+      for (int e = 0; e < 8; ++e) q[e] = (short)pw[OFF_W2 + kunit(kb, g, e) * NH + 16 * mt + li];
+      A2[mt][kb] = __builtin_bit_cast(bf16x8, q);
+    }
+  }
+#pragma unroll
+  for (int kb = 0; kb < 2; ++kb) {
+    s16x8 q;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) q[e] = (short)pw[OFF_W3 + kunit(kb, g, e) * NO + li];
+    A3[kb] = __builtin_bit_cast(bf16x8, q);
+  }
+  // biases of this lane's D rows (units 16 mt + 4 g + i)
+  float b1[NH / 16][4], b2[NH / 16][4], b3[4];
+#pragma unroll
+  for (int mt = 0; mt < NH / 16; ++mt)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      b1[mt][i] = pf[16 * mt + 4 * g + i];
+      b2[mt][i] = pf[NH + 16 * mt + 4 * g + i];
+    }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) b3[i] = pf[2 * NH + 4 * g + i];
+  const float* nrm = pf + 2 * NH + NO;  // fshift[4] fscale[4] oscale[4]
+  const float sh[3] = {nrm[0], nrm[1], nrm[2]}, sc[3] = {nrm[4], nrm[5], nrm[6]};
+  const float os0 = nrm[8], os1 = nrm[9];
+
+  const long long npts = (long long)a.nx * a.ny;
+  const long long ntiles = (npts + 15) / 16;
+  const long long twaves = (long long)gridDim.x * WAVES;
+  for (long long t = (long long)blockIdx.x * WAVES + (threadIdx.x >> 6); t < ntiles; t += twaves) {
+    const long long p = t * 16 + li;
+    const bool valid = p < npts;
+    const int pi = valid ? (int)(p % a.nx) : 0, pj = valid ? (int)(p / a.nx) : 0;
+    // this lane's eight inputs k = 8g + e of point p
+    float big = 0.f;
+    s16x8 xr;
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const int k = 8 * g + e;
+      float v = 0.f;
+      if (valid && k < NF) {
+        const int f = k / 9, r = k % 9;
+        const int ii = wrap_clamp(pi + r % 3 - 1, a.nx, a.px);
+        const int jj = wrap_clamp(pj + r / 3 - 1, a.ny, a.py);
+        v = ((float)a.f[f][(long long)jj * a.nx + ii] - sh[f]) * sc[f];
+        big = fmaxf(big, fabsf(v));
+      }
+      xr[e] = (short)f2bf(v);
+    }
+    const bf16x8 X = __builtin_bit_cast(bf16x8, xr);
+    const f32x4 z = {0.f, 0.f, 0.f, 0.f};
+    f32x4 d1[NH / 16];
+#pragma unroll
+    for (int mt = 0; mt < NH / 16; ++mt) d1[mt] = mma32(A1[mt], X, z);
+    const bf16x8 H1a = pack_relu(d1[0], b1[0], d1[1], b1[1]);
+    const bf16x8 H1b = pack_relu(d1[2], b1[2], d1[3], b1[3]);
+    f32x4 d2[NH / 16];
+#pragma unroll
+    for (int mt = 0; mt < NH / 16; ++mt) d2[mt] = mma32(A2[mt][1], H1b, mma32(A2[mt][0], H1a, z));
+    const bf16x8 H2a = pack_relu(d2[0], b2[0], d2[1], b2[1]);
+    const bf16x8 H2b = pack_relu(d2[2], b2[2], d2[3], b2[3]);
+    const f32x4 o = mma32(A3[1], H2b, mma32(A3[0], H2a, z));
+    // the point's largest |input| over the four lane groups
+    big = fmaxf(big, __shfl_xor(big, 16, 64));
+    big = fmaxf(big, __shfl_xor(big, 32, 64));
+    const bool flag = valid && a.sensor > 0.f && big > a.sensor;
+    if (g == 0 && valid) {
+      a.o[0][p] = (TS)(fmaxf(o[0] + b3[0], 0.f) * os0);
+      a.o[1][p] = (TS)(fmaxf(o[1] + b3[1], 0.f) * os1);
+      if (a.mask) a.mask[p] = flag ? 1 : 0;
+    }
+    if (a.count) {
+      const unsigned long long bal = __ballot(g == 0 && flag);
+      if (bal && l == 0) atomicAdd(a.count, (int)__popcll(bal));
+    }
   }
 }
 
@@ -327,14 +462,37 @@ int mlp_launch(void* stream, int nx, int ny, int px, int py, const void* const* 
   long long blocks = (ntiles + WAVES - 1) / WAVES;
   const hipStream_t s = static_cast<hipStream_t>(stream);
   if (!grad) {
-    if (blocks > 2048) blocks = 2048;
-    hipLaunchKernelGGL((k_mlp<TS, false>), dim3((unsigned)blocks), dim3(64 * WAVES), 0, s, a);
+    return pyr::report_error("mlp_launch: inference goes through infer_launch");
   } else {
     if (blocks > npartial) blocks = npartial;
-    hipLaunchKernelGGL((k_mlp<TS, true>), dim3((unsigned)blocks), dim3(64 * WAVES), 0, s, a);
+    hipLaunchKernelGGL((k_mlp_train<TS>), dim3((unsigned)blocks), dim3(64 * WAVES), 0, s, a);
     hipLaunchKernelGGL(k_mlp_sum, dim3((GRAD + SUM_I - 1) / SUM_I), dim3(SUM_I * SUM_Q), 0, s,
                        (const float*)partial, (int)blocks, grad);
   }
+  const hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return pyr::report_error(std::string("MLP launch: ") + hipGetErrorString(e));
+  return 0;
+}
+
+template <typename TS>
+int infer_launch(void* stream, int nx, int ny, int px, int py, const void* const* f, void* const* o,
+                 const void* params, float sensor, unsigned char* mask, int* count) {
+  InferArgs<TS> a;
+  a.nx = nx;
+  a.ny = ny;
+  a.px = px ? 1 : 0;
+  a.py = py ? 1 : 0;
+  for (int i = 0; i < 3; ++i) a.f[i] = static_cast<const TS*>(f[i]);
+  for (int i = 0; i < 2; ++i) a.o[i] = static_cast<TS*>(o[i]);
+  a.params = static_cast<const unsigned char*>(params);
+  a.sensor = sensor;
+  a.mask = mask;
+  a.count = count;
+  const long long ntiles = ((long long)nx * ny + 15) / 16;
+  long long blocks = (ntiles + WAVES - 1) / WAVES;
+  if (blocks > 2048) blocks = 2048;  // persistent: every wave loops over tiles
+  hipLaunchKernelGGL((k_mlp_infer32<TS>), dim3((unsigned)blocks), dim3(64 * WAVES), 0,
+                     static_cast<hipStream_t>(stream), a);
   const hipError_t e = hipGetLastError();
   if (e != hipSuccess) return pyr::report_error(std::string("MLP launch: ") + hipGetErrorString(e));
   return 0;
@@ -352,15 +510,20 @@ int check_args(int prec, int nx, int ny, const void* const* f, const void* param
 
 extern "C" {
 
-int pyr_mlp_infer(void* stream, int prec, int nx, int ny, int periodic_x, int periodic_y,
-                  const void* const* features, const void* params, void* const* out) {
+int pyr_mlp_infer_sensed(void* stream, int prec, int nx, int ny, int periodic_x, int periodic_y,
+                         const void* const* features, const void* params, void* const* out, float sensor,
+                         unsigned char* mask, int* count) {
   if (check_args(prec, nx, ny, features, params)) return 1;
   if (!out || !out[0] || !out[1]) return pyr::report_error("null output pointer to pyr_mlp_infer");
   if (prec == PYR_F64)
-    return mlp_launch<double>(stream, nx, ny, periodic_x, periodic_y, features, nullptr, out, params, nullptr, 0,
+    return infer_launch<double>(stream, nx, ny, periodic_x, periodic_y, features, out, params, sensor, mask, count);
+  return infer_launch<float>(stream, nx, ny, periodic_x, periodic_y, features, out, params, sensor, mask, count);
+}
+
+int pyr_mlp_infer(void* stream, int prec, int nx, int ny, int periodic_x, int periodic_y,
+                  const void* const* features, const void* params, void* const* out) {
+  return pyr_mlp_infer_sensed(stream, prec, nx, ny, periodic_x, periodic_y, features, params, out, 0.f, nullptr,
                               nullptr);
-  return mlp_launch<float>(stream, nx, ny, periodic_x, periodic_y, features, nullptr, out, params, nullptr, 0,
-                           nullptr);
 }
 
 int pyr_mlp_train_grad(void* stream, int prec, int nx, int ny, int periodic_x, int periodic_y,

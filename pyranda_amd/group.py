@@ -15,10 +15,13 @@ update and every operator is ONE launch for the whole ensemble:
   and rk4 takes dt as one value per member, so each member keeps its own
   time step, exactly as it would run alone.
 
-All members share the EOM, the precision and the boundary pattern (periodic
-directions; a direction with a single point in every member is a null
-direction). No counterpart in the reference, which advances one simulation
-per MPI communicator (pyranda/pyranda.py:32-100).
+All members share the EOM, the precision and the boundary kind of each
+direction: periodic, or bounded (NONE ends, with pyrandaBC's extrap/const on
+every member's boundary planes); SYMM ends are not grouped, and a direction
+with a single point in every member is a null direction. Lines may differ in
+length between members, up to 1024 points. No counterpart in the reference,
+which advances one simulation per MPI communicator
+(pyranda/pyranda.py:32-100).
 """
 import ctypes
 

@@ -257,6 +257,25 @@ class ParcopContext:
             self.timing.setdefault((op, direction, key), []).append((ev0, ev1))
         return out
 
+    def apply_to32(self, op, direction, v):
+        """One directional operator of an FP64 field whose result is stored
+        only as its FP32 rounding (pyr_apply_to_r32): the FP64 sweep reads 8
+        and writes 4 bytes per point."""
+        v = self._prep(v)
+        if v.dtype != torch.float64:
+            raise ValueError("apply_to32 sweeps FP64 fields")
+        out32 = empty_field(self.nn, torch.float32, self.device)
+        self._stream()
+        if self.timing is not None:
+            ev0 = torch.cuda.Event(enable_timing=True)
+            ev1 = torch.cuda.Event(enable_timing=True)
+            ev0.record()
+        _lib.check(self._L.pyr_apply_to_r32(self._h, op, direction, self._ptr(v), self._ptr(out32)))
+        if self.timing is not None:
+            ev1.record()
+            self.timing.setdefault((op, direction, "torch.float64->float32"), []).append((ev0, ev1))
+        return out32
+
     def timing_summary(self):
         """{(op, dir, dtype): (launches, mean ms)} of the recorded launches."""
         torch.cuda.synchronize(self.device)

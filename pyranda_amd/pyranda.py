@@ -40,6 +40,7 @@ as with host scalars, bit for bit.
 """
 import math
 import random
+import re
 import sys
 import time as _time
 
@@ -146,17 +147,31 @@ class pyrandaSim:
     def autotune(self, **kw):
         """Tune the line-operator launches of this grid and precision on the
         device (ParcopContext.autotune); returns the timing report
-        {(op, dir): (choice, times)}. FP32 runs that filter an FP64 state
-        also tune the FP64 filter, reported under (op, dir, "f64")."""
+        {(op, dir): (choice, times)}. FP32 runs whose operators also see FP64
+        fields (an FP64 state through the conservative filter, fields stored
+        in FP64) tune the FP64 launches too, reported under (op, dir, "f64");
+        the filter's z sweep is timed storing the FP32 rounding as well when
+        the state is filtered in FP64."""
         from . import _lib
         ctx = self.PyMPI.ctx
         rep = ctx.autotune(self.dtype, **kw)
-        if self.dtype != torch.float64 and self._self_filtered():
-            kw64 = dict(kw)
-            kw64["ops"] = (_lib.OP_SFILTER,)
-            kw64["rounded"] = self.dtype == torch.float32
-            for (op, d), r in ctx.autotune(torch.float64, **kw64).items():
-                rep[(op, d, "f64")] = r
+        if self.dtype != torch.float64:
+            wide = any(v == "f64" for v in self._storage.values())
+            filt = self._self_filtered()
+            ops = kw.get("ops") or (_lib.OP_D1, _lib.OP_D2, _lib.OP_D8, _lib.OP_SFILTER, _lib.OP_GFILTER)
+            passes = []
+            if wide:
+                passes.append(([o for o in ops if not (filt and o == _lib.OP_SFILTER)], False))
+            if filt or (wide and _lib.OP_SFILTER in ops):
+                passes.append(([_lib.OP_SFILTER], filt and self.dtype == torch.float32))
+            for pops, rounded in passes:
+                if not pops:
+                    continue
+                kw64 = dict(kw)
+                kw64["ops"] = tuple(pops)
+                kw64["rounded"] = rounded
+                for (op, d), r in ctx.autotune(torch.float64, **kw64).items():
+                    rep[(op, d, "f64")] = r
         return rep
 
     # -- bookkeeping -------------------------------------------------------------
@@ -461,6 +476,13 @@ class pyrandaSim:
             self.variables[U].data = out32
             self._mark_state(U)
             return
+        if self.named_storage and eq.rank == 1 and self._storage.get(eq.LHS[0]) == "f32":
+            # a derivative of an FP64 field stored in FP32: the FP64 sweep
+            # stores only the rounding (the value settle_storage would keep)
+            got = self._derivative_to32(eq)
+            if got is not None:
+                self.variables[eq.LHS[0]].data = got
+                return
         root = getattr(eq.RHS, "root", None)
         if self.named_storage and root is not None and eq.rank == 1 and root.out_dtype is None:
             st = self.storage_of(eq.LHS[0])
@@ -483,6 +505,26 @@ class pyrandaSim:
                 self.variables[eq.LHS[ii]].data = rhs[ii]
                 if self.named_storage:
                     self.settle_storage(eq.LHS[ii])
+
+    _D1_LINE = re.compile(r'^\s*self\.dd([xyz])\(self\.variables\["(\w+)"\]\.data\)\s*$')
+
+    def _derivative_to32(self, eq):
+        """FP32 result of a line `:a: = ddx(:b:)` (ddy, ddz) whose argument is
+        an FP64 field, from one FP64 sweep that stores only its rounding;
+        None where the line is not of that form."""
+        m = getattr(eq, "_d1_to32", False)
+        if m is False:
+            m = eq._d1_to32 = self._D1_LINE.match(eq.sRHS or "")
+        if m is None:
+            return None
+        d = "xyz".index(m.group(1))
+        if (self.nx, self.ny, self.nz)[d] <= 1 or m.group(2) not in self.variables:
+            return None
+        v = self.variables[m.group(2)].data
+        if not self._is_field(v) or v.dtype != torch.float64:
+            return None
+        from . import _lib
+        return self.PyMPI.ctx.apply_to32(_lib.OP_D1, d, v)
 
     def updateVars(self, verbose=False):
         # pointwise ALG lines run as fused multi-output kernels where possible

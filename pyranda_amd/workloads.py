@@ -397,18 +397,23 @@ RM_LAD_LINES = (":beta:      =  gbar( abs(ring(:div:)) * :rho: )  * 7.0e-3",
 RM_CLOSURE_LINE = "[:beta:,:adiff:] = lad.closure(:div:,:vx:,:uy:,:rho:,:Yh:,:dt:)"
 
 
-def rm_setup(pr, npts=128, precision="fp64", seed=1234, closure=None, name="RM_2D", device=None):
-    """examples/RM3D.py, 2-D branch, on an int(1.4 npts) x npts grid; the IC's
-    random perturbation draws from numpy's global generator seeded with
-    `seed`. With `closure` (an MLPClosure) the beta and adiff lines become
-    the learned closure (plain chain as fallback)."""
+def rm_setup(pr, npts=128, precision="fp64", seed=1234, closure=None, name="RM_2D", device=None, nx=None,
+             sensor_factor=None):
+    """examples/RM3D.py, 2-D branch, on an nx x npts grid (nx defaults to
+    int(1.4 npts), the script's choice; x spans 2.8 pi along the shock, y
+    2 pi periodic across it); the IC's random perturbation draws from numpy's
+    global generator seeded with `seed`. With `closure` (an MLPClosure) the
+    beta and adiff lines become the learned closure (plain chain as
+    fallback)."""
     import numpy
     from .closure import pyrandaLearnedLAD
+    nx = int(npts * 1.4) if nx is None else int(nx)
     imesh = """
-    xdom = (0.0, 2.8*pi , int(Npts*1.4), periodic=False)
+    xdom = (0.0, 2.8*pi , NX, periodic=False)
     ydom = (0.0, 2*pi*FF,  Npts, periodic=True)
     zdom = (0.0, 2*pi*FF,  1, periodic=True)
-    """.replace("Npts", str(npts)).replace("pi", str(numpy.pi)).replace("FF", str(float(npts - 1) / npts))
+    """.replace("NX", str(nx)).replace("Npts", str(npts)).replace("pi", str(numpy.pi)) \
+        .replace("FF", str(float(npts - 1) / npts))
     kw = {"silent": True, "precision": precision}
     if device is not None:
         kw["device"] = device
@@ -417,7 +422,8 @@ def rm_setup(pr, npts=128, precision="fp64", seed=1234, closure=None, name="RM_2
     ss.addPackage(pr.pyrandaBC(ss))
     eom = RM_EOM
     if closure is not None:
-        ss.addPackage(pyrandaLearnedLAD(ss, closure))
+        ss.addPackage(pyrandaLearnedLAD(ss, closure) if sensor_factor is None
+                      else pyrandaLearnedLAD(ss, closure, sensor_factor=sensor_factor))
         eom = eom.replace(RM_LAD_LINES[0], RM_CLOSURE_LINE)
         for line in RM_LAD_LINES[1:]:
             eom = eom.replace(line + "\n", "")

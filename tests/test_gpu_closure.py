@@ -141,6 +141,133 @@ def test_mlp_gradient_matches_torch(pr, shape, periodic):
     assert float(grads[2][:, 2:].abs().max()) == 0.0
 
 
+@pytest.mark.parametrize("shape,periodic", [((37, 29), (False, True)), ((96, 64), (True, True))])
+def test_mlp_kernels_match_cpu_oracle(pr, shape, periodic):
+    """Outputs and gradients of the matrix-core kernels against the numpy
+    restatement (oracle/mlp_cpu.py) on the same BF16-rounded operands: within
+    a few parts in a thousand of each block's largest magnitude (the kernels
+    accumulate in FP32 in another order, so an activation can round to the
+    neighbouring BF16 value now and then); the loss to 1e-3."""
+    import numpy as np
+    from oracle import mlp_cpu as mc
+    m = _model(pr)
+    nx, ny = shape
+    g = torch.Generator().manual_seed(31 + nx)
+    fields = [torch.randn(ny, nx, generator=g, dtype=torch.float64).t().reshape(nx, ny, 1).to("cuda", torch.float32)
+              for _ in range(3)]
+    targets = [torch.rand(ny, nx, generator=g, dtype=torch.float64).t().reshape(nx, ny, 1).to("cuda", torch.float32)
+               * s for s in (3.0, 0.25)]
+    out = m.infer(fields, periodic)
+    grads, mse = m.gradient(fields, targets, periodic)
+    torch.cuda.synchronize()
+    host = lambda t: t[:, :, 0].double().cpu().numpy()
+    params = [p.float().cpu().numpy() for p in m.params]
+    x = mc.features([host(f) for f in fields], m.fshift.cpu().numpy(), m.fscale.cpu().numpy(), periodic)
+    osc = m.oscale.cpu().numpy()
+    want_out = mc.infer(x, params, osc)
+    for j in range(2):
+        got = out[j][:, :, 0].t().reshape(-1).cpu().numpy()
+        err = float(np.abs(got - want_out[j]).max()) / (float(np.abs(want_out[j]).max()) or 1.0)
+        _say("  output", j, "max err / max", err)
+        assert err <= 4e-3, (j, err)
+    tg = [host(t).T.reshape(-1) for t in targets]
+    want, loss = mc.gradient(x, tg, params, osc)
+    for name, gk, ref in zip(["gW1", "gW2", "gW3", "gb1", "gb2", "gb3"], grads, want):
+        err = float(np.abs(gk.cpu().numpy() - ref).max()) / (float(np.abs(ref).max()) or 1.0)
+        _say("  ", name, "max err / max", err)
+        assert err <= 1e-2, (name, err)
+    assert abs(float(mse) * nx * ny - loss) <= 1e-3 * loss
+
+
+def test_mlp_sensor_flags_points(pr):
+    """pyr_mlp_infer_sensed: a point is flagged exactly when one of its 27
+    normalised inputs (its 3x3 neighbourhood, wrapped or clamped) exceeds the
+    threshold; the count is the number of flagged points and the outputs
+    are those of the plain inference."""
+    import numpy as np
+    from oracle import mlp_cpu as mc
+    m = _model(pr)
+    nx, ny = 53, 40
+    g = torch.Generator().manual_seed(5)
+    fields = [torch.randn(ny, nx, generator=g, dtype=torch.float64).t().reshape(nx, ny, 1).to("cuda") for _ in range(3)]
+    fields[1][7, 0, 0] = 40.0          # a spike on the wrapped y edge
+    fields[2][52, 20, 0] = -60.0       # and on the clamped x edge
+    thr = 8.0
+    (o0, o1), mask, count = m.infer(fields, (False, True), sensor=thr)
+    p0, p1 = m.infer(fields, (False, True))
+    torch.cuda.synchronize()
+    assert torch.equal(o0, p0) and torch.equal(o1, p1)
+    host = [f[:, :, 0].cpu().numpy() for f in fields]
+    x = mc.features(host, m.fshift.cpu().numpy(), m.fscale.cpu().numpy(), (False, True))
+    # the kernel tests the normalised inputs before BF16 rounding
+    xr = np.zeros_like(x)
+    for k in range(mc.NF):
+        f, r = divmod(k, 9)
+        dx, dy = r % 3 - 1, r // 3 - 1
+        ii = np.clip(np.arange(nx) + dx, 0, nx - 1)
+        jj = (np.arange(ny) + dy) % ny
+        v = host[f][np.ix_(ii, jj)].astype(np.float32)
+        xr[:, k] = ((v - np.float32(m.fshift[f].item())) * np.float32(m.fscale[f].item())).T.reshape(-1)
+    want = (np.abs(xr).max(1) > thr).astype(np.uint8)
+    got = mask[:, :, 0].t().reshape(-1).cpu().numpy()
+    _say("flagged", int(want.sum()), "count", int(count.item()))
+    # 9 points around the wrapped spike, 6 around the clamped one
+    assert want.sum() >= 15 and (got == want).all() and int(count.item()) == int(want.sum())
+
+
+def test_closure_per_point_fallback_takes_the_plain_chain_where_flagged(pr):
+    """With some points beyond the training range, the closure's beta and
+    adiff are the plain chain's values (bit for bit) at the flagged points
+    and the network's elsewhere."""
+    from pyranda_amd.closure import MLPClosure, pyrandaLearnedLAD
+    from pyranda_amd.workloads import rm_run, rm_setup
+    ss = rm_setup(pr, 48)
+    rm_run(ss, 3)
+    lad = pyrandaLearnedLAD(ss, None)
+    args = [ss.var(k).data for k in ("div", "vx", "uy", "rho", "Yh")] + [ss.var("dt").data]
+    feats = lad.features(*args[:4])
+    model = MLPClosure("cuda")
+    model.set_normalisation([feats], [[ss.var("beta").data, ss.var("adiff").data]])
+    model.fmax *= 0.3                  # as if trained on a narrower range
+    lad.model = model
+    beta, adiff = lad.closure(*args)
+    (mb, ma), mask, count = model.infer(feats, lad.periodic, sensor=lad.sensor_factor * model.fmax)
+    pb, pa = lad.plain()
+    torch.cuda.synchronize()
+    sel = mask.bool()
+    n = int(count.item())
+    _say("flagged points", n, "of", sel.numel(), "partial", lad.partial)
+    assert 0 < n < sel.numel() and lad.partial == 1 and lad.fallbacks == 0
+    assert torch.equal(beta[sel], pb[sel].to(beta.dtype)) and torch.equal(adiff[sel], pa[sel].to(adiff.dtype))
+    assert torch.equal(beta[~sel], mb[~sel]) and torch.equal(adiff[~sel], ma[~sel])
+
+
+def test_closure_judge_switches_the_closure_off(pr):
+    """judge_every / judge_tol: a judged evaluation whose closure error is
+    above the tolerance switches the closure off; that evaluation and every
+    later one are the plain chain."""
+    from pyranda_amd.closure import MLPClosure, pyrandaLearnedLAD
+    from pyranda_amd.workloads import rm_run, rm_setup
+    ss = rm_setup(pr, 48)
+    rm_run(ss, 2)
+    lad = pyrandaLearnedLAD(ss, None, judge_every=2, judge_tol=1e-3)
+    args = [ss.var(k).data for k in ("div", "vx", "uy", "rho", "Yh")] + [ss.var("dt").data]
+    model = MLPClosure("cuda")
+    feats = lad.features(*args[:4])
+    model.set_normalisation([feats], [[ss.var("beta").data, ss.var("adiff").data]])
+    lad.model = model
+    lad.closure(*args)                       # call 1: the (untrained) network
+    assert lad.enabled and lad.fallbacks == 0
+    b2, a2 = lad.closure(*args)              # call 2: judged, far off -> off
+    assert not lad.enabled and lad.fallbacks == 1 and len(lad.judged) == 1
+    pb, pa = lad.plain()
+    b3, a3 = lad.closure(*args)
+    torch.cuda.synchronize()
+    assert lad.fallbacks == 2
+    for got in ((b2, a2), (b3, a3)):
+        assert torch.equal(got[0], pb) and torch.equal(got[1], pa)
+
+
 def test_mlp_learns_a_smooth_map(pr):
     """Training on the matrix cores reduces the error of a map the network
     can represent (targets from another network of the same shape)."""
@@ -174,6 +301,25 @@ def test_closure_fallback_is_the_plain_chain(pr):
     assert lad.calls > 0 and lad.fallbacks == lad.calls
     assert sa == sb
     for k in ("rho", "Et", "beta", "adiff", "Yh"):
+        assert torch.equal(a.var(k).data, b.var(k).data), k
+
+
+def test_closure_fallback_is_the_plain_chain_at_full_size(pr):
+    """The learned-closure workload at its full size, 1024 x 512 (x along the
+    shock, y periodic): an untrained closure falls back on every evaluation
+    and three RM steps with it are bit for bit the plain run (FP64)."""
+    from pyranda_amd.closure import MLPClosure
+    from pyranda_amd.workloads import rm_run, rm_setup
+    a = rm_setup(pr, 512, nx=1024)
+    b = rm_setup(pr, 512, nx=1024, closure=MLPClosure("cuda"))
+    assert (a.nx, a.ny) == (1024, 512)
+    sa = rm_run(a, 3)
+    sb = rm_run(b, 3)
+    lad = b.packages["LearnedLAD"]
+    _say("calls", lad.calls, "fallbacks", lad.fallbacks, "state", sa)
+    assert lad.calls == 15 and lad.fallbacks == lad.calls
+    assert sa == sb
+    for k in ("rho", "rhou", "Et", "beta", "adiff", "Yh"):
         assert torch.equal(a.var(k).data, b.var(k).data), k
 
 

@@ -148,6 +148,31 @@ def test_wall_group_matches_members_alone(pr, precision):
         assert float(v[:, 0, :].abs().max()) == 0.0 and float(v[:, -1, :].abs().max()) == 0.0
 
 
+def test_wall_group_full_ensemble_matches_members_alone(pr):
+    """The full wall-bounded KH ensemble: 64 members, each axis log-uniform in
+    [128, 1024] (8.6M points), FP32 fields with the refined solve, as one
+    pyrandaGroupSim: after each of two steps every member's conserved fields
+    and dt equal that member run alone, bit for bit."""
+    from pyranda_amd.workloads import ensemble_sizes, kh_advance, kh_wall_ensemble, kh_wall_group
+    sizes = ensemble_sizes(64, 128, 1024)
+    _say("full wall ensemble", len(sizes), "members,", sum(a * b for a, b in sizes), "points")
+    g, gst = kh_wall_group(pr, sizes, precision="fp32")
+    solo, solo_state = kh_wall_ensemble(pr, sizes, precision="fp32")
+    for step in range(2):
+        kh_advance(g, gst)
+        for s, st in zip(solo, solo_state):
+            kh_advance(s, st)
+        torch.cuda.synchronize()
+        for m, s in enumerate(solo):
+            for k in ("rho", "rhou", "rhov", "Et"):
+                assert _same(g.member_field(k, m), s.var(k).data), (step, m, sizes[m], k)
+            assert _same(gst["dt"][m], solo_state[m]["dt"]), (step, m, "dt")
+        _say("step", step, "ok")
+    for m in range(len(sizes)):
+        v = g.member_field("v", m)
+        assert float(v[:, 0, :].abs().max()) == 0.0 and float(v[:, -1, :].abs().max()) == 0.0
+
+
 def test_group_mode_switch_builds_new_plans(pr):
     """A grouped FP32 operator applied under the refined solve, then with the
     members switched to FP64 solves: each result equals every member's own

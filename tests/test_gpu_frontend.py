@@ -395,3 +395,47 @@ def test_fp32_offset_dominated_fields_stored_fp64(pr):
     err = {n: float((runs[n].var("rhou").data.double() - ref).norm() / ref.norm()) for n in ("wide", "f32")}
     _say("rhou rel l2 vs fp64", err)
     assert err["wide"] < err["f32"]
+
+
+def test_fp32_derivatives_of_fp64_fields_stored_by_the_sweep(pr, monkeypatch):
+    """In an FP32 run, `:ux: = ddx(:u:)` with u stored in FP64 and ux in FP32
+    runs one FP64 sweep that stores only the FP32 rounding
+    (pyr_apply_to_r32): the run is bit for bit the one that sweeps into an
+    FP64 temporary and rounds it afterwards."""
+    import torch
+    imesh = """
+    xdom = (0.0, 2*pi*FF,  32, periodic=True)
+    ydom = (0.0, 2*pi*FF,  32, periodic=True)
+    zdom = (0.0, 2*pi*FF,  32, periodic=True)
+    """.replace("pi", str(np.pi)).replace("FF", str(31.0 / 32))
+
+    def run():
+        ss = pr.pyrandaSim("TGvortex", imesh, silent=True, precision="fp32")
+        ss.addPackage(pr.pyrandaTimestep(ss))
+        ss.EOM(TG_EOM)
+        ss.setIC(TG_IC)
+        dt = float(ss.variables["dt"].data) * 0.5
+        t = 0.0
+        for _ in range(3):
+            t = ss.rk4(t, dt)
+        torch.cuda.synchronize()
+        return ss
+
+    calls = []
+    orig = pr.pyrandaSim._derivative_to32
+
+    def counted(self, eq):
+        r = orig(self, eq)
+        if r is not None:
+            calls.append(eq.LHS[0])
+        return r
+
+    monkeypatch.setattr(pr.pyrandaSim, "_derivative_to32", counted)
+    fast = run()
+    assert fast._storage.get("u") == "f64" and fast._storage.get("ux") == "f32"
+    assert "ux" in calls and "wy" in calls
+    monkeypatch.setattr(pr.pyrandaSim, "_derivative_to32", lambda self, eq: None)
+    slow = run()
+    for k in ("rho", "rhou", "rhov", "rhow", "Et", "ux", "vz", "mu", "beta"):
+        a, b = fast.var(k).data, slow.var(k).data
+        assert a.dtype == b.dtype and torch.equal(a, b), k

@@ -264,10 +264,33 @@ def test_filter_with_rounded_copy(dev, nn, kinds):
     out_t, out32_t = ctx.sfilter_rounded(v)
     ctx.timing = None
     assert torch.equal(out_t, ref) and torch.equal(out32_t, ref.float())
-    # only FP64 y/z sweeps with the store epilogue write the copy
-    with pytest.raises(_lib.PyrandaAMDError):
-        _lib.check(ctx._L.pyr_apply_rounded(ctx._h, _lib.OP_SFILTER, 0, ctx._ptr(v), ctx._ptr(out),
-                                            ctx._ptr(out32)))
+    # an x sweep writes the copy too
+    want = ctx.apply(_lib.OP_SFILTER, 0, v)
+    o64 = torch.empty_like(want)
+    _lib.check(ctx._L.pyr_apply_rounded(ctx._h, _lib.OP_SFILTER, 0, ctx._ptr(v), ctx._ptr(o64),
+                                        ctx._ptr(out32)))
+    torch.cuda.synchronize()
+    assert torch.equal(o64, want) and torch.equal(out32, want.float())
+    ctx.close()
+
+
+@pytest.mark.parametrize("nn,kinds", CASES)
+def test_fp64_sweep_storing_fp32(dev, nn, kinds):
+    """pyr_apply_to_r32: an FP64 sweep that stores only the FP32 rounding of
+    its result is bit for bit the rounding of the FP64 sweep's result, for
+    every operator and direction (periodic, bounded, SYMM, null directions,
+    long lines)."""
+    from pyranda_amd import _lib
+    import oracle.pyoracle as po
+    _say("fp64 -> fp32 sweeps", nn, kinds)
+    ctx, _, v = _setup(nn, kinds, po)
+    vd = to_dev(v, dev)
+    for op in (_lib.OP_D1, _lib.OP_D2, _lib.OP_D8, _lib.OP_SFILTER, _lib.OP_GFILTER):
+        for d in range(3):
+            want = ctx.apply(op, d, vd)
+            got = ctx.apply_to32(op, d, vd)
+            torch.cuda.synchronize()
+            assert got.dtype == torch.float32 and torch.equal(got, want.float()), (op, d)
     ctx.close()
 
 

@@ -511,3 +511,74 @@ def test_tg_cpu_golden(oracle_mod):
     got = tg.V["enst"].sum() / enst0
     want = _golden()["taylorgreen"]["value"]
     assert abs(got - want) / want < 1e-10, (got, want)
+
+
+# --- learned-closure MLP restatement (oracle/mlp_cpu.py) --------------------
+def _mlp_case(seed=4, n=200):
+    from oracle import mlp_cpu as mc
+    rng = np.random.default_rng(seed)
+    params = [rng.standard_normal((mc.NIN, mc.NH)) * 0.3, rng.standard_normal((mc.NH, mc.NH)) * 0.2,
+              rng.standard_normal((mc.NH, mc.NO)) * 0.2, rng.standard_normal(mc.NH) * 0.1,
+              rng.standard_normal(mc.NH) * 0.1, rng.standard_normal(mc.NO) * 0.1]
+    params[0][mc.NF:] = 0.0
+    x = np.zeros((n, mc.NIN))
+    x[:, :mc.NF] = rng.standard_normal((n, mc.NF))
+    t = [np.abs(rng.standard_normal(n)) * 3.0, np.abs(rng.standard_normal(n)) * 0.25]
+    return mc, params, x, t, (3.0, 0.25)
+
+
+def test_mlp_oracle_backward_is_the_loss_gradient():
+    """The backward pass of the MLP restatement (without BF16 rounding) is the
+    gradient of loss / npoints: central differences of the float64 loss in
+    every parameter block agree to 1e-6 (a dropped bias term, a missing
+    ReLU mask or a transposed weight fails this)."""
+    mc, params, x, t, osc = _mlp_case()
+    grads, loss = mc.gradient(x, t, params, osc, exact=True)
+    n = x.shape[0]
+    assert abs(loss - mc.loss_of(x, t, params, osc, exact=True)) < 1e-9 * loss
+    rng = np.random.default_rng(9)
+    for b, g in enumerate(grads):
+        for _ in range(6):
+            idx = tuple(rng.integers(0, s) for s in params[b].shape)
+            if b == 0 and idx[0] >= mc.NF:
+                continue
+            if b in (2, 5) and idx[-1] >= 2:
+                assert g[idx] == 0.0
+                continue
+            h = 1e-6
+            p1 = [p.copy() for p in params]
+            p2 = [p.copy() for p in params]
+            p1[b][idx] += h
+            p2[b][idx] -= h
+            fd = (mc.loss_of(x, t, p1, osc, exact=True) - mc.loss_of(x, t, p2, osc, exact=True)) / (2 * h) / n
+            assert abs(fd - g[idx]) <= 1e-6 * max(1.0, abs(fd)), (b, idx, fd, g[idx])
+
+
+def test_mlp_oracle_bf16_rounding_is_torch_bfloat16():
+    """bf16() rounds as torch's float32 -> bfloat16 conversion (nearest,
+    ties to even), including ties, signs, tiny and huge values and NaN."""
+    import torch
+    from oracle import mlp_cpu as mc
+    rng = np.random.default_rng(3)
+    v = np.concatenate([rng.standard_normal(5000).astype(np.float32) * 10.0 ** rng.integers(-30, 30, 5000),
+                        np.array([1.0 + 2 ** -8, 1.0 + 3 * 2 ** -8, -(1.0 + 2 ** -8), 0.0, -0.0, 3.0e38,
+                                  1e-40, np.nan], dtype=np.float32)]).astype(np.float32)
+    got = mc.bf16(v)
+    want = torch.from_numpy(v).to(torch.bfloat16).to(torch.float32).numpy()
+    same = (got == want) | (np.isnan(got) & np.isnan(want))
+    assert same.all(), v[~same][:5]
+
+
+def test_mlp_oracle_features_layout():
+    """Feature k = f*9 + 3*(dy+1) + (dx+1) of point p = j*nx + i is field f at
+    (i+dx, j+dy), shifted and scaled, wrapped on periodic axes and clamped on
+    bounded ones."""
+    from oracle import mlp_cpu as mc
+    nx, ny = 5, 4
+    fields = [np.arange(nx * ny, dtype=np.float64).reshape(ny, nx).T + 100 * f for f in range(3)]
+    x = mc.features(fields, (0.0, 0.0, 0.0), (1.0, 1.0, 1.0), (False, True))
+    i, j = 0, 0                       # corner: clamped in x, wrapped in y
+    p = j * nx + i
+    assert x[p, 1 * 9 + 0] == fields[1][0, ny - 1]        # dx=-1 -> clamp 0, dy=-1 -> wrap
+    assert x[p, 2 * 9 + 8] == fields[2][1, 1]             # dx=+1, dy=+1
+    assert x[p, 4] == fields[0][0, 0] and (x[:, mc.NF:] == 0).all()
