@@ -360,7 +360,7 @@ def closure_bench(args, dev):
     from pyranda_amd.workloads import rm_run, rm_setup
     n, prec = args.closure_n, args.closure_precision
     nxr = args.closure_nx or None
-    plain = rm_setup(pr, n, precision=prec, device=dev, nx=nxr)
+    plain = rm_setup(pr, n, precision=prec, device=dev, nx=nxr, single_mode=True)
     nx, ny = plain.nx, plain.ny
     print("[closure] RM 2-D %dx%d %s" % (nx, ny, prec), file=sys.stderr, flush=True)
     helper = pyrandaLearnedLAD(plain, None)
@@ -368,8 +368,7 @@ def closure_bench(args, dev):
     samples, st = [], None
     for _ in range(args.closure_snapshots):
         st = rm_run(plain, args.closure_every, state=st)
-        feats = [f.clone() for f in helper.features(*[plain.var(k).data for k in lad_in])]
-        samples.append((feats, [plain.var("beta").data.clone(), plain.var("adiff").data.clone()]))
+        samples.append(helper.sample())
     model = MLPClosure(dev)
     model.set_normalisation([s_[0] for s_ in samples], [s_[1] for s_ in samples])
     torch.cuda.synchronize()
@@ -403,7 +402,8 @@ def closure_bench(args, dev):
     # steps of the RM run with either evaluation, from the same initial state
     runs = {}
     for mode in ("plain", "closure"):
-        ss = rm_setup(pr, n, precision=prec, device=dev, nx=nxr, closure=model if mode == "closure" else None)
+        ss = rm_setup(pr, n, precision=prec, device=dev, nx=nxr, single_mode=True,
+                      closure=model if mode == "closure" else None)
         stt = rm_run(ss, 1)
         torch.cuda.synchronize()
         t0 = time.perf_counter()

@@ -25,8 +25,10 @@ with the plain chain as fallback, at two levels:
     `judge_tol` in beta or adiff switches the closure off for the rest of
     the run (every later evaluation is the plain chain).
 An untrained closure (no training range) falls back on every evaluation.
-`judge()` reports the closure error against the plain chain on the current
-state.
+The network's second output is adiff * dt (the chain's value before its
+division by the time step, which is not an input); the closure divides by
+the current dt. `judge()` reports the closure error against the plain chain
+on the current state.
 
 No counterpart in the reference.
 """
@@ -223,6 +225,16 @@ class pyrandaLearnedLAD(pyrandaPackage):
         gx, gy = s.ddx(rho), s.ddy(rho)
         return [div, vx - uy, torch.sqrt(gx * gx + gy * gy)]
 
+    def sample(self):
+        """One training sample from the simulation's current fields:
+        (features, [beta, adiff * dt]). The network learns adiff * dt, the
+        chain's value before its division by the time step (dt is not an
+        input and changes along a run); closure() divides by the current dt."""
+        s = self.pysim
+        v = {k: s.var(k).data for k in ("div", "vx", "uy", "rho", "beta", "adiff", "dt")}
+        feats = [f.clone() for f in self.features(v["div"], v["vx"], v["uy"], v["rho"])]
+        return feats, [v["beta"].clone(), v["adiff"] * v["dt"]]
+
     def plain(self):
         """The plain LAD chain on the simulation's current fields: (beta, adiff)."""
         from .pyrandaEq import pyrandaEq
@@ -255,7 +267,8 @@ class pyrandaLearnedLAD(pyrandaPackage):
                 self.enabled = False
                 self.fallbacks += 1
                 return list(self.plain())
-        (beta, adiff), mask, count = m.infer(feats, self.periodic, sensor=self.sensor_factor * m.fmax)
+        (beta, adiff_dt), mask, count = m.infer(feats, self.periodic, sensor=self.sensor_factor * m.fmax)
+        adiff = adiff_dt / dt
         n = int(count.item())
         if n:
             self.partial += 1
@@ -272,6 +285,7 @@ class pyrandaLearnedLAD(pyrandaPackage):
         if feats is None:
             feats = self.features(div, vx, uy, rho)
         got = self.model.infer(feats, self.periodic)
+        got = [got[0], got[1] / dt]
         ref = self.plain()
         out = {}
         for name, g, r in zip(("beta", "adiff"), got, ref):

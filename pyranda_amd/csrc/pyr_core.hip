@@ -925,15 +925,16 @@ __global__ void k_rk_update(long long n, const TF* __restrict__ F, double* __res
        i += (long long)gridDim.x * blockDim.x) {
     const double dti = mid ? dtp[mid[i]] : dt;
     const double tmp1 = A == 0.0 ? 0.0 : A * PHI[i];
-    const double phi = dti * (double)F[i] + tmp1;
+    // explicit FMAs: the generated kernels that fuse a right-hand side into
+    // this update round the same way
+    const double phi = fma(dti, (double)F[i], tmp1);
     PHI[i] = phi;
-    const double tmp2 = B * phi;
     if (U64) {
-      const double u = U64[i] + tmp2;
+      const double u = fma(B, phi, U64[i]);
       U64[i] = u;
       if (Uout) Uout[i] = (TS)u;
     } else {
-      Uout[i] = (TS)((double)Uin[i] + tmp2);
+      Uout[i] = (TS)fma(B, phi, (double)Uin[i]);
     }
   }
 }

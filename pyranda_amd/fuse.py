@@ -146,10 +146,13 @@ class _Gen:
 WIDE_RATIO = 8.0
 
 # value_storage: True while an FP32 run with storage by value evaluates its
-# equations: a pointwise kernel's unnamed result (e.g. the flux argument of a
-# derivative) is then stored in FP64 when its first evaluation (in FP64) is
-# offset-dominated, else in FP32 (the decision is kept per kernel and input
-# signature); with False a result is FP64 when an input field is
+# equations. With wide_if_wide_input (the "auto" policy) a pointwise kernel's
+# unnamed result (e.g. the flux argument of a derivative) is FP64 when an
+# input field is FP64 and FP32 otherwise, a decision by types alone that a
+# run resumed from a restart repeats; without it (the "offset" policy) the
+# result is stored in FP64 when its first evaluation (in FP64) is
+# offset-dominated, else in FP32 (kept per kernel and input signature).
+# With value_storage False a result is FP64 when an input field is.
 STORAGE = {"value_storage": False, "wide_if_wide_input": False}
 
 
@@ -231,6 +234,83 @@ class FusedSubtree:
         # when every field input is FP64 (last_all64 tells the owner)
         self.named_all64 = False
         self.last_all64 = False
+        # set by FusedRHS(defer=True) around the call of a root subtree
+        self._defer = False
+
+    def _rk_source(self, sig, out_dt, variant):
+        """The subtree's value F (rounded to its storage type) fused into the
+        RK4 stage update of pyranda.py:800-804: PHI = dt F + A PHI, then the
+        state U advanced by B PHI; the arithmetic of k_rk_update
+        (pyr_core.hip), explicit FMAs included. variant = (ts, shadow,
+        store, dtp): the field type, an FP64 state advanced in place, the
+        field stored, dt read from device memory."""
+        ts, shadow, store, dtp = variant
+        name = "pyr_rk_" + hashlib.sha1((self.body + repr(sig) + out_dt + repr(variant)).encode()).hexdigest()[:16]
+        lines = [PRELUDE, 'extern "C" __global__ void %s(PyrPwArgs a) {' % name]
+        ip = isc = 0
+        for i, s in enumerate(sig):
+            if s == "S":
+                lines.append("  const double x%d = a.s[%d];" % (i, isc))
+                isc += 1
+            elif s in ("D64", "D32"):
+                ct = "double" if s == "D64" else "float"
+                lines.append("  const double x%d = (double)*(const %s*)a.p[%d];" % (i, ct, ip))
+                ip += 1
+            else:
+                ct = "double" if s == "T64" else "float"
+                lines.append("  const %s* __restrict__ p%d = (const %s*)a.p[%d];" % (ct, i, ct, ip))
+                ip += 1
+        tt = "double" if ts == "f64" else "float"
+        ot = "double" if out_dt == "f64" else "float"
+        lines.append("  double* __restrict__ phi_ = (double*)a.p[%d];" % ip)
+        ip += 1
+        if not shadow:
+            lines.append("  const %s* __restrict__ uin = (const %s*)a.p[%d];" % (tt, tt, ip))
+            ip += 1
+        if store:
+            lines.append("  %s* __restrict__ uout = (%s*)a.p[%d];" % (tt, tt, ip))
+            ip += 1
+        if shadow:
+            lines.append("  double* __restrict__ u64 = (double*)a.p[%d];" % ip)
+            ip += 1
+        if dtp:
+            lines.append("  const double dt = *(const double*)a.p[%d];" % ip)
+        else:
+            lines.append("  const double dt = a.s[%d];" % isc)
+            isc += 1
+        lines.append("  const double A = a.s[%d], B = a.s[%d];" % (isc, isc + 1))
+        lines.append("  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;"
+                     " i += (long long)gridDim.x * blockDim.x) {")
+        for i, s in enumerate(sig):
+            if s in ("T64", "T32"):
+                lines.append("    const double x%d = (double)p%d[i];" % (i, i))
+        lines.append("    const double f = (double)(%s)(%s);" % (ot, self.body))
+        lines.append("    const double tmp1 = A == 0.0 ? 0.0 : A * phi_[i];")
+        lines.append("    const double phi = fma(dt, f, tmp1);")
+        lines.append("    phi_[i] = phi;")
+        if shadow:
+            lines.append("    const double u = fma(B, phi, u64[i]);")
+            lines.append("    u64[i] = u;")
+            if store:
+                lines.append("    uout[i] = (%s)u;" % tt)
+        else:
+            lines.append("    uout[i] = (%s)fma(B, phi, (double)uin[i]);" % tt)
+        lines.append("  }")
+        lines.append("}")
+        return name, "\n".join(lines)
+
+    def rk_kernel(self, sig, out_dt, variant):
+        key = ("rk", sig, out_dt, variant)
+        fn = self._kernels.get(key)
+        if fn is None:
+            name, src = self._rk_source(sig, out_dt, variant)
+            h = ctypes.c_void_p()
+            L = _lib.lib()
+            if L.pyr_pw_compile(src.encode(), name.encode(), ctypes.byref(h)) != 0:
+                raise _lib.PyrandaAMDError(L.pyr_pw_last_error().decode())
+            fn = h
+            self._kernels[key] = fn
+        return fn
 
     def _source(self, sig, out_dt):
         name = "pyr_pw_" + hashlib.sha1((self.body + repr(sig) + out_dt).encode()).hexdigest()[:16]
@@ -326,14 +406,19 @@ class FusedSubtree:
             out_dt = self.out_dtype
         elif STORAGE["value_storage"] and self.named_all64 and STORAGE["wide_if_wide_input"] and all64:
             out_dt = "f64"
-        elif STORAGE["value_storage"] and not self.named_all64 and STORAGE["wide_if_wide_input"] and any64:
-            out_dt = "f64"
+        elif STORAGE["value_storage"] and not self.named_all64 and STORAGE["wide_if_wide_input"]:
+            # an unnamed result: by its input types alone (no decision by
+            # value, which a restarted run could not repeat)
+            out_dt = "f64" if any64 else "f32"
         elif STORAGE["value_storage"]:
             out_dt = self._decided.get(sig)
             if out_dt is None:
                 out_dt, decide = "f64", True
         else:
             out_dt = "f64" if any64 else "f32"
+        if self._defer and not decide and not any(k[0] == "G" for k in sig):
+            # the caller (an RK stage) evaluates the result inside its update
+            return DeferredF(self, sig, out_dt, fields, scalars, tuple(shape), dev)
         fn = self.kernel(sig, out_dt)
         nx, ny, nz = shape
         out = torch.empty((nz, ny, nx), dtype=torch.float64 if out_dt == "f64" else torch.float32,
@@ -466,14 +551,15 @@ class FusedGroup:
                 return self._separate(shape, vals)
             out_dts.append("f64" if any(is64[j] for j in m) else "f32")
         decide = False
-        if STORAGE["value_storage"]:
+        if STORAGE["value_storage"] and STORAGE["wide_if_wide_input"]:
+            # unnamed results by their input types alone (out_dts as set above)
+            pass
+        elif STORAGE["value_storage"]:
             known = self._decided.get(tuple(sig))
             if known is None:
                 out_dts, decide = ["f64"] * len(self.maps), True
             else:
                 out_dts = list(known)
-            if STORAGE["wide_if_wide_input"]:
-                out_dts = ["f64" if any(is64[j] for j in m) else dt for m, dt in zip(self.maps, out_dts)]
         if any(k[0] == "G" for k in sig):
             fields.append(shape.mid)
         nout = len(self.subs)
@@ -597,6 +683,50 @@ class _DropAbsOfRing(ast.NodeTransformer):
         return node
 
 
+class DeferredF:
+    """The value of a PDE right-hand side whose last pointwise kernel has not
+    run: its operands are evaluated (operator results included) and the
+    RK4 stage evaluates the kernel's expression inside its update
+    (`rk_update`), saving the write and read of F. `materialize()` runs the
+    kernel as FusedSubtree would."""
+
+    def __init__(self, sub, sig, out_dt, fields, scalars, shape, dev):
+        self.sub, self.sig, self.out_dt = sub, sig, out_dt
+        self.fields, self.scalars, self.shape, self.device = fields, scalars, shape, dev
+        self.dtype = torch.float64 if out_dt == "f64" else torch.float32
+
+    def materialize(self):
+        fn = self.sub.kernel(self.sig, self.out_dt)
+        nx, ny, nz = self.shape
+        out = torch.empty((nz, ny, nx), dtype=self.dtype, device=self.device).permute(2, 1, 0)
+        ptrs = (ctypes.c_void_p * (len(self.fields) + 1))(*[f.data_ptr() for f in self.fields], out.data_ptr())
+        sc = (ctypes.c_double * max(1, len(self.scalars)))(*self.scalars)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        _lib.check_pw(_lib.lib().pyr_pw_launch(fn, stream, nx * ny * nz, len(self.fields) + 1, ptrs,
+                                               len(self.scalars), sc))
+        return out
+
+    def rk_update(self, phi, uin, uout, u64, dt, A, B, ts):
+        """PHI, the FP64 state u64 (if given) and the field uout (if given)
+        advanced by one stage with F evaluated in registers; dt a float or a
+        0-d FP64 device tensor. Returns False when the operands do not fit one
+        launch (the caller materializes F instead)."""
+        shadow, store = u64 is not None, uout is not None
+        dtp = torch.is_tensor(dt)
+        ptrs = list(self.fields) + [phi] + ([] if shadow else [uin]) + ([uout] if store else []) \
+            + ([u64] if shadow else []) + ([dt] if dtp else [])
+        scal = list(self.scalars) + ([] if dtp else [float(dt)]) + [float(A), float(B)]
+        if len(ptrs) > _lib.PW_MAX_PTRS or len(scal) > _lib.PW_MAX_SCALARS:
+            return False
+        fn = self.sub.rk_kernel(self.sig, self.out_dt, (ts, shadow, store, dtp))
+        nx, ny, nz = self.shape
+        pp = (ctypes.c_void_p * len(ptrs))(*[t.data_ptr() for t in ptrs])
+        sc = (ctypes.c_double * len(scal))(*scal)
+        stream = ctypes.c_void_p(torch.cuda.current_stream(self.device).cuda_stream)
+        _lib.check_pw(_lib.lib().pyr_pw_launch(fn, stream, nx * ny * nz, len(ptrs), pp, len(scal), sc))
+        return True
+
+
 class FusedRHS:
     """Callable replacing `lambda self: <rhs>` with fused pointwise kernels."""
 
@@ -621,10 +751,18 @@ class FusedRHS:
         ast.fix_missing_locations(lam)
         self._fn = eval(compile(lam, "<eom %s>" % src[:40], "eval"), globals_)
 
-    def __call__(self, sim):
+    def __call__(self, sim, defer=False):
+        """defer: a right-hand side whose last step is one pointwise kernel
+        returns a DeferredF (its operands evaluated, the kernel not run)."""
         shape = sim.mesh.shape
-        return self._fn(sim, shape if isinstance(shape, GroupShape) else tuple(shape), self.subtrees,
-                        self.group)
+        shape = shape if isinstance(shape, GroupShape) else tuple(shape)
+        if not defer or self.root is None:
+            return self._fn(sim, shape, self.subtrees, self.group)
+        self.root._defer = True
+        try:
+            return self._fn(sim, shape, self.subtrees, self.group)
+        finally:
+            self.root._defer = False
 
 
 _RED = {

@@ -361,6 +361,8 @@ class pyrandaGroupSim(pyrandaSim):
             dev = torch.device("cuda", torch.cuda.current_device())
         self.name = name
         self.silent = silent
+        # the update of a group reads one dt per member: F is stored first
+        self.fuse_rk = False
         self.precision = precision
         self.dtype = torch.float64 if precision == "fp64" else torch.float32
         # every field of a group is stored in the run's precision (no FP64

@@ -39,6 +39,7 @@ can be captured in a HIP graph (pyranda_amd.ensemble). Values are the same
 as with host scalars, bit for bit.
 """
 import math
+import os
 import random
 import re
 import sys
@@ -74,6 +75,9 @@ class pyrandaSim:
         # fp32 modes: conserved variables rewritten only by the conservative
         # filter keep an FP64 state through it (False: the FP32 field is the state)
         self.fp64_state = fp64_state
+        # each PDE right-hand side's last pointwise kernel runs inside the RK
+        # update (PYRANDA_AMD_FUSE_RK=0: F is stored, then read by the update)
+        self.fuse_rk = os.environ.get("PYRANDA_AMD_FUSE_RK", "1") != "0"
         self.silent = silent
         self.mesh = pyrandaMesh()
         if isinstance(meshOptions, str):
@@ -434,12 +438,16 @@ class pyrandaSim:
 
         return _Mode()
 
-    def updateFlux(self):
+    def updateFlux(self, defer=False):
+        """Right-hand sides of the PDEs. defer: a right-hand side whose last
+        step is one fused pointwise kernel comes back as a fuse.DeferredF,
+        which the RK update evaluates in its own kernel."""
         flux = {}
         with self._storage_mode():
             for eqo in self.equations:
                 if eqo.kind == "PDE":
-                    flux[eqo.LHS[0]] = eqo.RHS(self)
+                    rhs = eqo.RHS
+                    flux[eqo.LHS[0]] = rhs(self, defer=True) if defer and hasattr(rhs, "root") else rhs(self)
         return flux
 
     def _alg_plan(self):
@@ -574,7 +582,7 @@ class pyrandaSim:
         time_i = time
         self.deltat = dt
         for ii in range(5):
-            FLUX = self.updateFlux()
+            FLUX = self.updateFlux(defer=getattr(self, "fuse_rk", False))
             for U in self.conserved:
                 self.variables[U].data = self._rk_update(U, FLUX[U], PHI, dt, RK_A[ii], RK_B[ii], fp32,
                                                          store=U not in skip)
@@ -665,12 +673,18 @@ class pyrandaSim:
         with an FP64 state): only the state advances and the field returned
         is the current one, which the caller replaces before reading it."""
         from . import _lib
+        from .fuse import DeferredF
         from .parcop import _contig_equiv
         u = self.variables[U].data
         nn = tuple(self.mesh.shape)
         floats = (torch.float32, torch.float64)
         # an FP64 field of an fp32 run whose FP64 state is the field itself
         state_is_field = fp32 and torch.is_tensor(u) and u.dtype == torch.float64 and self._shadow.get(U) is u
+        if isinstance(F, DeferredF):
+            got = self._rk_update_deferred(U, F, PHI, dt, A, B, fp32, store, u, state_is_field)
+            if got is not None:
+                return got
+            F = F.materialize()
         ok = (torch.is_tensor(F) and tuple(F.shape) == nn and _contig_equiv(F)
               and F.dtype in floats and torch.is_tensor(u) and tuple(u.shape) == nn
               and _contig_equiv(u) and u.dtype in floats)
@@ -713,6 +727,37 @@ class pyrandaSim:
                                             uin, optr, shadow, float(dt), float(A),
                                             float(B)))
         return out
+
+    def _rk_update_deferred(self, U, F, PHI, dt, A, B, fp32, store, u, state_is_field):
+        """The stage update with the right-hand side's last pointwise kernel
+        evaluated inside it (one kernel, F never stored); the same branches
+        and arithmetic as the pyr_rk_update path. None where it does not
+        apply (the caller materializes F)."""
+        from .parcop import _contig_equiv
+        nn = tuple(self.mesh.shape)
+        if not (torch.is_tensor(u) and tuple(u.shape) == nn and _contig_equiv(u)
+                and u.dtype in (torch.float32, torch.float64)):
+            return None
+        if torch.is_tensor(dt) and dt.numel() != 1:
+            return None
+        if torch.is_tensor(dt):
+            dt = dt.to(device=u.device, dtype=torch.float64).reshape(())
+        ts = "f64" if u.dtype == torch.float64 else "f32"
+        if state_is_field:
+            uin, out, u64, ret = None, None, u, u
+        else:
+            shadow = (self._shadow[U] if fp32 and u.dtype != torch.float64 and self._shadow[U] is not None
+                      else None)
+            uin, u64 = (None if shadow is not None else u), shadow
+            if shadow is not None and not store:
+                out, ret = None, u
+            else:
+                out = torch.empty((self.nz, self.ny, self.nx), dtype=u.dtype, device=u.device).permute(2, 1, 0)
+                ret = out
+        self.PyMPI.ctx._stream()
+        if not F.rk_update(PHI[U], uin, out, u64, dt, A, B, ts):
+            return None
+        return ret
 
     # -- restart and viz files (pyranda.py:431-588) -------------------------------
     def _value_of(self, name):

@@ -34,8 +34,16 @@ class _Der:
 
     def div(self, fx, fy, fz): return self.ctx.divergence(fx, fy, fz)
 
-    def divT(self, *a):
-        raise NotImplementedError("divT (tensor divergence) is outside this build")
+    def divT(self, fxx, fxy, fxz, fyx, fyy, fyz, fzx, fzy, fzz):
+        """Tensor divergence (operators.f90:104-122): each diagonal term takes
+        the symmetric operator (isym**2), each off-diagonal one the
+        antisymmetric operator where its direction has a SYMM end."""
+        from . import _lib
+        c, D1, A = self.ctx, _lib.OP_D1, _lib.OP_D1 | _lib.OP_ANTISYM
+        dfx = c.apply(D1, 0, fxx) + c.apply(A, 1, fyx) + c.apply(A, 2, fzx)
+        dfy = c.apply(A, 0, fxy) + c.apply(D1, 1, fyy) + c.apply(A, 2, fzy)
+        dfz = c.apply(A, 0, fxz) + c.apply(A, 1, fyz) + c.apply(D1, 2, fzz)
+        return dfx, dfy, dfz
 
     def grad(self, v): return self.ctx.grads(v)
     def laplacian(self, v): return self.ctx.plaplacian(v)
@@ -75,8 +83,8 @@ class pyrandaMPI:
         self.periodic = opts["periodic"]
         symmetric = opts["symmetric"]
         self.coordsys = opts["coordsys"]
-        if self.coordsys != 0:
-            raise NotImplementedError("only Cartesian meshes (coordsys=0) are built")
+        if self.coordsys not in (0, 1):
+            raise NotImplementedError("Cartesian (coordsys=0) and cylindrical (coordsys=1) meshes are built")
         self.comm = comm
         self.master = True
         self.dtype = dtype
@@ -109,6 +117,14 @@ class pyrandaMPI:
         self.der = _Der(self.ctx)
         self.gfil = _GFil(self.ctx)
         self.fil = _SFil(self.ctx)
+        self.metrics = None
+        if self.coordsys == 1:
+            # radius metrics in div/divT/grad/lap/ring and the cell-volume
+            # weighted fbar; gbar (filter 'smooth') stays Cartesian
+            from .cylindrical import CylindricalDer, CylindricalMetrics, CylindricalSFil
+            self.metrics = CylindricalMetrics(self.ctx, (self.nx, self.ny, self.nz), self.device)
+            self.der = CylindricalDer(self.ctx, self.metrics)
+            self.fil = CylindricalSFil(self.ctx, self.metrics)
 
     # -- allocation (pyrandaMPI.py:249-254) ---------------------------------
     def emptyScalar(self):

@@ -1,4 +1,5 @@
-"""Mesh description and Cartesian grid (pyranda/pyrandaMesh.py:16-233).
+"""Mesh description and grid (pyranda/pyrandaMesh.py:16-233): Cartesian, or
+cylindrical (coordsys 1, radius/theta/z, see cylindrical.py).
 
 `xdom=(x1, xn, n, periodic=True)` lines are executed through the same
 substitution map as the reference (pyrandaMesh.py:33-56). Coordinates follow
@@ -86,3 +87,23 @@ class pyrandaMesh:
         self.d1, self.d2, self.d3 = ctx.dmesh
         self.GridLen = ctx.gridlen
         self.CellVol = ctx.dpatch[0] * ctx.dpatch[1] * ctx.dpatch[2]
+        if mpi.coordsys == 1:
+            # cylindrical (pyrandaMesh.py:152-155, mesh.f90:213-224): the
+            # native (radius, theta, z) grid is kept, the mesh coordinates
+            # become x = r cos(theta), y = r sin(theta), z
+            m = mpi.metrics
+            self.coordsNative = [pyrandaVar("x", "mesh", "scalar"), pyrandaVar("y", "mesh", "scalar"),
+                                 pyrandaVar("z", "mesh", "scalar")]
+            for cor, nat in zip(self.coordsNative, self.coords):
+                cor.PyMPI = mpi
+                cor.allocated = True
+                cor.data = nat.data
+            r, th = X, Y
+            cx, cy = torch.empty_like(X), torch.empty_like(Y)     # same (x fastest) layout
+            torch.mul(r, torch.cos(th), out=cx)
+            torch.mul(r, torch.sin(th), out=cy)
+            self.coords[0].data = cx
+            self.coords[1].data = cy
+            self.d1, self.d2, self.d3 = m.d1, m.d2, m.d3
+            self.GridLen = m.GridLen
+            self.CellVol = m.CellVol

@@ -398,13 +398,18 @@ RM_CLOSURE_LINE = "[:beta:,:adiff:] = lad.closure(:div:,:vx:,:uy:,:rho:,:Yh:,:dt
 
 
 def rm_setup(pr, npts=128, precision="fp64", seed=1234, closure=None, name="RM_2D", device=None, nx=None,
-             sensor_factor=None):
+             sensor_factor=None, single_mode=False):
     """examples/RM3D.py, 2-D branch, on an nx x npts grid (nx defaults to
     int(1.4 npts), the script's choice; x spans 2.8 pi along the shock, y
     2 pi periodic across it); the IC's random perturbation draws from numpy's
-    global generator seeded with `seed`. With `closure` (an MLPClosure) the
-    beta and adiff lines become the learned closure (plain chain as
-    fallback)."""
+    global generator seeded with `seed`. single_mode: the interface is
+    perturbed by one Fourier mode (wavenumber 4 across y, amplitude 0.05,
+    phase drawn from `seed`) instead of the script's mode plus filtered
+    per-cell noise, whose roughness grows with resolution (at 1024 x 512
+    the noise moves the interface by about two cells from one cell to the
+    next, and the first step, whose dt is the initial Courant limit, is
+    unstable). With `closure` (an MLPClosure) the beta and adiff lines become
+    the learned closure (plain chain as fallback)."""
     import numpy
     from .closure import pyrandaLearnedLAD
     nx = int(npts * 1.4) if nx is None else int(nx)
@@ -429,7 +434,12 @@ def rm_setup(pr, npts=128, precision="fp64", seed=1234, closure=None, name="RM_2
             eom = eom.replace(line + "\n", "")
     ss.EOM(eom)
     numpy.random.seed(seed)
-    ss.setIC(RM_IC, {"waveLength": 4, "delta": 2.0 * numpy.pi / npts * 4})
+    ic = RM_IC
+    if single_mode:
+        phase = float(numpy.random.default_rng(seed).uniform(0.0, 2.0 * numpy.pi))
+        ic = ic.replace("Amp = 3.5 + .05*(sin(meshy*L) + cos(meshz*L) ) + .25*gbar( random3D() - .5 )",
+                        "Amp = 3.5 + .05*sin(meshy*L + %r)" % phase)
+    ss.setIC(ic, {"waveLength": 4, "delta": 2.0 * numpy.pi / npts * 4})
     return ss
 
 

@@ -59,6 +59,13 @@ CASES = [
     ((1024, 12, 8), ("PERI", "NONE", "NONE")),
     ((12, 1000, 8), ("NONE", "NONE", "PERI")),
     ((8, 9, 1024), ("NONE", "PERI", "NONE")),
+    # bounded x lines of more than 16 chunks (x kernel, LU tables from global
+    # memory): 1024 points, and 716 (uneven partition, 23 chunks)
+    ((1024, 12, 8), ("NONE", "PERI", "NONE")),
+    ((716, 10, 3), ("NONE", "PERI", "NONE")),
+    # the 2-D RM grids: bounded x along the shock, periodic y of 512 points
+    ((1024, 512, 1), ("NONE", "PERI", "PERI")),
+    ((716, 512, 1), ("NONE", "PERI", "PERI")),
 ]
 
 

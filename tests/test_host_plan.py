@@ -120,6 +120,7 @@ CASES = [
     # global memory on the device), and an uneven partition of 1000 points
     ((1024, 8, 3), ("PERI", "NONE", "NONE")),
     ((8, 1000, 3), ("NONE", "NONE", "NONE")),
+    ((716, 8, 3), ("NONE", "PERI", "NONE")),
 ]
 
 

@@ -582,3 +582,38 @@ def test_mlp_oracle_features_layout():
     assert x[p, 1 * 9 + 0] == fields[1][0, ny - 1]        # dx=-1 -> clamp 0, dy=-1 -> wrap
     assert x[p, 2 * 9 + 8] == fields[2][1, 1]             # dx=+1, dy=+1
     assert x[p, 4] == fields[0][0, 0] and (x[:, mc.NF:] == 0).all()
+
+
+# --- cylindrical compositions (oracle/cyl_cpu.py) ----------------------------
+def _cyl(oracle_mod, nx=48, nz=40):
+    """(r, z) grid whose SYMM face sits exactly on the axis (x1 = xn/(2nx-1)
+    with the face shift of parcop.f90:46-56), so even and odd extensions of
+    polynomials in r are polynomials."""
+    from oracle.cyl_cpu import CylOracle
+    L = 2 * np.pi
+    dom = [(1.0 / (2 * nx - 1), 1.0, nx), (-0.1, 0.1, 1), (0.0, L * (nz - 1) / nz, nz)]
+    (X, Y, Z), dd = mesh(dom)
+    P = oracle_mod.Oracle((nx, 1, nz), dd, (False, False, True), symmetric=[[True, False], [False, False], [False, False]])
+    return CylOracle(P, X, dd, isym=(-1, 1, 1)), X, Z
+
+
+def test_cylindrical_closed_forms(oracle_mod):
+    """Polynomials in r with the parities the reference assigns
+    (operators.f90:54-63, 124-147, 527-544, 876-885): div(r, 0, sin z) =
+    2 + cos z; the radial component of divT with fxx = 1 + r^2 (r fxx odd:
+    isymX**3), fyy = 1 is 3r; lap(r^2) = 4; fbar(1) = 1 (the cell volume is
+    odd in r: the flipped parity). Each fails if a parity is swapped or the
+    radius weight is dropped."""
+    C, r, z = _cyl(oracle_mod)
+    zero = 0 * r
+    assert np.abs(C.div(r, zero, np.sin(z)) - (2 + np.cos(z))).max() < 1e-11
+    dfx, dfy, dfz = C.divT(1 + r ** 2, zero, zero, zero, 1 + zero, zero, zero, zero, zero)
+    assert np.abs(dfx - 3 * r).max() < 1e-11 and np.abs(dfy).max() < 1e-12 and np.abs(dfz).max() < 1e-12
+    assert np.abs(C.laplacian(r ** 2) - 4).max() < 1e-10
+    assert np.abs(C.fbar(1 + zero) - 1).max() < 1e-12
+    # the planted slips fail
+    from oracle import pyoracle as po
+    wrong = (C.P.apply(po.D1, 0, r * (1 + r ** 2)) - 1) / r        # even member on odd data
+    assert np.abs(wrong - 3 * r).max() > 1e-3
+    flat = C.P.apply(po.SF, 0, r) / r                                # no flipped parity
+    assert np.abs(flat - 1).max() > 1e-4
