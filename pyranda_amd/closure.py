@@ -279,17 +279,27 @@ class pyrandaLearnedLAD(pyrandaPackage):
             adiff = torch.where(sel, pa.to(adiff.dtype), adiff)
         return [beta, adiff]
 
-    def judge(self, div, vx, uy, rho, Yh, dt, feats=None):
+    def judge(self, div, vx, uy, rho, Yh, dt, feats=None, raw=False):
         """Closure error against the plain chain on the same state:
-        {name: (rel_l2, rel_max)} for beta and adiff."""
+        {name: (rel_l2, rel_max)} for beta and adiff, of what closure()
+        returns (the plain values at the points the sensor flags) or, with
+        raw=True, of the network alone."""
         if feats is None:
             feats = self.features(div, vx, uy, rho)
-        got = self.model.infer(feats, self.periodic)
+        m = self.model
+        if raw:
+            got = m.infer(feats, self.periodic)
+            sel = None
+        else:
+            got, mask, _ = m.infer(feats, self.periodic, sensor=self.sensor_factor * m.fmax)
+            sel = mask.bool()
         got = [got[0], got[1] / dt]
         ref = self.plain()
         out = {}
         for name, g, r in zip(("beta", "adiff"), got, ref):
             r64, g64 = r.double(), g.double()
+            if sel is not None:
+                g64 = torch.where(sel, r64, g64)
             den = float(r64.norm()) or 1.0
             out[name] = (float((g64 - r64).norm()) / den,
                          float((g64 - r64).abs().max()) / (float(r64.abs().max()) or 1.0))

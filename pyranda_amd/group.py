@@ -299,7 +299,8 @@ class _GroupBC(pyrandaBC):
 
     def constant(self, var, direction, val):
         i0, _, _ = self._planes(direction)
-        self.pyranda.variables[var].data.view(-1)[i0] = val
+        # the value as a kernel argument (no host-to-device copy: capturable)
+        self.pyranda.variables[var].data.view(-1).index_fill_(0, i0, float(val))
 
     def field(self, var, direction, field):
         raise NotImplementedError("bc.field over a group")

@@ -173,6 +173,33 @@ def test_wall_group_full_ensemble_matches_members_alone(pr):
         assert float(v[:, 0, :].abs().max()) == 0.0 and float(v[:, -1, :].abs().max()) == 0.0
 
 
+def test_wall_group_graph_replays_match_eager(pr):
+    """One step of a wall-bounded group (grouped bounded LU, BC gathers and
+    scatters, segmented reductions) captured as a HIP graph: each replay, with
+    eager work and host synchronisation between replays, is bit for bit the
+    eager step of an identical group."""
+    from pyranda_amd.ensemble import pyrandaEnsemble
+    from pyranda_amd.workloads import kh_advance, kh_wall_group
+    sizes = [(128, 96), (200, 130), (64, 48)]
+    g, gst = kh_wall_group(pr, sizes, precision="fp32")
+    ref, rst = kh_wall_group(pr, sizes, precision="fp32")
+    ens = pyrandaEnsemble([g], kh_advance, [gst])
+    ens.step_eager(1)
+    kh_advance(ref, rst)
+    ens.capture(concurrent=False)
+    junk = torch.randn(1 << 20, device="cuda")
+    for rep in range(3):
+        ens.step()
+        junk = junk * 0.5 + 1.0
+        torch.cuda.synchronize()
+        kh_advance(ref, rst)
+        torch.cuda.synchronize()
+        for k in ("rho", "rhou", "rhov", "Et", "v"):
+            assert _same(g.var(k).data, ref.var(k).data), (rep, k)
+        assert _same(ens.state[0]["dt"], rst["dt"]), (rep, "dt")
+        _say("replay", rep, "ok")
+
+
 def test_group_mode_switch_builds_new_plans(pr):
     """A grouped FP32 operator applied under the refined solve, then with the
     members switched to FP64 solves: each result equals every member's own
