@@ -40,15 +40,15 @@ HBM_PEAK_GBS = 8000.0
 # commands are in profiles/line_ops_hbm_traffic_pmc.md. Key: (op, direction,
 # field dtype) of the circulant solve.
 MEASURED_TRAFFIC = {
-    ("d1", "x", "torch.float64"): (131401 + 131072) * 1024,
-    ("d1", "x", "torch.float32"): (65950 + 65536) * 1024,
-    ("d1", "y", "torch.float64"): (131468 + 131072) * 1024,
-    ("d1", "y", "torch.float32"): (65848 + 65536) * 1024,
+    ("d1", "x", "torch.float64"): (131472 + 131072) * 1024,
+    ("d1", "x", "torch.float32"): (65959 + 65536) * 1024,
+    ("d1", "y", "torch.float64"): (131687 + 131072) * 1024,
+    ("d1", "y", "torch.float32"): (65845 + 65536) * 1024,
     ("fbar", "y", "torch.float64"): (131510 + 131072) * 1024,
     ("fbar", "y", "torch.float32"): (65952 + 65536) * 1024,
     ("gbar", "y", "torch.float64"): (140248 + 131072) * 1024,
     ("gbar", "y", "torch.float32"): (65844 + 65536) * 1024,
-    ("fbar", "x", "torch.float64"): (131490 + 131072) * 1024,
+    ("fbar", "x", "torch.float64"): (131496 + 131072) * 1024,
     # FP64 fbar z sweep that also stores the FP32 rounding (20 B/pt)
     ("fbar", "z", "torch.float64+float32"): (131502 + 196608) * 1024,
 }
@@ -269,7 +269,28 @@ def ops_bench(n, steps, warmup, dev, only=None, solver="circulant", r32_solve="r
             print("[ops] %-10s %-4s %8.1f us %7.1f GB/s (%.1f%%) err %.2e" %
                   (cname, name, us, gbs, 100 * gbs / HBM_PEAK_GBS, err), file=sys.stderr, flush=True)
     del exact
-    best = max((kv for kv in out.items() if not kv[0].startswith("copy")),
+    if wanted("host_d1_x_f64"):
+        # the f2py-module stand-in: a host Fortran-order array in, a new host
+        # array out (host -> device copy, the d1 x sweep, device -> host copy),
+        # wall clock per call
+        from pyranda_amd.f2py_parcop import parcop
+        parcop.setup(0, 0, 0, n, n, n, 1, 1, 1, 0, 0.0, L, 0.0, L, 0.0, L,
+                     "PERI", "PERI", "PERI", "PERI", "PERI", "PERI")
+        host = np.asfortranarray(f64.permute(2, 1, 0).contiguous().cpu().numpy().transpose(2, 1, 0))
+        hreps = 3
+        r = parcop.ddx(host)
+        t0 = time.perf_counter()
+        for _ in range(hreps):
+            r = parcop.ddx(host)
+        us = (time.perf_counter() - t0) * 1e6 / hreps
+        ref = ctx.apply(0, 0, f64).permute(2, 1, 0).contiguous().cpu().numpy().transpose(2, 1, 0)
+        gbs = n ** 3 * 16 / (us * 1e-6) / 1e9
+        out["host_d1_x_f64"] = {"us": round(us, 1), "GB_s": round(gbs, 1), "hbm_frac": None,
+                                "equal_to_device_path": bool(np.array_equal(r, ref))}
+        print("[ops] %-10s %-4s %8.1f us %7.1f GB/s (host arrays: PCIe-inclusive)" % ("host_d1_x", "f64", us, gbs),
+              file=sys.stderr, flush=True)
+        del host, r, ref
+    best = max((kv for kv in out.items() if not kv[0].startswith(("copy", "host"))),
                key=lambda kv: kv[1]["GB_s"], default=(None, None))
     print(json.dumps({"metric": "compact operator throughput, 256^3 periodic Fourier field",
                       "unit": "GB/s (algorithmic: one read + one write per point)",
@@ -353,7 +374,9 @@ def closure_bench(args, dev):
     2-D branch): snapshots of the plain run train the BF16-MFMA MLP; then the
     MLP's evaluation of beta and adiff is timed against the plain chain on the
     same state, RK4 steps with either are timed, and the judge reports the
-    closure error and the solution drift after the same steps."""
+    closure error and the solution drift after the same steps. The closure's
+    first evaluation is judged (pyrandaLearnedLAD's judge_first): above its
+    tolerance the closure is off for the run, and the line says so."""
     import torch
     import pyranda_amd as pr
     from pyranda_amd.closure import MLPClosure, pyrandaLearnedLAD
@@ -434,6 +457,9 @@ def closure_bench(args, dev):
                      "useful_tflops": useful_flops / (ms_mlp * 1e-3) / 1e12, "traffic": None},
         "step_ms": {"plain": 1e3 * runs["plain"][1], "closure": 1e3 * runs["closure"][1]},
         "step_speedup": runs["plain"][1] / runs["closure"][1],
+        "closure_enabled_at_end": lad.enabled,
+        "judged": [{"call": c_, **{k: {"rel_l2": a_, "rel_max": b_} for k, (a_, b_) in r_.items()}}
+                   for c_, r_ in lad.judged],
         "closure_calls": lad.calls, "fallbacks": lad.fallbacks, "partial_fallbacks": lad.partial,
         "points_flagged": lad.points_flagged,
         "judge": {k: {"rel_l2": a_, "rel_max": b_} for k, (a_, b_) in judge.items()},

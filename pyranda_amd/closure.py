@@ -20,10 +20,11 @@ with the plain chain as fallback, at two levels:
     (inputs the network was never fitted on); when any point is flagged the
     plain chain is evaluated and its values replace the closure's at the
     flagged points (one host read of the flagged count per evaluation);
-  * per run: every `judge_every` evaluations the judge compares the closure
-    with the plain chain on the current state; a relative L2 error above
-    `judge_tol` in beta or adiff switches the closure off for the rest of
-    the run (every later evaluation is the plain chain).
+  * per run: the first evaluation (`judge_first`) and every `judge_every`-th
+    one are judged: the closure is compared with the plain chain on the
+    current state, and a relative L2 error above `judge_tol` in beta or
+    adiff switches the closure off for the rest of the run (every later
+    evaluation is the plain chain).
 An untrained closure (no training range) falls back on every evaluation.
 The network's second output is adiff * dt (the chain's value before its
 division by the time step, which is not an input); the closure divides by
@@ -199,7 +200,8 @@ class pyrandaLearnedLAD(pyrandaPackage):
     arguments are the simulation's own fields; the plain chain reads them
     through its EOM lines (`plain_lines`, written to the same variables)."""
 
-    def __init__(self, pysim, model, sensor_factor=1.5, plain_lines=RM_PLAIN_LAD, judge_every=0, judge_tol=1.0):
+    def __init__(self, pysim, model, sensor_factor=1.5, plain_lines=RM_PLAIN_LAD, judge_every=0, judge_tol=0.5,
+                 judge_first=True):
         pyrandaPackage.__init__(self, "LearnedLAD", pysim)
         self.pysim = pysim
         self.model = model
@@ -207,6 +209,7 @@ class pyrandaLearnedLAD(pyrandaPackage):
         self.plain_lines = plain_lines
         self.judge_every = int(judge_every)
         self.judge_tol = float(judge_tol)
+        self.judge_first = bool(judge_first)
         self._plain_eqs = None
         self.enabled = True
         self.calls = 0
@@ -260,7 +263,8 @@ class pyrandaLearnedLAD(pyrandaPackage):
             self.fallbacks += 1
             return list(self.plain())
         feats = self.features(div, vx, uy, rho)
-        if self.judge_every and self.calls % self.judge_every == 0:
+        first = self.judge_first and not self.judged
+        if first or (self.judge_every and self.calls % self.judge_every == 0):
             rep = self.judge(div, vx, uy, rho, Yh, dt, feats=feats)
             self.judged.append((self.calls, rep))
             if max(rep["beta"][0], rep["adiff"][0]) > self.judge_tol:

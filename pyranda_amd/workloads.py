@@ -398,7 +398,7 @@ RM_CLOSURE_LINE = "[:beta:,:adiff:] = lad.closure(:div:,:vx:,:uy:,:rho:,:Yh:,:dt
 
 
 def rm_setup(pr, npts=128, precision="fp64", seed=1234, closure=None, name="RM_2D", device=None, nx=None,
-             sensor_factor=None, single_mode=False):
+             closure_kw=None, single_mode=False):
     """examples/RM3D.py, 2-D branch, on an nx x npts grid (nx defaults to
     int(1.4 npts), the script's choice; x spans 2.8 pi along the shock, y
     2 pi periodic across it); the IC's random perturbation draws from numpy's
@@ -409,7 +409,8 @@ def rm_setup(pr, npts=128, precision="fp64", seed=1234, closure=None, name="RM_2
     the noise moves the interface by about two cells from one cell to the
     next, and the first step, whose dt is the initial Courant limit, is
     unstable). With `closure` (an MLPClosure) the beta and adiff lines become
-    the learned closure (plain chain as fallback)."""
+    the learned closure (plain chain as fallback); `closure_kw` are keyword
+    arguments of its pyrandaLearnedLAD (sensor_factor, judge_first, ...)."""
     import numpy
     from .closure import pyrandaLearnedLAD
     nx = int(npts * 1.4) if nx is None else int(nx)
@@ -427,8 +428,7 @@ def rm_setup(pr, npts=128, precision="fp64", seed=1234, closure=None, name="RM_2
     ss.addPackage(pr.pyrandaBC(ss))
     eom = RM_EOM
     if closure is not None:
-        ss.addPackage(pyrandaLearnedLAD(ss, closure) if sensor_factor is None
-                      else pyrandaLearnedLAD(ss, closure, sensor_factor=sensor_factor))
+        ss.addPackage(pyrandaLearnedLAD(ss, closure, **(closure_kw or {})))
         eom = eom.replace(RM_LAD_LINES[0], RM_CLOSURE_LINE)
         for line in RM_LAD_LINES[1:]:
             eom = eom.replace(line + "\n", "")

@@ -223,7 +223,7 @@ def test_closure_per_point_fallback_takes_the_plain_chain_where_flagged(pr):
     from pyranda_amd.workloads import rm_run, rm_setup
     ss = rm_setup(pr, 48)
     rm_run(ss, 3)
-    lad = pyrandaLearnedLAD(ss, None)
+    lad = pyrandaLearnedLAD(ss, None, judge_first=False)
     args = [ss.var(k).data for k in ("div", "vx", "uy", "rho", "Yh")] + [ss.var("dt").data]
     feats = lad.features(*args[:4])
     model = MLPClosure("cuda")
@@ -251,7 +251,7 @@ def test_closure_judge_switches_the_closure_off(pr):
     from pyranda_amd.workloads import rm_run, rm_setup
     ss = rm_setup(pr, 48)
     rm_run(ss, 2)
-    lad = pyrandaLearnedLAD(ss, None, judge_every=2, judge_tol=1e-3)
+    lad = pyrandaLearnedLAD(ss, None, judge_every=2, judge_tol=1e-3, judge_first=False)
     args = [ss.var(k).data for k in ("div", "vx", "uy", "rho", "Yh")] + [ss.var("dt").data]
     model = MLPClosure("cuda")
     feats = lad.features(*args[:4])
@@ -267,6 +267,30 @@ def test_closure_judge_switches_the_closure_off(pr):
     assert lad.fallbacks == 2
     for got in ((b2, a2), (b3, a3)):
         assert torch.equal(got[0], pb) and torch.equal(got[1], pa)
+
+
+def test_closure_first_evaluation_is_judged(pr):
+    """judge_first (the default): the first evaluation of a trained closure
+    is judged; above judge_tol the closure is off from that evaluation on,
+    and it returns the plain chain's values."""
+    from pyranda_amd.closure import MLPClosure, pyrandaLearnedLAD
+    from pyranda_amd.workloads import rm_run, rm_setup
+    ss = rm_setup(pr, 48)
+    rm_run(ss, 2)
+    args = [ss.var(k).data for k in ("div", "vx", "uy", "rho", "Yh")] + [ss.var("dt").data]
+    for tol, on in ((1e-3, False), (1e30, True)):
+        lad = pyrandaLearnedLAD(ss, None, judge_tol=tol)
+        model = MLPClosure("cuda")
+        model.set_normalisation([lad.features(*args[:4])], [[ss.var("beta").data, ss.var("adiff").data]])
+        lad.model = model
+        b1, a1 = lad.closure(*args)
+        assert len(lad.judged) == 1 and lad.enabled == on, tol
+        lad.closure(*args)
+        assert len(lad.judged) == 1                 # judge_every = 0: the first only
+        if not on:
+            pb, pa = lad.plain()
+            torch.cuda.synchronize()
+            assert lad.fallbacks == 2 and torch.equal(b1, pb) and torch.equal(a1, pa)
 
 
 def test_mlp_learns_a_smooth_map(pr):
@@ -346,7 +370,7 @@ def test_closure_trained_on_rm(pr):
     losses = model.train(samples, (False, True), steps=200, lr=2e-3)
     _say("training mse first/last", float(losses[0]), float(losses[-1]))
     assert float(losses[-8:].mean()) < float(losses[:8].mean())
-    run = rm_setup(pr, 64, closure=model)
+    run = rm_setup(pr, 64, closure=model, closure_kw={"judge_first": False})
     rm_run(run, 5)
     lad = run.packages["LearnedLAD"]
     rep = lad.judge(*[run.var(k).data for k in ("div", "vx", "uy", "rho", "Yh")], run.var("dt").data)
