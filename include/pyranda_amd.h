@@ -196,7 +196,7 @@ int pyr_rk_update_dt(pyr_ctx* ctx, int prec, long long n, const void* F, double*
  * fields: in the context's current FP32 solve mode): x_lds_kb is the LDS
  * budget of an x-line workgroup (8..160 KB; it sets the lines per
  * workgroup), yz_pipe switches the pipelined y/z kernel (0/1), yz_lines the
- * lines per plain y/z workgroup (32 or 64); -1 keeps the context default.
+ * lines per plain y/z workgroup (16, 32 or 64); -1 keeps the context default.
  * Set by pyranda_amd's autotuner (ParcopContext.autotune), which times the
  * candidates on the device. No counterpart in the reference. */
 int pyr_set_tuning(pyr_ctx* ctx, int op, int dir, int prec, int x_lds_kb, int yz_pipe,

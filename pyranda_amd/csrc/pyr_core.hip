@@ -977,7 +977,7 @@ struct pyr_ctx {
   int yz_pipe = 1;                           // pipelined y/z kernels where the shape allows
   // per operator, direction and solve mode (MODE_*): tuned values of the two
   // knobs above, -1 = the context default (pyr_set_tuning)
-  int yz_lines = 0;                          // lines per plain y/z workgroup (0: 64, or 32 past 8 chunks)
+  int yz_lines = 0;                          // lines per plain y/z workgroup (0: 64, or 32 past 8 chunks; 16, 32)
   int tune_xlds[NOPS][3][3];
   int tune_pipe[NOPS][3][3];
   int tune_ly[NOPS][3][3];
@@ -1279,8 +1279,9 @@ static int launch_typed(pyr_ctx* c, int dir, LineArgs a, const TS* in, TS* out) 
     hipLaunchKernelGGL(k, grid, block, shm, c->stream, a, in, out);
   } else {
     // lines per workgroup: 64 x P chunk threads within WG_MAX (32 or 16 lines
-    // for longer lines)
-    const int ly = (P <= 8 && c->yz_lines != 32) ? 64 : (P <= WG_MAX / 32 ? 32 : 16);
+    // for longer lines); tuned: 32 or 16 lines (more, smaller workgroups per CU)
+    const int ly = c->yz_lines == 16 ? 16
+                 : ((P <= 8 && c->yz_lines != 32) ? 64 : (P <= WG_MAX / 32 ? 32 : 16));
     const size_t ex = align16((size_t)4 * P * ly * sizeof(CS));
     const size_t shm = SOLVE == SOLVE_CIRC ? 2 * ex : (SOLVE == SOLVE_LU ? ex + stage : 0);
     const dim3 grid((a.nlines + ly - 1) / ly), block(ly, P);
@@ -2042,7 +2043,8 @@ int pyr_set_tuning(pyr_ctx* c, int op, int dir, int prec, int x_lds_kb, int yz_p
   if (op < 0 || op >= NOPS || dir < 0 || dir > 2) return fail("bad operator or direction");
   if (x_lds_kb >= 0 && (x_lds_kb < 8 || x_lds_kb > 160)) return fail("x_lds_kb must be 8..160 (or -1)");
   if (yz_pipe > 1) return fail("yz_pipe must be 0, 1 (or -1)");
-  if (yz_lines >= 0 && yz_lines != 32 && yz_lines != 64) return fail("yz_lines must be 32 or 64 (or -1)");
+  if (yz_lines >= 0 && yz_lines != 16 && yz_lines != 32 && yz_lines != 64)
+    return fail("yz_lines must be 16, 32 or 64 (or -1)");
   int mode;
   if (prec == PYR_F64) mode = MODE_F64;
   else if (prec == PYR_R32) mode = r32_mode(c, op, dir) == PYR_R32_SOLVE_FP64 ? MODE_S32 : MODE_R32;

@@ -160,7 +160,7 @@ class ParcopContext:
                     cands = [(("x_lds_kb", kb), ("yz_pipe", pp)) for pp in (0, 1) for kb in x_lds_kb]
                 else:
                     cands = [(("yz_pipe", 1),), (("yz_pipe", 0), ("yz_lines", 64)),
-                             (("yz_pipe", 0), ("yz_lines", 32))]
+                             (("yz_pipe", 0), ("yz_lines", 32)), (("yz_pipe", 0), ("yz_lines", 16))]
                 if solve_modes and dtype == torch.float32:
                     cands = [(("r32_solve", m),) + c for m in (0, 1) for c in cands]
                 times = {}

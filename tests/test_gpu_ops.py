@@ -247,6 +247,35 @@ def test_autotune_keeps_results(dev, dtype):
 
 
 @pytest.mark.parametrize("nn,kinds", [((64, 96, 64), ("PERI", "PERI", "PERI")),
+                                      ((40, 72, 50), ("NONE", "NONE", "NONE"))])
+def test_yz_lines_per_workgroup_keep_results(dev, nn, kinds):
+    """16, 32 and 64 lines per y/z workgroup (pyr_set_tuning yz_lines, the
+    plain kernel) give the same result bit for bit, FP64 and FP32, periodic
+    (circulant) and bounded (chunked LU) lines, line counts that do not fill
+    the last workgroup included."""
+    from pyranda_amd import _lib
+    from pyranda_amd.parcop import ParcopContext
+    L = 2 * math.pi
+    per = kinds[0] == "PERI"
+    xn = tuple(L * (n - 1) / n for n in nn) if per else (L, L, L)
+    ctx = ParcopContext(nn, (0.0, 0.0, 0.0), xn, [k for k in kinds for _ in (0, 1)], device=dev)
+    g = torch.Generator(device=dev).manual_seed(5)
+    ops = (_lib.OP_D1, _lib.OP_D2, _lib.OP_D8, _lib.OP_SFILTER, _lib.OP_GFILTER)
+    for dtype in (torch.float64, torch.float32):
+        v = torch.randn(tuple(reversed(nn)), generator=g, device=dev, dtype=dtype).permute(2, 1, 0)
+        for op in ops:
+            for d in (1, 2):
+                got = {}
+                for ly in (64, 32, 16):
+                    ctx.set_tuning(op, d, dtype, yz_pipe=0, yz_lines=ly)
+                    got[ly] = ctx.apply(op, d, v).clone()
+                ctx.set_tuning(op, d, dtype)
+                _say("yz lines", nn, kinds[0], dtype, op, d)
+                assert torch.equal(got[16], got[64]) and torch.equal(got[32], got[64]), (dtype, op, d)
+    ctx.close()
+
+
+@pytest.mark.parametrize("nn,kinds", [((64, 96, 64), ("PERI", "PERI", "PERI")),
                                       ((33, 40, 20), ("NONE", "NONE", "NONE")),
                                       ((64, 48, 1), ("PERI", "PERI", "PERI"))])
 def test_filter_with_rounded_copy(dev, nn, kinds):
