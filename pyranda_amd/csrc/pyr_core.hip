@@ -242,6 +242,12 @@ __device__ __forceinline__ const typename SolveT<MODE>::type* kmat(const LineArg
   else return which ? a.kb : a.kf;
 }
 
+// Entries per chunk of each of the two exchange buffers of solve_chunk_circ
+// (per line): the forward and backward carries take 2; the refined FP32 solve
+// also exchanges the 4 edge values of its first solution.
+template <int MODE>
+constexpr int circ_words() { return MODE == MODE_R32 ? 4 : 2; }
+
 // kf/kb: the carry rows in the solve type (global memory or an LDS copy)
 template <int MODE, typename CS, bool GRP = false, typename Emit>
 __device__ __forceinline__ void solve_chunk_circ(const LineArgs& a, const CS* kf, const CS* kb, int p,
@@ -695,8 +701,8 @@ __device__ __forceinline__ void line_x_body(const LineArgs& a, long long blk, co
   if (SOLVE == SOLVE_CIRC) {
     // the first carry barrier also retires every read of the tile
     CS* X0 = reinterpret_cast<CS*>(smem + align16((size_t)LX * pitch * sizeof(TS)));
-    solve_chunk_circ<MODE, CS, GRP>(a, kmat<MODE>(a, 0), kmat<MODE>(a, 1), p, m, X0, X0 + 4 * a.P * LX, LX, l,
-                                    rhs, emit);
+    solve_chunk_circ<MODE, CS, GRP>(a, kmat<MODE>(a, 0), kmat<MODE>(a, 1), p, m, X0,
+                                    X0 + circ_words<MODE>() * a.P * LX, LX, l, rhs, emit);
   } else {
 #pragma unroll
     for (int k = 0; k < CHUNK; ++k)
@@ -1229,7 +1235,7 @@ static int launch_typed(pyr_ctx* c, int dir, LineArgs a, const TS* in, TS* out) 
     a.pitch = a.vec ? a.N + VW : a.N + 1;
     auto lds = [&](int lx) {
       const size_t tile = align16((size_t)lx * a.pitch * sizeof(TS));
-      if (SOLVE == SOLVE_CIRC) return tile + 2 * align16((size_t)4 * P * lx * sizeof(CS));
+      if (SOLVE == SOLVE_CIRC) return tile + 2 * align16((size_t)circ_words<MODE>() * P * lx * sizeof(CS));
       if (SOLVE == SOLVE_LU) {
         const size_t work = align16((size_t)4 * P * lx * sizeof(CS)) + stage;
         return tile > work ? tile : work;

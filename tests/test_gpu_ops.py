@@ -336,7 +336,7 @@ def test_autotune_solve_modes(dev):
     the one of a context using the chosen arithmetic everywhere, and within
     the FP32-refined bar of the FP64 path on the same input."""
     from pyranda_amd import _lib
-    from pyranda_amd.parcop import ParcopContext
+    from pyranda_amd.parcop import SOLVE_MODE_MARGIN, ParcopContext
     nn = (64, 96, 64)
     L = 2 * math.pi
     xn = tuple(L * (n - 1) / n for n in nn)
@@ -352,6 +352,10 @@ def test_autotune_solve_modes(dev):
     for (op, d), (best, times) in rep.items():
         assert best[0][0] == "r32_solve" and best in times
         mode = ("refined", "fp64")[best[0][1]]
+        if mode == "fp64":
+            # the FP64 solve only for a clear gain over the refined solve
+            t_ref = min(t for c, t in times.items() if c[0] == ("r32_solve", 0))
+            assert times[best] <= (1.0 - SOLVE_MODE_MARGIN) * t_ref, (op, d)
         chosen[(op, d)] = mode
         got = ctx.apply(op, d, v32)
         assert torch.equal(got, ref[mode].apply(op, d, v32)), (op, d, mode)
