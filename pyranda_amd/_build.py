@@ -15,6 +15,12 @@ FLAGS = [
     "-O3", "-std=c++17", "--offload-arch=gfx950", "-fPIC", "-shared",
     # keep packed-FP32 instructions out of the device code
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # multiply-adds fused only within one source expression: the same source
+    # then rounds the same way in every kernel it is inlined into (the
+    # default, fast, lets the backend fuse across statements depending on
+    # the surrounding code, so two launch shapes of one operator could differ
+    # in the last bit)
+    "-ffp-contract=on",
 ]
 
 

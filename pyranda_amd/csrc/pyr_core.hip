@@ -860,6 +860,142 @@ k_line_x_pipe(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
   }
 }
 
+// x lines, software-pipelined through registers (periodic lines of whole
+// 32-row chunks, 16-byte aligned rows, circulant solve or explicit filter).
+// A persistent workgroup walks groups of LX lines through ONE LDS tile: while
+// it solves group g from the tile, the rows of its next group are already in
+// flight from HBM into registers (CHUNK elements per thread, as 16-byte
+// vectors), and they go into the tile once group g's results have left it.
+// Against k_line_x_pipe (a second LDS tile filled by DMA) this keeps the LDS
+// footprint, hence the workgroups per CU, of the one-pass kernel, and pays
+// for the prefetch in registers, which the LDS-bound occupancy leaves free.
+// Every LX * P threads move LX * N elements: CHUNK per thread exactly. The
+// prefetch is issued after the windows have been turned into right-hand
+// sides, so that it overlaps the solve and the stores.
+template <typename TS, int KIND, int SOLVE, int MODE>
+__global__ void __launch_bounds__(WG_MAX) __attribute__((amdgpu_waves_per_eu(2, 8)))
+k_line_x_rpipe(LineArgs a, const TS* __restrict__ in, TS* __restrict__ out) {
+  constexpr int R = Halo<KIND>::R;
+  using CS = typename SolveT<MODE>::type;
+  using V = typename Vec16<TS>::type;
+  constexpr int VW = Vec16<TS>::W;
+  constexpr int NV = CHUNK / VW;  // prefetched vectors per thread
+  extern __shared__ __align__(16) unsigned char smem[];
+  const int N = a.N, LX = a.lx, pitch = a.pitch;
+  const int tid = threadIdx.x, nthr = blockDim.x;
+  const int l = tid % LX, p = tid / LX;
+  const int vpl = N / VW;
+  TS* tile = reinterpret_cast<TS*>(smem);
+  CS* X0 = reinterpret_cast<CS*>(smem + align16((size_t)LX * pitch * sizeof(TS)));
+  CS* X1 = X0 + circ_words<MODE>() * a.P * LX;
+  const int ngroups = (a.nlines + LX - 1) / LX;
+  auto nval_of = [&](int g) {
+    const long long rem = (long long)a.nlines - (long long)g * LX;
+    return (int)(rem < LX ? rem : LX);
+  };
+  V pf[NV];
+  const int s = p * CHUNK;
+  constexpr int m = CHUNK;
+  const bool need_old = (a.epi == EPI_ADD || a.epi == EPI_MAXABS);
+  int g = blockIdx.x;
+  if (g < ngroups) {
+    const int nvec = nval_of(g) * vpl;
+    const V* sv = reinterpret_cast<const V*>(in + (long long)g * LX * N);
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      const int e = tid + j * nthr;
+      pf[j] = e < nvec ? sv[e] : V{};
+    }
+  }
+  for (; g < ngroups; g += gridDim.x) {
+    const int nval = nval_of(g);
+    const int nvec = nval * vpl;
+    // group g's rows into the tile (the previous group's stores have read it:
+    // barrier at the end of the last iteration)
+#pragma unroll
+    for (int j = 0; j < NV; ++j) {
+      const int e = tid + j * nthr;
+      if (e < nvec) {
+        const int ll = e / vpl, i = e - ll * vpl;
+        *reinterpret_cast<V*>(tile + ll * pitch + i * VW) = pf[j];
+      }
+    }
+    __syncthreads();
+    const int lr = l < nval ? l : 0;
+    const TS* row0 = tile + lr * pitch;
+    double rhs[CHUNK];
+    {
+      TS win[CHUNK + 2 * R];
+#pragma unroll
+      for (int w = 0; w < CHUNK + 2 * R; ++w) {
+        int row = s - R + w;
+        row = row < 0 ? row + N : (row >= N ? row - N : row);
+        win[w] = row0[row];
+      }
+      rhs_interior<KIND>(a, win, rhs, m);
+    }
+    // the next group's rows travel while this group is solved and stored
+    // (issued once the windows are consumed: their registers take the rows)
+    const int gn = g + gridDim.x;
+    __builtin_amdgcn_sched_barrier(0);  // keep the prefetch below the windows
+    if (gn < ngroups) {
+      const int nvn = nval_of(gn) * vpl;
+      const V* sv = reinterpret_cast<const V*>(in + (long long)gn * LX * N);
+#pragma unroll
+      for (int j = 0; j < NV; ++j) {
+        const int e = tid + j * nthr;
+        pf[j] = e < nvn ? sv[e] : V{};
+      }
+    }
+    TS* rowo = tile + l * pitch;
+    auto emit = [&](int k, double y) {
+      if (l < nval) {
+        double val = y * a.scale;
+        if (a.addv) val += (double)rowo[s + k];
+        rowo[s + k] = (TS)val;
+      }
+    };
+    if constexpr (SOLVE == SOLVE_CIRC) {
+      // the first carry barrier also retires every window read of the tile
+      solve_chunk_circ<MODE, CS>(a, kmat<MODE>(a, 0), kmat<MODE>(a, 1), p, m, X0, X1, LX, l, rhs, emit);
+    } else {
+      __syncthreads();  // every window read before any result lands in the tile
+#pragma unroll
+      for (int k = 0; k < CHUNK; ++k) emit(k, rhs[k]);
+    }
+    __syncthreads();
+    const long long L0 = (long long)g * LX;
+    if constexpr (sizeof(TS) == 8) {
+      if (a.out32) {
+        // FP64 result rounded to FP32, with or without the FP64 store
+        double* dst = reinterpret_cast<double*>(out) + L0 * N;
+        float* d32 = a.out32 + L0 * N;
+#pragma unroll XU
+        for (int e = tid; e < nvec; e += nthr) {
+          const int ll = e / vpl, i = e - ll * vpl;
+          const double2 r = *reinterpret_cast<const double2*>(tile + ll * pitch + i * VW);
+          if (!a.only32) reinterpret_cast<double2*>(dst)[e] = r;
+          reinterpret_cast<float2*>(d32)[e] = make_float2((float)r.x, (float)r.y);
+        }
+        __syncthreads();  // every read of the tile before the next group lands
+        continue;
+      }
+    }
+    V* dv = reinterpret_cast<V*>(out + L0 * N);
+#pragma unroll XU
+    for (int e = tid; e < nvec; e += nthr) {
+      const int ll = e / vpl, i = e - ll * vpl;
+      V r = *reinterpret_cast<const V*>(tile + ll * pitch + i * VW);
+      if (a.epi != EPI_STORE) {
+        const V o = need_old ? dv[e] : V{};
+        vzip(r, o, [&](TS y, TS old) { return epilogue<TS>(a.epi, a.epi_c, (double)y, old); });
+      }
+      dv[e] = r;
+    }
+    __syncthreads();  // every read of the tile before the next group lands
+  }
+}
+
 // Grouped launches: one launch applies the same operator to the fields of
 // several simulations (an ensemble's members, each at `off` elements into
 // one allocation), with lines of any lengths. Entry j covers workgroups
@@ -1242,10 +1378,38 @@ static int launch_typed(pyr_ctx* c, int dir, LineArgs a, const TS* in, TS* out) 
       }
       return tile;
     };
+    if constexpr (FULL && SOLVE == SOLVE_NONE) {
+      // register-pipelined persistent kernel (yz_pipe 2 on x; explicit
+      // filters only: with a solve the prefetched rows and the solve's
+      // registers do not fit together without spills): one tile of LX lines
+      // with 16-byte aligned rows, LX * P threads in whole waves
+      const bool rok = c->yz_pipe == 2 && a.periodic && a.N % VW == 0 && ((uintptr_t)in % 16 == 0) &&
+                       ((uintptr_t)out % 16 == 0) && (!a.out32 || (uintptr_t)a.out32 % 8 == 0);
+      if (rok) {
+        a.pitch = a.N + VW;
+        a.vec = 1;
+        int lx = WG_MAX / P;
+        if (lx > 64) lx = 64;
+        while (lx > 1 && lds(lx) > (size_t)c->x_lds_kb * 1024) lx /= 2;
+        if ((lx * P) % 64 == 0) {
+          a.lx = lx;
+          const size_t shm = lds(lx);
+          const int ngroups = (a.nlines + lx - 1) / lx;
+          const int per_cu = std::max(1, (int)((160 * 1024) / shm));
+          const int nblocks = std::min(ngroups, c->ncu * per_cu);
+          auto k = k_line_x_rpipe<TS, KIND, SOLVE, MODE>;
+          if (set_lds(k, shm)) return 1;
+          hipLaunchKernelGGL(k, dim3(nblocks), dim3(lx * P), shm, c->stream, a, in, out);
+          HIPCHECK(hipGetLastError());
+          return 0;
+        }
+        a.pitch = a.vec ? a.N + VW : a.N + 1;
+      }
+    }
     if constexpr (FULL && (SOLVE == SOLVE_CIRC || SOLVE == SOLVE_NONE)) {
       // pipelined persistent kernel: two tiles of LX lines (16-byte aligned
       // rows, whole KiB per line), LX * P threads in whole waves
-      const bool ok = c->yz_pipe && !a.out32 && a.periodic && (a.N * (int)sizeof(TS)) % 1024 == 0 &&
+      const bool ok = c->yz_pipe == 1 && !a.out32 && a.periodic && (a.N * (int)sizeof(TS)) % 1024 == 0 &&
                       ((uintptr_t)in % 16 == 0) && ((uintptr_t)out % 16 == 0);
       if (ok) {
         const int pitch = a.N + VW;
@@ -2048,7 +2212,7 @@ int pyr_set_tuning(pyr_ctx* c, int op, int dir, int prec, int x_lds_kb, int yz_p
   if (!c) return fail("null context");
   if (op < 0 || op >= NOPS || dir < 0 || dir > 2) return fail("bad operator or direction");
   if (x_lds_kb >= 0 && (x_lds_kb < 8 || x_lds_kb > 160)) return fail("x_lds_kb must be 8..160 (or -1)");
-  if (yz_pipe > 1) return fail("yz_pipe must be 0, 1 (or -1)");
+  if (yz_pipe > 2 || (yz_pipe == 2 && dir != 0)) return fail("yz_pipe must be 0, 1, 2 for x (or -1)");
   if (yz_lines >= 0 && yz_lines != 16 && yz_lines != 32 && yz_lines != 64)
     return fail("yz_lines must be 16, 32 or 64 (or -1)");
   int mode;

@@ -120,8 +120,10 @@ class ParcopContext:
 
     def set_tuning(self, op, direction, dtype, x_lds_kb=-1, yz_pipe=-1, yz_lines=-1):
         """Per-operator knobs (pyr_set_tuning): LDS budget of x-line
-        workgroups (KB), the pipelined y/z kernel, lines per plain y/z
-        workgroup; -1 = default."""
+        workgroups (KB), the pipelined kernels (y/z: 1; x: 1 with a second
+        LDS tile, 2 with the next tile prefetched into registers, explicit
+        filter only), lines per
+        plain y/z workgroup; -1 = default."""
         prec = _lib.F64 if dtype == torch.float64 else _lib.R32
         _lib.check(self._L.pyr_set_tuning(self._h, op, direction, prec, int(x_lds_kb), int(yz_pipe),
                                           int(yz_lines)))
@@ -131,7 +133,8 @@ class ParcopContext:
         """Time each operator and direction at this precision with every
         candidate (x: LDS budget, hence lines per workgroup, and the pipelined
         kernel or not; y/z: pipelined kernel or not) on random fields and keep
-        the fastest. The timed
+        the fastest (x: the one-pass kernel or a pipelined one: 1, or for the
+        explicit filter also 2). The timed
         launches cycle through enough fields to exceed the device's 256 MB
         last-level cache, so every launch reads its input from HBM as the
         operators of a time step do. rounded: FP64
@@ -162,7 +165,9 @@ class ParcopContext:
                     # the LDS budget (lines per workgroup) with the pipelined
                     # persistent x kernel (yz_pipe=1, where the line shape
                     # allows it) or the one-pass kernel
-                    cands = [(("x_lds_kb", kb), ("yz_pipe", pp)) for pp in (0, 1) for kb in x_lds_kb]
+                    # (2: the register-pipelined kernel of the explicit filter)
+                    pps = (0, 1, 2) if op == _lib.OP_GFILTER else (0, 1)
+                    cands = [(("x_lds_kb", kb), ("yz_pipe", pp)) for pp in pps for kb in x_lds_kb]
                 else:
                     cands = [(("yz_pipe", 1),), (("yz_pipe", 0), ("yz_lines", 64)),
                              (("yz_pipe", 0), ("yz_lines", 32)), (("yz_pipe", 0), ("yz_lines", 16))]

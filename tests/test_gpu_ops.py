@@ -246,6 +246,40 @@ def test_autotune_keeps_results(dev, dtype):
     ctx.close()
 
 
+@pytest.mark.parametrize("nn", [(256, 21, 9), (128, 24, 11)])
+def test_x_kernel_variants_keep_results(dev, nn):
+    """The one-pass x kernel at several LDS budgets, the pipelined kernel
+    with a second LDS tile (yz_pipe 1) and, for the explicit filter, the one
+    that prefetches the next lines into registers (yz_pipe 2) give the same
+    result bit for bit, FP64 and FP32, also for an FP64 sweep that stores
+    only its FP32 rounding; whole-KiB periodic lines (the pipelined kernels'
+    shape), line counts that leave the last group of lines partly filled."""
+    from pyranda_amd import _lib
+    from pyranda_amd.parcop import ParcopContext
+    L = 2 * math.pi
+    ctx = ParcopContext(nn, (0.0, 0.0, 0.0), tuple(L * (n - 1) / n for n in nn), ["PERI"] * 6, device=dev)
+    g = torch.Generator(device=dev).manual_seed(9)
+    ops = (_lib.OP_D1, _lib.OP_D2, _lib.OP_D8, _lib.OP_SFILTER, _lib.OP_GFILTER)
+    for dtype in (torch.float64, torch.float32):
+        v = torch.randn(tuple(reversed(nn)), generator=g, device=dev, dtype=dtype).permute(2, 1, 0)
+        for op in ops:
+            ctx.set_tuning(op, 0, dtype, x_lds_kb=40, yz_pipe=0)
+            base = ctx.apply(op, 0, v).clone()
+            base32 = ctx.apply_to32(op, 0, v).clone() if dtype == torch.float64 else None
+            shapes = [(kb, pp) for pp in (0, 1) for kb in (12, 24, 80)]
+            if op == _lib.OP_GFILTER:
+                shapes += [(kb, 2) for kb in (12, 24, 80)]
+            for kb, pp in shapes:
+                ctx.set_tuning(op, 0, dtype, x_lds_kb=kb, yz_pipe=pp)
+                got = ctx.apply(op, 0, v)
+                _say("x variants", nn, dtype, op, kb, pp)
+                assert torch.equal(got, base), (dtype, op, kb, pp)
+                if base32 is not None:
+                    assert torch.equal(ctx.apply_to32(op, 0, v), base32), (op, kb, pp)
+            ctx.set_tuning(op, 0, dtype)
+    ctx.close()
+
+
 @pytest.mark.parametrize("nn,kinds", [((64, 96, 64), ("PERI", "PERI", "PERI")),
                                       ((40, 72, 50), ("NONE", "NONE", "NONE"))])
 def test_yz_lines_per_workgroup_keep_results(dev, nn, kinds):
