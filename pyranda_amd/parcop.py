@@ -63,6 +63,10 @@ def _d1(d):
     return lambda ctx, v: ctx.apply(_lib.OP_D1, d, v)
 
 
+# the operator of each composite (timed launch by launch when it is timed)
+_COMPOSITE_OP = {"sfilter": _lib.OP_SFILTER, "gfilter": _lib.OP_GFILTER, "pring": _lib.OP_D8,
+                 "plaplacian": _lib.OP_D2, "ddx": _lib.OP_D1, "ddy": _lib.OP_D1, "ddz": _lib.OP_D1}
+
 # the launches each composite makes, as separately timed apply() calls
 _COMPOSITE = {
     "sfilter": _filter3(_lib.OP_SFILTER), "gfilter": _filter3(_lib.OP_GFILTER),
@@ -96,8 +100,10 @@ class ParcopContext:
         self.dpatch = tuple(dp)
         self.dmesh = tuple(dm)
         self.gridlen = gl.value
-        # optional per-launch timing: {(op, dir, dtype): [n_launches, [(start, end), ...]]}
+        # optional per-launch timing: {(op, dir, dtype): [(start, end), ...]},
+        # of every launch or, with timing_only a set of such keys, of those
         self.timing = None
+        self.timing_only = None
         self.coords1d = []
         for ax in range(3):
             buf = (ctypes.c_double * self.nn[ax])()
@@ -258,7 +264,8 @@ class ParcopContext:
                                   or tuple(out32.shape) != self.nn):
             raise ValueError("out32: an FP32 field for the store epilogue of an FP64 sweep")
         self._stream()
-        if self.timing is not None:
+        tkey = self._timed((op, direction, str(v.dtype) + ("+float32" if out32 is not None else "")))
+        if tkey is not None:
             ev0 = torch.cuda.Event(enable_timing=True)
             ev1 = torch.cuda.Event(enable_timing=True)
             ev0.record()
@@ -268,11 +275,20 @@ class ParcopContext:
         else:
             _lib.check(self._L.pyr_apply(self._h, op, direction, prec, self._ptr(v), self._ptr(out), epi,
                                          float(epi_scale)))
-        if self.timing is not None:
+        if tkey is not None:
             ev1.record()
-            key = str(v.dtype) + ("+float32" if out32 is not None else "")
-            self.timing.setdefault((op, direction, key), []).append((ev0, ev1))
+            self.timing.setdefault(tkey, []).append((ev0, ev1))
         return out
+
+    def _timed(self, key):
+        """key if launches of this kind are being timed, else None."""
+        if self.timing is None or (self.timing_only is not None and key not in self.timing_only):
+            return None
+        return key
+
+    def _split_composite(self, op):
+        """Time a composite launch by launch (its operator is being timed)."""
+        return self.timing is not None and (self.timing_only is None or any(k[0] == op for k in self.timing_only))
 
     def apply_to32(self, op, direction, v):
         """One directional operator of an FP64 field whose result is stored
@@ -283,14 +299,15 @@ class ParcopContext:
             raise ValueError("apply_to32 sweeps FP64 fields")
         out32 = empty_field(self.nn, torch.float32, self.device)
         self._stream()
-        if self.timing is not None:
+        tkey = self._timed((op, direction, "torch.float64->float32"))
+        if tkey is not None:
             ev0 = torch.cuda.Event(enable_timing=True)
             ev1 = torch.cuda.Event(enable_timing=True)
             ev0.record()
         _lib.check(self._L.pyr_apply_to_r32(self._h, op, direction, self._ptr(v), self._ptr(out32)))
-        if self.timing is not None:
+        if tkey is not None:
             ev1.record()
-            self.timing.setdefault((op, direction, "torch.float64->float32"), []).append((ev0, ev1))
+            self.timing.setdefault(tkey, []).append((ev0, ev1))
         return out32
 
     def timing_summary(self):
@@ -303,7 +320,7 @@ class ParcopContext:
         return out
 
     def _unary(self, name, v):
-        if self.timing is not None and name in _COMPOSITE:
+        if name in _COMPOSITE and self._split_composite(_COMPOSITE_OP[name]):
             return _COMPOSITE[name](self, v)
         v = self._prep(v)
         out = self._out(v)
@@ -329,7 +346,7 @@ class ParcopContext:
         if v.dtype != torch.float64:
             raise ValueError("sfilter_rounded filters FP64 fields")
         out32 = empty_field(self.nn, torch.float32, self.device)
-        if self.timing is not None:
+        if self._split_composite(_lib.OP_SFILTER):
             a = self.apply(_lib.OP_SFILTER, 0, v)
             b = self.apply(_lib.OP_SFILTER, 1, a)
             out = self.apply(_lib.OP_SFILTER, 2, b, out32=out32)

@@ -407,3 +407,33 @@ def test_autotune_solve_modes(dev):
     _say("solve modes", {k: m for k, m in chosen.items()})
     for c_ in [ctx] + list(ref.values()):
         c_.close()
+
+
+def test_timing_only_selected_launches(dev):
+    """Per-launch timing (ParcopContext.timing) of every launch, or with
+    timing_only of the selected (op, dir, dtype) launches only; a composite
+    is split into its sweeps only when its operator is timed. Results are
+    the untimed ones bit for bit."""
+    from pyranda_amd import _lib
+    from pyranda_amd.parcop import ParcopContext
+    nn = (64, 40, 24)
+    L = 2 * math.pi
+    ctx = ParcopContext(nn, (0.0, 0.0, 0.0), tuple(L * (n - 1) / n for n in nn), ["PERI"] * 6, device=dev)
+    g = torch.Generator(device=dev).manual_seed(4)
+    v = torch.randn(tuple(reversed(nn)), generator=g, device=dev, dtype=torch.float64).permute(2, 1, 0)
+    want = (ctx.sfilter(v).clone(), ctx.pring(v).clone(), ctx.ddx(v).clone())
+    for only in (None, {(_lib.OP_SFILTER, 0, "torch.float64")}):
+        ctx.timing, ctx.timing_only = {}, only
+        got = (ctx.sfilter(v), ctx.pring(v), ctx.ddx(v))
+        summ = ctx.timing_summary()
+        ctx.timing, ctx.timing_only = None, None
+        for a, b in zip(got, want):
+            assert torch.equal(a, b)
+        _say("timed", only, sorted(summ))
+        if only is None:
+            assert summ[(_lib.OP_SFILTER, 2, "torch.float64")][0] == 1
+            assert summ[(_lib.OP_D8, 1, "torch.float64")][0] == 1
+            assert summ[(_lib.OP_D1, 0, "torch.float64")][0] == 1
+        else:
+            assert list(summ) == [(_lib.OP_SFILTER, 0, "torch.float64")] and summ[list(summ)[0]][0] == 1
+    ctx.close()
