@@ -234,7 +234,9 @@ int pyr_rk_update_member_dt(void* stream, int prec, long long n, const void* F, 
 /* --- fused elementwise kernels for EOM pointwise algebra ----------------
  * No counterpart in the reference (its pointwise algebra is numpy, pyrandaEq.py:39).
  * Source is generated by pyranda_amd/fuse.py: one kernel taking a by-value
- * argument block {long long n; void* p[PYR_PW_MAX_PTRS]; double s[PYR_PW_MAX_SCALARS];}. */
+ * argument block {long long n; void* p[PYR_PW_MAX_PTRS]; double s[PYR_PW_MAX_SCALARS];
+ * int vec;}; the launch sets vec to 1 when every pointer passed is 16-byte
+ * aligned (the generated elementwise loops then move 16-byte vectors). */
 #define PYR_PW_MAX_PTRS 24
 #define PYR_PW_MAX_SCALARS 16
 int pyr_pw_compile(const char* source, const char* kernel_name, void** fn);

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 #include <hip/hiprtc.h>
 
+#include <cstdint>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -37,11 +38,13 @@ std::unordered_map<std::string, Loaded> g_cache;  // keyed by source text
 }  // namespace
 
 // Kernel argument block shared with the generated code (fuse.py writes the
-// same layout): element count, pointers, double scalars.
+// same layout): element count, pointers, double scalars, and whether every
+// pointer is 16-byte aligned (the elementwise loops then use vector accesses).
 struct PyrPwArgs {
   long long n;
   void* p[PYR_PW_MAX_PTRS];
   double s[PYR_PW_MAX_SCALARS];
+  int vec;
 };
 
 extern "C" {
@@ -115,7 +118,11 @@ static int pw_launch(void* fn, void* stream, long long n, long long blocks, int 
   PyrPwArgs a;
   std::memset(&a, 0, sizeof(a));
   a.n = n;
-  for (int i = 0; i < nptr; ++i) a.p[i] = ptrs[i];
+  a.vec = 1;
+  for (int i = 0; i < nptr; ++i) {
+    a.p[i] = ptrs[i];
+    if ((uintptr_t)ptrs[i] % 16 != 0) a.vec = 0;
+  }
   for (int i = 0; i < nscal; ++i) a.s[i] = scal[i];
   void* params[] = {&a};
   if (hipModuleLaunchKernel((hipFunction_t)fn, (unsigned)blocks, 1, 1, PYR_PW_BLOCK, 1, 1, 0,

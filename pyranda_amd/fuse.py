@@ -21,6 +21,7 @@ import ast
 import ctypes
 import hashlib
 import os
+import re
 
 import torch
 
@@ -38,11 +39,62 @@ _CMP = {ast.Lt: "<", ast.Gt: ">", ast.LtE: "<=", ast.GtE: ">=", ast.Eq: "==", as
 GROUP_SIBLINGS = os.environ.get("PYRANDA_AMD_FUSE_GROUP", "1") != "0"
 
 PRELUDE = r"""
-struct PyrPwArgs { long long n; void* p[24]; double s[16]; };
+struct PyrPwArgs { long long n; void* p[24]; double s[16]; int vec; };
 __device__ __forceinline__ double pyr_min(double a, double b) { return (a < b || a != a) ? a : b; }
 __device__ __forceinline__ double pyr_max(double a, double b) { return (a > b || a != a) ? a : b; }
 __device__ __forceinline__ double pyr_sign(double a) { return a != a ? a : (double)((a > 0.0) - (a < 0.0)); }
 """
+
+
+# consecutive elements per thread and iteration of the elementwise loops
+VEC = 4
+_PTR_DECL = re.compile(r"^\s*((?:const )?\w+\* __restrict__ \w+) = (\((?:const )?\w+\*\))(a\.p\[\d+\]);$")
+_LOOP_HEAD = ("  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;"
+              " i += (long long)gridDim.x * blockDim.x) {")
+
+
+def _vectorized(lines):
+    """A generated elementwise kernel (its lines: prelude, signature,
+    declarations, the grid-stride loop _LOOP_HEAD ... "  }", "}") with its
+    loop split in two. With every pointer 16-byte aligned (a.vec, set at
+    launch) each thread takes groups of VEC consecutive elements (group q:
+    elements VEC q ... VEC q + VEC - 1) through a device function whose
+    pointers are __restrict__ parameters: the compiler may then gather each
+    stream's loads and stores of a group into 16-byte accesses (restrict on
+    local pointers does not tell it that streams do not overlap); the
+    elements past the last whole group, or every element without a.vec, go
+    through the plain grid-stride loop."""
+    h = lines.index(_LOOP_HEAD)
+    assert lines[-2] == "  }" and lines[-1] == "}"
+    prelude, sig, decls, body = lines[0], lines[1], lines[2:h], lines[h + 1:-2]
+    kname = re.search(r"void (?:__launch_bounds__\(\d+\) )?(\w+)\(", sig).group(1)
+    ptrs = [_PTR_DECL.match(d) for d in decls]
+    rest = [d for d, m in zip(decls, ptrs) if m is None]
+    ptrs = [m for m in ptrs if m is not None]
+    fn = kname + "_q"
+    params = ", ".join(["const PyrPwArgs& a"] + [m.group(1) for m in ptrs] + ["long long q_"])
+    args = ", ".join(["a"] + ["%s__builtin_assume_aligned(%s, 16)" % (m.group(2), m.group(3)) for m in ptrs]
+                     + ["q_"])
+    ind = lambda ls, k: [" " * k + l.strip() for l in ls]
+    F = ["static __device__ __forceinline__ void %s(%s) {" % (fn, params)]
+    F += ind(rest, 2)
+    F += ["#pragma unroll", "  for (int j_ = 0; j_ < %d; ++j_) {" % VEC,
+          "    const long long i = q_ * %d + j_;" % VEC]
+    F += ind(body, 4)
+    F += ["  }", "}"]
+    K = [sig] + decls
+    K += ["  const long long nthr_ = (long long)gridDim.x * blockDim.x;",
+          "  const long long t_ = (long long)blockIdx.x * blockDim.x + threadIdx.x;",
+          "  long long i_ = t_;",
+          "  if (a.vec) {",
+          "    const long long nq_ = a.n / %d;" % VEC,
+          "    for (long long q_ = t_; q_ < nq_; q_ += nthr_) %s(%s);" % (fn, args),
+          "    i_ = nq_ * %d + t_;" % VEC,
+          "  }",
+          "  for (long long i = i_; i < a.n; i += nthr_) {"]
+    K += ind(body, 4)
+    K += ["  }", "}"]
+    return [prelude] + F + K
 
 
 def _is_numpy_call(node):
@@ -242,9 +294,10 @@ class FusedSubtree:
         RK4 stage update of pyranda.py:800-804: PHI = dt F + A PHI, then the
         state U advanced by B PHI; the arithmetic of k_rk_update
         (pyr_core.hip), explicit FMAs included. variant = (ts, shadow,
-        store, dtp): the field type, an FP64 state advanced in place, the
-        field stored, dt read from device memory."""
-        ts, shadow, store, dtp = variant
+        store, dtp, first): the field type, an FP64 state advanced in place,
+        the field stored, dt read from device memory, A == 0 (the first
+        stage: PHI is written, not read)."""
+        ts, shadow, store, dtp, first = variant
         name = "pyr_rk_" + hashlib.sha1((self.body + repr(sig) + out_dt + repr(variant)).encode()).hexdigest()[:16]
         lines = [PRELUDE, 'extern "C" __global__ void %s(PyrPwArgs a) {' % name]
         ip = isc = 0
@@ -285,7 +338,8 @@ class FusedSubtree:
             if s in ("T64", "T32"):
                 lines.append("    const double x%d = (double)p%d[i];" % (i, i))
         lines.append("    const double f = (double)(%s)(%s);" % (ot, self.body))
-        lines.append("    const double tmp1 = A == 0.0 ? 0.0 : A * phi_[i];")
+        # the first stage (A == 0) neither reads PHI nor lets a stale value in
+        lines.append("    const double tmp1 = %s;" % ("0.0" if first else "A * phi_[i]"))
         lines.append("    const double phi = fma(dt, f, tmp1);")
         lines.append("    phi_[i] = phi;")
         if shadow:
@@ -297,7 +351,7 @@ class FusedSubtree:
             lines.append("    uout[i] = (%s)fma(B, phi, (double)uin[i]);" % tt)
         lines.append("  }")
         lines.append("}")
-        return name, "\n".join(lines)
+        return name, "\n".join(_vectorized(lines))
 
     def rk_kernel(self, sig, out_dt, variant):
         key = ("rk", sig, out_dt, variant)
@@ -345,7 +399,7 @@ class FusedSubtree:
         lines.append("    out[i] = (%s)(%s);" % (ot, self.body))
         lines.append("  }")
         lines.append("}")
-        return name, "\n".join(lines)
+        return name, "\n".join(_vectorized(lines))
 
     def kernel(self, sig, out_dt):
         key = (sig, out_dt)
@@ -502,7 +556,7 @@ class FusedGroup:
             lines.append("    out%d[i] = (%s)(%s);" % (k, "double" if dt == "f64" else "float", body))
         lines.append("  }")
         lines.append("}")
-        return name, "\n".join(lines)
+        return name, "\n".join(_vectorized(lines))
 
     def kernel(self, sig, out_dts):
         key = (sig, out_dts)
@@ -718,7 +772,7 @@ class DeferredF:
         scal = list(self.scalars) + ([] if dtp else [float(dt)]) + [float(A), float(B)]
         if len(ptrs) > _lib.PW_MAX_PTRS or len(scal) > _lib.PW_MAX_SCALARS:
             return False
-        fn = self.sub.rk_kernel(self.sig, self.out_dt, (ts, shadow, store, dtp))
+        fn = self.sub.rk_kernel(self.sig, self.out_dt, (ts, shadow, store, dtp, float(A) == 0.0))
         nx, ny, nz = self.shape
         pp = (ctypes.c_void_p * len(ptrs))(*[t.data_ptr() for t in ptrs])
         sc = (ctypes.c_double * len(scal))(*scal)
@@ -1141,7 +1195,7 @@ class PointwiseRun:
             L.append("    const double l%d = (double)r%d;" % (j, j))
         L.append("  }")
         L.append("}")
-        return name, "\n".join(L)
+        return name, "\n".join(_vectorized(L))
 
     def __call__(self, sim):
         """Evaluate the run on `sim`; False if an input is not a field or scalar

@@ -171,3 +171,33 @@ def test_nan_propagates_like_numpy(dev, dtype):
         np.testing.assert_array_equal(np.isnan(got), np.isnan(ref), err_msg=src)
         ok = ~np.isnan(ref)
         np.testing.assert_array_equal(got[ok], ref[ok], err_msg=src)
+
+
+@pytest.mark.parametrize("dtype", [torch.float64, torch.float32])
+def test_vector_and_scalar_paths_agree(dev, dtype):
+    """The generated elementwise loop takes groups of four elements with
+    16-byte accesses when every pointer is 16-byte aligned and goes element
+    by element otherwise and past the last whole group: a pointwise
+    expression over fields whose length is not a multiple of four, over
+    16-byte aligned storage and over views one element into it, equals the
+    same expression in torch (FP64 arithmetic, one rounding) bit for bit."""
+    from pyranda_amd import devnumpy, fuse
+    g = {"numpy": devnumpy.numpy}
+    src = ('self.variables["a"].data*self.variables["b"].data-self.variables["c"].data'
+           '/(1.0+self.variables["a"].data*self.variables["a"].data)')
+    rhs = fuse.FusedRHS(src, g)
+    gg = torch.Generator().manual_seed(5)
+    for shape in ((37, 11, 5), (64, 16, 4)):
+        n = shape[0] * shape[1] * shape[2]
+        base = {k: torch.rand(n + 1, generator=gg, dtype=torch.float64).to(dev, dtype) for k in "abc"}
+        for off in (0, 1):
+            sim = types.SimpleNamespace(mesh=types.SimpleNamespace(shape=list(shape)), variables={})
+            for k in "abc":
+                t = base[k][off:off + n].view(shape[2], shape[1], shape[0]).permute(2, 1, 0)
+                sim.variables[k] = types.SimpleNamespace(data=t)
+            got = rhs(sim)
+            a, b, c = (sim.variables[k].data.double() for k in "abc")
+            want = (a * b - c / (1.0 + a * a)).to(dtype)
+            torch.cuda.synchronize()
+            _say("vector/scalar paths", dtype, shape, "offset", off, got.data_ptr() % 16)
+            assert torch.equal(got, want), (shape, off)
