@@ -197,7 +197,8 @@ int pyr_rk_update_dt(pyr_ctx* ctx, int prec, long long n, const void* F, double*
  * budget of an x-line workgroup (8..160 KB; it sets the lines per
  * workgroup), yz_pipe switches the pipelined kernels (0/1; on x also 2: the
  * next group of lines prefetched into registers, used by the explicit
- * filter), yz_lines the
+ * filter; on y/z also 3: the plain kernel with runtime row counts, fewer
+ * registers), yz_lines the
  * lines per plain y/z workgroup (16, 32 or 64); -1 keeps the context default.
  * Set by pyranda_amd's autotuner (ParcopContext.autotune), which times the
  * candidates on the device. No counterpart in the reference. */

@@ -1561,7 +1561,10 @@ static int launch_op(pyr_ctx* c, int op, int dir, const TS* in, TS* out, int epi
   if (c->tune_pipe[op][dir][MODE] >= 0) c->yz_pipe = c->tune_pipe[op][dir][MODE];
   else if (dir == 0) c->yz_pipe = 0;  // the pipelined x kernel only where tuning chose it
   if (c->tune_ly[op][dir][MODE] >= 0) c->yz_lines = c->tune_ly[op][dir][MODE];
-  const int r = (PYR_FULL_CHUNKS && pl.N % CHUNK == 0 && sv != SOLVE_LU)
+  // yz_pipe 3 on y/z: the plain kernel with runtime row counts even when
+  // CHUNK | N (fewer registers, more waves per SIMD; the same results)
+  const bool full = PYR_FULL_CHUNKS && pl.N % CHUNK == 0 && sv != SOLVE_LU && !(dir != 0 && c->yz_pipe == 3);
+  const int r = full
                     ? launch_kind<TS, MODE, true>(c, dir, pl.kind, sv, a, in, out)
                     : launch_kind<TS, MODE, false>(c, dir, pl.kind, sv, a, in, out);
   c->x_lds_kb = sx;
@@ -2212,7 +2215,8 @@ int pyr_set_tuning(pyr_ctx* c, int op, int dir, int prec, int x_lds_kb, int yz_p
   if (!c) return fail("null context");
   if (op < 0 || op >= NOPS || dir < 0 || dir > 2) return fail("bad operator or direction");
   if (x_lds_kb >= 0 && (x_lds_kb < 8 || x_lds_kb > 160)) return fail("x_lds_kb must be 8..160 (or -1)");
-  if (yz_pipe > 2 || (yz_pipe == 2 && dir != 0)) return fail("yz_pipe must be 0, 1, 2 for x (or -1)");
+  if (yz_pipe > 3 || (yz_pipe == 2 && dir != 0) || (yz_pipe == 3 && dir == 0))
+    return fail("yz_pipe must be 0, 1, 2 (x) or 3 (y/z) (or -1)");
   if (yz_lines >= 0 && yz_lines != 16 && yz_lines != 32 && yz_lines != 64)
     return fail("yz_lines must be 16, 32 or 64 (or -1)");
   int mode;

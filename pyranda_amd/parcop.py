@@ -128,7 +128,7 @@ class ParcopContext:
         """Per-operator knobs (pyr_set_tuning): LDS budget of x-line
         workgroups (KB), the pipelined kernels (y/z: 1; x: 1 with a second
         LDS tile, 2 with the next tile prefetched into registers, explicit
-        filter only), lines per
+        filter only; y/z 3: the plain kernel with runtime row counts), lines per
         plain y/z workgroup; -1 = default."""
         prec = _lib.F64 if dtype == torch.float64 else _lib.R32
         _lib.check(self._L.pyr_set_tuning(self._h, op, direction, prec, int(x_lds_kb), int(yz_pipe),
@@ -176,7 +176,8 @@ class ParcopContext:
                     cands = [(("x_lds_kb", kb), ("yz_pipe", pp)) for pp in pps for kb in x_lds_kb]
                 else:
                     cands = [(("yz_pipe", 1),), (("yz_pipe", 0), ("yz_lines", 64)),
-                             (("yz_pipe", 0), ("yz_lines", 32)), (("yz_pipe", 0), ("yz_lines", 16))]
+                             (("yz_pipe", 0), ("yz_lines", 32)), (("yz_pipe", 0), ("yz_lines", 16)),
+                             (("yz_pipe", 3), ("yz_lines", 64))]
                 if solve_modes and dtype == torch.float32:
                     cands = [(("r32_solve", m),) + c for m in (0, 1) for c in cands]
                 times = {}

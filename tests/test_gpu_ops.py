@@ -284,9 +284,10 @@ def test_x_kernel_variants_keep_results(dev, nn):
                                       ((40, 72, 50), ("NONE", "NONE", "NONE"))])
 def test_yz_lines_per_workgroup_keep_results(dev, nn, kinds):
     """16, 32 and 64 lines per y/z workgroup (pyr_set_tuning yz_lines, the
-    plain kernel) give the same result bit for bit, FP64 and FP32, periodic
-    (circulant) and bounded (chunked LU) lines, line counts that do not fill
-    the last workgroup included."""
+    plain kernel) and the plain kernel with runtime row counts (yz_pipe 3)
+    give the same result bit for bit, FP64 and FP32, periodic (circulant) and
+    bounded (chunked LU) lines, line counts that do not fill the last
+    workgroup included."""
     from pyranda_amd import _lib
     from pyranda_amd.parcop import ParcopContext
     L = 2 * math.pi
@@ -303,9 +304,13 @@ def test_yz_lines_per_workgroup_keep_results(dev, nn, kinds):
                 for ly in (64, 32, 16):
                     ctx.set_tuning(op, d, dtype, yz_pipe=0, yz_lines=ly)
                     got[ly] = ctx.apply(op, d, v).clone()
+                # the plain kernel with runtime row counts
+                ctx.set_tuning(op, d, dtype, yz_pipe=3, yz_lines=64)
+                got["rt"] = ctx.apply(op, d, v).clone()
                 ctx.set_tuning(op, d, dtype)
                 _say("yz lines", nn, kinds[0], dtype, op, d)
                 assert torch.equal(got[16], got[64]) and torch.equal(got[32], got[64]), (dtype, op, d)
+                assert torch.equal(got["rt"], got[64]), (dtype, op, d)
     ctx.close()
 
 
