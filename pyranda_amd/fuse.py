@@ -843,22 +843,38 @@ class FusedReduce:
         ident, comb = _RED[self.op]
         name = "pyr_red_" + hashlib.sha1((self.op + self.body + repr(sig) + "u%d" % _RED_UNROLL).encode()
                                          ).hexdigest()[:16]
-        L = [PRELUDE, 'extern "C" __global__ void __launch_bounds__(256) %s(PyrPwArgs a) {' % name]
-        for i, t in enumerate(sig):
-            ct = "double" if t == "T64" else "float"
-            L.append("  const %s* __restrict__ p%d = (const %s*)a.p[%d];" % (ct, i, ct, i))
-        for j in range(self.ns):
-            L.append("  const double x%d = a.s[%d];" % (self.nf + j, j))
+        ptrs = ["const %s* __restrict__ p%d" % ("double" if t == "T64" else "float", i) for i, t in enumerate(sig)]
+        scal = ["  const double x%d = a.s[%d];" % (self.nf + j, j) for j in range(self.ns)]
+        body = ["    const double x%d = (double)p%d[i];" % (i, i) for i in range(len(sig))]
+        body += ["    const double v = (%s);" % self.body, "    acc = %s;" % comb]
+        # groups of VEC elements through restrict parameters (16-byte loads
+        # when every pointer is 16-byte aligned, as for the elementwise
+        # kernels); max/min do not depend on the order
+        F = ["static __device__ __forceinline__ double %s_q(%s) {"
+             % (name, ", ".join(["const PyrPwArgs& a"] + ptrs + ["long long q_", "double acc"]))]
+        F += scal
+        F += ["#pragma unroll", "  for (int j_ = 0; j_ < %d; ++j_) {" % VEC,
+              "    const long long i = q_ * %d + j_;" % VEC] + body + ["  }", "  return acc;", "}"]
+        args = ", ".join(["a"] + ["(const %s*)__builtin_assume_aligned(a.p[%d], 16)"
+                                  % ("double" if t == "T64" else "float", i) for i, t in enumerate(sig)]
+                         + ["q_", "acc"])
+        L = [PRELUDE] + F + ['extern "C" __global__ void __launch_bounds__(256) %s(PyrPwArgs a) {' % name]
+        L += ["  %s = (%s)a.p[%d];" % (d, d.split("*")[0] + "*", i) for i, d in enumerate(ptrs)]
+        L += scal
         L.append("  double* __restrict__ part = (double*)a.p[%d];" % len(sig))
         L.append("  double acc = %s;" % ident)
-        # several iterations' loads in flight (max/min do not depend on the order)
+        L += ["  const long long nthr_ = (long long)gridDim.x * blockDim.x;",
+              "  const long long t_ = (long long)blockIdx.x * blockDim.x + threadIdx.x;",
+              "  long long i_ = t_;",
+              "  if (a.vec) {",
+              "    const long long nq_ = a.n / %d;" % VEC,
+              "    for (long long q_ = t_; q_ < nq_; q_ += nthr_) acc = %s_q(%s);" % (name, args),
+              "    i_ = nq_ * %d + t_;" % VEC,
+              "  }"]
+        # several iterations' loads in flight
         L.append("#pragma unroll %d" % _RED_UNROLL)
-        L.append("  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < a.n;"
-                 " i += (long long)gridDim.x * blockDim.x) {")
-        for i in range(len(sig)):
-            L.append("    const double x%d = (double)p%d[i];" % (i, i))
-        L.append("    const double v = (%s);" % self.body)
-        L.append("    acc = %s;" % comb)
+        L.append("  for (long long i = i_; i < a.n; i += nthr_) {")
+        L += body
         L.append("  }")
         L.append("  for (int off = 32; off > 0; off >>= 1) {")
         L.append("    const double v = __shfl_xor(acc, off, 64);")
