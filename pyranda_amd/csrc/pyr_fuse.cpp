@@ -133,9 +133,12 @@ static int pw_launch(void* fn, void* stream, long long n, long long blocks, int 
 
 int pyr_pw_launch(void* fn, void* stream, long long n, int nptr, void* const* ptrs, int nscal,
                   const double* scal) {
-  // grid-stride loop: enough blocks to fill 256 CUs several times over
+  // one group of four elements per thread (the grid-stride loops of the
+  // generated kernels cover what a grid of at most 65535 workgroups does not):
+  // measured faster than a capped, persistent-style grid (8192 workgroups:
+  // +0.36 ms per 256^3 Taylor-Green step; 1024: +0.9 ms)
   long long blocks = (n + PYR_PW_BLOCK * 4 - 1) / (PYR_PW_BLOCK * 4);
-  if (blocks > 8192) blocks = 8192;
+  if (blocks > 65535) blocks = 65535;
   if (blocks < 1) blocks = 1;
   return pw_launch(fn, stream, n, blocks, nptr, ptrs, nscal, scal);
 }
