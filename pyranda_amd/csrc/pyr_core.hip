@@ -395,26 +395,64 @@ __device__ __forceinline__ void line_yz_body(const LineArgs& a, long long blk, c
     if constexpr (LY == 64) buf_store(v, rout, lane_off, row * row_bytes);
     else buf_store(v, rout, lane_off + row * row_bytes, 0);
   };
+  // With fewer than 64 lines per workgroup a wave holds several chunks, so a
+  // row index is not wave-uniform; for whole chunks the row offset splits
+  // into a per-lane base (the chunk's first row, or the first row of a halo)
+  // and a wave-uniform row count, one VGPR in all instead of one per row.
+  constexpr bool SPLIT = LY != 64 && FULL;
+  const int vmid = lane_off + s * row_bytes;
+  // row s + k of this line (0 <= k < CHUNK)
+  auto ldk = [&](__amdgpu_buffer_rsrc_t r, int k) -> TS {
+    if constexpr (SPLIT) return buf_load(TS(), r, vmid, k * row_bytes);
+    else return ld(r, s + k);
+  };
+  auto stk = [&](TS v, int k) {
+    if constexpr (SPLIT) buf_store(v, rout, vmid, k * row_bytes);
+    else st(v, s + k);
+  };
   TS pre[CHUNK];
   if constexpr (PRE == PRE_OLD) {
 #pragma unroll
-    for (int k = 0; k < CHUNK; ++k) pre[k] = k < m ? ld(rout, s + k) : (TS)0;
+    for (int k = 0; k < CHUNK; ++k) pre[k] = k < m ? ldk(rout, k) : (TS)0;
   }
 
   TS win[CHUNK + 2 * R];
+  if constexpr (SPLIT) {
+    // rows s-R .. s-1 (wrapped to N-R .. N-1 for the first chunk of a
+    // periodic line), s .. s+CHUNK-1, s+CHUNK .. s+CHUNK+R-1 (wrapped to
+    // 0 .. R-1 for the last chunk); a bounded line's missing halo rows are 0
+    const bool wrap = PER || a.periodic;
+    const int vlo = lane_off + (s >= R ? s - R : a.N - R) * row_bytes;
+    const int vhi = lane_off + (s + CHUNK < a.N ? s + CHUNK : 0) * row_bytes;
+    const bool lo_ok = act && (wrap || s >= R);
+    const bool hi_ok = act && (wrap || s + CHUNK < a.N);
 #pragma unroll
-  for (int w = 0; w < CHUNK + 2 * R; ++w) {
-    int row = s - R + w;
-    TS v = (TS)0;
-    if (act && w < m + 2 * R) {
-      if (PER || a.periodic) {
-        row = row < 0 ? row + a.N : (row >= a.N ? row - a.N : row);
-        v = ld(rin, row);
-      } else if (row >= 0 && row < a.N) {
-        v = ld(rin, row);
+    for (int w = 0; w < CHUNK + 2 * R; ++w) {
+      TS v = (TS)0;
+      if (w < R) {
+        if (lo_ok) v = buf_load(TS(), rin, vlo, w * row_bytes);
+      } else if (w < CHUNK + R) {
+        if (act) v = buf_load(TS(), rin, vmid, (w - R) * row_bytes);
+      } else {
+        if (hi_ok) v = buf_load(TS(), rin, vhi, (w - CHUNK - R) * row_bytes);
       }
+      win[w] = v;
     }
-    win[w] = v;
+  } else {
+#pragma unroll
+    for (int w = 0; w < CHUNK + 2 * R; ++w) {
+      int row = s - R + w;
+      TS v = (TS)0;
+      if (act && w < m + 2 * R) {
+        if (PER || a.periodic) {
+          row = row < 0 ? row + a.N : (row >= a.N ? row - a.N : row);
+          v = ld(rin, row);
+        } else if (row >= 0 && row < a.N) {
+          v = ld(rin, row);
+        }
+      }
+      win[w] = v;
+    }
   }
   // grouped launches (runtime row counts, longer register live ranges) re-read
   // v when it is added back instead of holding the chunk across the solve
@@ -431,21 +469,22 @@ __device__ __forceinline__ void line_yz_body(const LineArgs& a, long long blk, c
     if (!valid) return;
     double val = y * a.scale;
     if constexpr (HOLDV) val += (double)pre[k];
-    if constexpr (PRE == PRE_V && GRP) val += (double)ld(rin, s + k);
+    if constexpr (PRE == PRE_V && GRP) val += (double)ldk(rin, k);
     if constexpr (PRE == PRE_LATE)
-      if (a.addv) val += (double)ld(rin, s + k);
+      if (a.addv) val += (double)ldk(rin, k);
     TS old = (TS)0;
     if constexpr (PRE == PRE_OLD) old = pre[k];
-    if constexpr (PRE == PRE_LATE) old = (a.epi == EPI_ADD || a.epi == EPI_MAXABS) ? ld(rout, s + k) : (TS)0;
+    if constexpr (PRE == PRE_LATE) old = (a.epi == EPI_ADD || a.epi == EPI_MAXABS) ? ldk(rout, k) : (TS)0;
     // a filter with an accumulating epilogue (only through pyr_apply) reads
     // the output in place
-    if constexpr (PRE == PRE_V) old = (a.epi == EPI_ADD || a.epi == EPI_MAXABS) ? ld(rout, s + k) : (TS)0;
+    if constexpr (PRE == PRE_V) old = (a.epi == EPI_ADD || a.epi == EPI_MAXABS) ? ldk(rout, k) : (TS)0;
     const TS r = epilogue<TS>(a.epi, a.epi_c, val, old);
-    if (sizeof(TS) != 8 || GRP || !a.only32) st(r, s + k);
+    if (sizeof(TS) != 8 || GRP || !a.only32) stk(r, k);
     // (store epilogues only: the accumulating variants keep their registers)
     if constexpr (sizeof(TS) == 8 && PRE != PRE_OLD && !GRP)
       if (a.out32) {
         if constexpr (LY == 64) buf_store((float)r, rout32, lane_off / 2, (s + k) * (row_bytes / 2));
+        else if constexpr (SPLIT) buf_store((float)r, rout32, vmid / 2, k * (row_bytes / 2));
         else buf_store((float)r, rout32, lane_off / 2 + (s + k) * (row_bytes / 2), 0);
       }
   };
