@@ -521,7 +521,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on one GPU only (this build has no multi-GPU path); "
+        print("bench.py runs on one GPU only (pyranda_amd has no multi-GPU path); "
               "got --gpus %d" % args.gpus, file=sys.stderr)
         sys.exit(2)
 

@@ -65,7 +65,7 @@ class CylindricalDer:
     def dd8z(self, v): return self.ctx.dd8z(v)
 
     def dd4x(self, v):
-        raise NotImplementedError("dd4x (explicit 4th derivative e4d4) is outside this build")
+        raise NotImplementedError("dd4x (explicit 4th derivative e4d4) is not implemented in pyranda_amd")
 
     dd4y = dd4z = dd4x
 
@@ -119,7 +119,7 @@ class CylindricalDer:
         return torch.maximum(torch.maximum(a, b.to(a.dtype)), e)
 
     def ringV(self, vx, vy, vz):
-        raise NotImplementedError("ringV on cylindrical meshes is outside this build")
+        raise NotImplementedError("ringV on cylindrical meshes is not implemented in pyranda_amd")
 
 
 class CylindricalSFil:

@@ -809,7 +809,7 @@ class pyrandaSim:
         })
         names = sorted(wVars)
         serial["vars"] = {v: i for i, v in enumerate(names)}
-        # this build's own keys (other readers ignore them): the storage of an
+        # pyranda_amd's own keys (other readers ignore them): the storage of an
         # fp32 run's fields
         serial["storage"] = dict(self._storage)
         _io.write_serial(os.path.join(dump, "serial.dat"), serial)
@@ -1022,7 +1022,7 @@ def pyrandaRestart(rootname, suffix=None, comm=None, **sim_options):
     for pack in serial["packages"]:
         name = pack.split(".")[1] if "." in pack else pack
         if name not in known:
-            raise ValueError("restart names package %s, which this build does not have" % pack)
+            raise ValueError("restart names package %s, which pyranda_amd does not have" % pack)
         pysim.addPackage(known[name](pysim))
     pysim.sMap = dict(serial["sMap"])
     if "storage" in serial and pysim.dtype != torch.float64:
@@ -1037,7 +1037,7 @@ def pyrandaRestart(rootname, suffix=None, comm=None, **sim_options):
     n = (pysim.nx, pysim.ny, pysim.nz)
     decomp = [int(v) for v in serial["decomp"]]
     if decomp != [1, 1, 1]:
-        raise ValueError("restart written by %s ranks; this build reads single-rank restarts" % decomp)
+        raise ValueError("restart written by %s ranks; pyranda_amd reads single-rank restarts" % decomp)
     mm = _io.open_proc_bin(os.path.join(dump, "proc-%s.bin" % str(0).zfill(5)), n, len(names))
     dev = pysim.PyMPI.device
     for v in names:

@@ -2,7 +2,7 @@
 
 extrap: v[0] = 2 v[1] - v[2] (order 2) or v[1]; const: v[0] = val; on the
 x1/xn/y1/yn/z1/zn planes that this block owns. Slip/exit/farfield/curvilinear
-conditions are outside this build and raise.
+conditions are not implemented in pyranda_amd and raise.
 """
 from .pyrandaPackage import pyrandaPackage
 
@@ -90,6 +90,6 @@ class pyrandaBC(pyrandaPackage):
         data[_plane(data, axis, i0)] = field[_plane(field, axis, i0)]
 
     def _unsupported(self, *a, **k):
-        raise NotImplementedError("this boundary condition is outside this build")
+        raise NotImplementedError("this boundary condition is not implemented in pyranda_amd")
 
     exitbc = slipbc = farfieldbc = symm = _unsupported

@@ -28,7 +28,7 @@ class _Der:
     def dd8z(self, v): return self.ctx.dd8z(v)
 
     def dd4x(self, v):
-        raise NotImplementedError("dd4x (explicit 4th derivative e4d4) is outside this build")
+        raise NotImplementedError("dd4x (explicit 4th derivative e4d4) is not implemented in pyranda_amd")
 
     dd4y = dd4z = dd4x
 
