@@ -83,8 +83,8 @@ class pyrandaMPI:
         self.periodic = opts["periodic"]
         symmetric = opts["symmetric"]
         self.coordsys = opts["coordsys"]
-        if self.coordsys not in (0, 1):
-            raise NotImplementedError("Cartesian (coordsys=0) and cylindrical (coordsys=1) meshes are built")
+        if self.coordsys not in (0, 1, 2):
+            raise NotImplementedError("Cartesian (coordsys=0), cylindrical (1) and spherical (2) meshes are built")
         self.comm = comm
         self.master = True
         self.dtype = dtype
@@ -125,6 +125,13 @@ class pyrandaMPI:
             self.metrics = CylindricalMetrics(self.ctx, (self.nx, self.ny, self.nz), self.device)
             self.der = CylindricalDer(self.ctx, self.metrics)
             self.fil = CylindricalSFil(self.ctx, self.metrics)
+        elif self.coordsys == 2:
+            # radius and polar-angle metrics; fbar divides by the filtered
+            # cell volume; gbar (filter 'smooth') stays Cartesian
+            from .spherical import SphericalDer, SphericalMetrics, SphericalSFil
+            self.metrics = SphericalMetrics(self.ctx, (self.nx, self.ny, self.nz), self.device)
+            self.der = SphericalDer(self.ctx, self.metrics)
+            self.fil = SphericalSFil(self.ctx, self.metrics)
 
     # -- allocation (pyrandaMPI.py:249-254) ---------------------------------
     def emptyScalar(self):

@@ -1,5 +1,6 @@
-"""Mesh description and grid (pyranda/pyrandaMesh.py:16-233): Cartesian, or
-cylindrical (coordsys 1, radius/theta/z, see cylindrical.py).
+"""Mesh description and grid (pyranda/pyrandaMesh.py:16-233): Cartesian,
+cylindrical (coordsys 1, radius/theta/z, see cylindrical.py) or spherical
+(coordsys 2, radius/theta/phi, see spherical.py).
 
 `xdom=(x1, xn, n, periodic=True)` lines are executed through the same
 substitution map as the reference (pyrandaMesh.py:33-56). Coordinates follow
@@ -87,10 +88,13 @@ class pyrandaMesh:
         self.d1, self.d2, self.d3 = ctx.dmesh
         self.GridLen = ctx.gridlen
         self.CellVol = ctx.dpatch[0] * ctx.dpatch[1] * ctx.dpatch[2]
-        if mpi.coordsys == 1:
+        if mpi.coordsys in (1, 2):
             # cylindrical (pyrandaMesh.py:152-155, mesh.f90:213-224): the
             # native (radius, theta, z) grid is kept, the mesh coordinates
-            # become x = r cos(theta), y = r sin(theta), z
+            # become x = r cos(theta), y = r sin(theta), z; spherical
+            # (pyrandaMesh.py:156-164, mesh.f90:225-241): native (radius,
+            # theta, phi), x = r sin(theta) cos(phi), y = r sin(theta)
+            # sin(phi), z = r cos(theta)
             m = mpi.metrics
             self.coordsNative = [pyrandaVar("x", "mesh", "scalar"), pyrandaVar("y", "mesh", "scalar"),
                                  pyrandaVar("z", "mesh", "scalar")]
@@ -100,8 +104,17 @@ class pyrandaMesh:
                 cor.data = nat.data
             r, th = X, Y
             cx, cy = torch.empty_like(X), torch.empty_like(Y)     # same (x fastest) layout
-            torch.mul(r, torch.cos(th), out=cx)
-            torch.mul(r, torch.sin(th), out=cy)
+            if mpi.coordsys == 1:
+                torch.mul(r, torch.cos(th), out=cx)
+                torch.mul(r, torch.sin(th), out=cy)
+            else:
+                ph = self.coords[2].data
+                cz = torch.empty_like(X)
+                rs = r * torch.sin(th)
+                torch.mul(rs, torch.cos(ph), out=cx)
+                torch.mul(rs, torch.sin(ph), out=cy)
+                torch.mul(r, torch.cos(th), out=cz)
+                self.coords[2].data = cz
             self.coords[0].data = cx
             self.coords[1].data = cy
             self.d1, self.d2, self.d3 = m.d1, m.d2, m.d3

@@ -17,6 +17,12 @@ _COURANT = fuse.FusedReduce("max", "(((fabs(x0) / x4 + fabs(x1) / x5) + fabs(x2)
 _DIFF = fuse.FusedReduce("min", "((x1 * x2) * x2) / pyr_max(1.0e-12, x0)", 2, 1, "diff")
 
 
+def _uniform(*metrics):
+    """True when every metric is one number (Cartesian meshes); curvilinear
+    meshes carry d2, d3 or GridLen as fields and take the elementwise path."""
+    return all(not isinstance(m, torch.Tensor) or m.numel() == 1 for m in metrics)
+
+
 class pyrandaTimestep(pyrandaPackage):
 
     def __init__(self, pysim):
@@ -36,7 +42,7 @@ class pyrandaTimestep(pyrandaPackage):
 
     def courant(self, u, v, w, c):
         shape = self.pyranda.mesh.shape
-        if fuse.fields_ok(shape, u, v, w, c):
+        if fuse.fields_ok(shape, u, v, w, c) and _uniform(self.dx, self.dy, self.dz, self.GridLen):
             r = _COURANT(shape, [u, v, w, c], [self.dx, self.dy, self.dz, self.GridLen],
                          device=self.pyranda.PyMPI.device_scalars)
             # 1/x on the device is the correctly rounded quotient, as on the host
@@ -48,7 +54,7 @@ class pyrandaTimestep(pyrandaPackage):
     def diff(self, bulk, density):
         delta = self.GridLen
         shape = self.pyranda.mesh.shape
-        if fuse.fields_ok(shape, bulk, density):
+        if fuse.fields_ok(shape, bulk, density) and _uniform(delta):
             return _DIFF(shape, [bulk, density], [delta], device=self.pyranda.PyMPI.device_scalars)
         drate = density * delta * delta / numpy.maximum(1.0e-12, bulk)
         return self.pyranda.PyMPI.min3D(drate)

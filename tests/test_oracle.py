@@ -617,3 +617,80 @@ def test_cylindrical_closed_forms(oracle_mod):
     assert np.abs(wrong - 3 * r).max() > 1e-3
     flat = C.P.apply(po.SF, 0, r) / r                                # no flipped parity
     assert np.abs(flat - 1).max() > 1e-4
+
+
+# --- spherical compositions (oracle/cyl_cpu.py SphOracle) ---------------------
+def _sph(oracle_mod, nn=(48, 40, 32)):
+    """(r, theta, phi) block away from the origin and the poles, phi
+    periodic over the full circle."""
+    from oracle.cyl_cpu import SphOracle
+    nx, ny, nz = nn
+    dom = [(0.5, 1.5, nx), (0.3, 1.2, ny), (0.0, 2 * np.pi * (nz - 1) / nz, nz)]
+    (R, T, F), dd = mesh(dom)
+    P = oracle_mod.Oracle(nn, dd, (False, False, True))
+    return SphOracle(P, R, T, dd), R, T, F
+
+
+def test_spherical_closed_forms(oracle_mod):
+    """The spherical compositions (operators.f90:64-76, 148-180, 198-202,
+    545-559) on fields whose results are known: div(1, 0, 0) = 2/r,
+    div(0, theta/sin(theta), 0) = 1/(r sin(theta)), div(0, 0, sin(phi)) =
+    cos(phi)/(r sin(theta)); grad(r^2) = (2r, 0, 0), grad(theta) = (0, 1/r,
+    0); lap(r) = 2/r, lap(cos(phi)) = -cos(phi)/(r sin(theta))^2, lap(theta) =
+    cot(theta)/r^2; the divergence of the identity tensor is zero and that of
+    (1 + theta) I is (0, 1/r, 0), which needs every curvature term of divT.
+    A dropped weight or curvature term moves these by O(1)."""
+    S, r, t, f = _sph(oracle_mod)
+    z = 0 * r
+    s = np.sin(t)
+
+    def small(a, tol):
+        assert np.abs(a).max() < tol, np.abs(a).max()
+
+    small(S.div(1 + z, z, z) - 2 / r, 1e-12)
+    small(S.div(z, t / s, z) - 1 / (r * s), 1e-12)
+    small(S.div(z, z, np.sin(f)) - np.cos(f) / (r * s), 1e-10)
+    g = S.grad(r ** 2)
+    small(g[0] - 2 * r, 1e-12); small(g[1], 1e-14); small(g[2], 1e-14)
+    g = S.grad(t)
+    small(g[0], 1e-14); small(g[1] - 1 / r, 1e-12); small(g[2], 1e-14)
+    small(S.laplacian(r) - 2 / r, 1e-10)
+    small(S.laplacian(np.cos(f)) + np.cos(f) / (r * s) ** 2, 1e-10)
+    small(S.laplacian(t) - np.cos(t) / (r ** 2 * s), 1e-4)
+    one = 1 + z
+    for d in S.divT(one, z, z, z, one, z, z, z, one):
+        small(d, 1e-4)
+    p = 1 + t
+    d = S.divT(p, z, z, z, p, z, z, z, p)
+    small(d[0], 1e-12); small(d[1] - 1 / r, 1e-3); small(d[2], 1e-14)
+    # the planted slips fail: the r^2 weight of the radial flux dropped, the
+    # 1/tan(theta) term of the theta row dropped
+    assert np.abs(S.P.ddx(1 + z) - 2 / r).max() > 1.0
+    noc = S.divT(one, z, z, z, one, z, z, z, 0 * one)[1]
+    assert np.abs(noc).max() > 0.1
+
+
+def test_spherical_filter_volume_weighting(oracle_mod):
+    """fbar on a spherical mesh (operators.f90:889-894, mesh.f90:374-382):
+    conservative form -- fbar(c) = c exactly; fbar(1/V) = 1/sf(V), close to
+    1/V since the one-dimensional cell volume 4 pi r^2 dr is a quadratic the
+    filter nearly keeps, while the unweighted filter of 1/V is 30 times
+    further off near the origin; 3-D: the filtered volume is the x -> y -> z
+    sharp filter of d1 d2 d3."""
+    from oracle import pyoracle as po
+    from oracle.cyl_cpu import SphOracle
+    n = 64
+    dom = [(1.0 / n, 1.0, n), (np.pi / 2 - .01, np.pi / 2 + .01, 1), (-np.pi / 16, np.pi / 16, 1)]
+    (R, T, F), dd = mesh(dom)
+    P = oracle_mod.Oracle((n, 1, 1), dd, (False, False, True))
+    S = SphOracle(P, R, T, dd)
+    assert np.allclose(S.vol, dd[0] * 4 * np.pi * R ** 2, rtol=1e-15, atol=0)
+    assert np.abs(S.fbar(3.0 + 0 * R) - 3.0).max() < 1e-14
+    inv = 1 / S.vol
+    err = np.abs(S.fbar(inv) - inv).max() / inv.max()
+    slip = np.abs(P.apply(po.SF, 0, np.asfortranarray(inv)) - inv).max() / inv.max()
+    assert err < 2e-4 and slip > 10 * err
+    S3, r, t, f = _sph(oracle_mod, (24, 20, 16))
+    v = S3.d1 * S3.d2 * S3.d3
+    assert np.allclose(S3.vol, v, rtol=1e-15, atol=0)
+    assert np.abs(S3.vol_s - v).max() / v.max() < 1e-5
