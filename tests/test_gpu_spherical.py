@@ -87,6 +87,53 @@ def test_spherical_operators_match_oracle(pr, oracle_mod, nn):
     assert np.allclose(Z, R * np.cos(T), rtol=0, atol=1e-14)
 
 
+SPH_EOM_2D = """
+ddt(:rho:)  =  -:mass:
+ddt(:rhou:) =  -:xmom:
+ddt(:rhow:) =  -:zmom:
+ddt(:Et:)   =  -div( (:Et: + :p: - :tau:)*:u:,:v:, (:Et: + :p: - :tau:)*:w: )
+:rho:       =  fbar( :rho:  )
+:rhou:      =  fbar( :rhou: )
+:rhow:      =  fbar( :rhow: )
+:Et:        =  fbar( :Et:   )
+:u:         =  :rhou: / :rho:
+:w:         =  :rhow: / :rho:
+:p:         =  ( :Et: - .5*:rho:*(:u:*:u: + :w:*:w:) ) * ( :gamma: - 1.0 )
+:div:       =  div(:u:,:v:,:w:)
+:beta:      =  gbar(abs(ring(:div:))) * :rho: * 7.0e-2
+:tau:       =  :beta:*:div:
+:v:         = :v: * 0.0
+:mass: = div(:rho:*:u:               , :v:, :rho:*:w:)
+[:fxx:,:fxy:,:fxz:] = [:rhou:*:u: + :p: - :tau:, :v:       , :rhou:*:w:]
+[:fyx:,:fyy:,:fyz:] = [      :v:               , :p:-:tau: , :v:       ]
+[:fzx:,:fzy:,:fzz:] = [:rhow:*:u:,               :v:       , :rhow:*:w: + :p: - :tau:]
+[:xmom:,:ymom:,:zmom:] = divT(:fxx:,:fxy:,:fxz:,:fyx:,:fyy:,:fyz:,:fzx:,:fzy:,:fzz:)
+bc.const(['u'],['x1','xn'],0.0)
+:cs:  = sqrt( :p: / :rho: * :gamma: )
+:dt: = dt.courant(:u:,:v:,:w:,:cs:)
+"""
+
+SPH_IC = """rad = sqrt( (meshx-pi)**2  +  (meshy-0.0)**2 )
+    :gamma: = 1.4
+    wgt = .5*(1.0-tanh( (rad-pi/16.0)/.1) )
+    :Et:  = 1.0/(:gamma:-1.0) * (wgt*.9 + .1)
+    :rho: = 1.0*wgt + (1.0-wgt)*.125
+    """
+
+
+def _sph_run(ss, tt, dt_max):
+    """The examples' time loop: dt from the dt variable, capped at dt_max
+    and at the time left."""
+    time, dt = 0.0, dt_max
+    steps = 0
+    while tt > time:
+        time = ss.rk4(time, dt)
+        dt = min(dt_max, float(ss.var("dt").data))
+        dt = min(dt, (tt - time))
+        steps += 1
+    return steps
+
+
 SPH_EOM = """
 ddt(:rho:)  =  -:mass:
 ddt(:rhou:) =  -:xmom:
@@ -136,26 +183,42 @@ def test_euler_spherical_1d_curve(pr):
     ss.addPackage(pr.pyrandaBC(ss))
     ss.addPackage(pr.pyrandaTimestep(ss))
     ss.EOM(SPH_EOM)
-    ss.setIC("""rad = sqrt( (meshx-pi)**2  +  (meshy-0.0)**2 )
-    :gamma: = 1.4
-    wgt = .5*(1.0-tanh( (rad-pi/16.0)/.1) )
-    :Et:  = 1.0/(:gamma:-1.0) * (wgt*.9 + .1)
-    :rho: = 1.0*wgt + (1.0-wgt)*.125
-    """)
-    tt = 0.1
-    dt_max = 1.0 / ss.mesh.nn[0] * 0.05
-    time, dt = 0.0, dt_max
-    steps = 0
-    while tt > time:
-        time = ss.rk4(time, dt)
-        dt = min(dt_max, float(ss.var("dt").data))
-        dt = min(dt, (tt - time))
-        steps += 1
+    ss.setIC(SPH_IC)
+    steps = _sph_run(ss, 0.1, 1.0 / ss.mesh.nn[0] * 0.05)
     x = ss.mesh.coordsNative[0].data[:, 0, 0].cpu().numpy()
     p = ss.var("p").data[:, 0, 0].double().cpu().numpy()
     base = numpy.loadtxt(os.path.join(GOLDEN, "euler-SPH-1D.dat"))
     assert numpy.allclose(x, base[0], rtol=0, atol=1e-14)
     err = numpy.abs(p - base[1]).max() / numpy.abs(base[1]).max()
     _say("euler-SPH-1D", steps, "steps, max rel diff", err)
+    # the reference harness accepts 1e-4 (tests/testObj.py:52-58)
+    assert err <= 1e-9
+
+
+def test_euler_spherical_2d_curve(pr):
+    """examples/eulerSpherical.py at 32 (a 128 x 1 x 32 (r, theta, phi)
+    mesh, phi periodic over pi/8: the azimuthal terms of div and divT and
+    the three-direction filtered cell volume) against the reference-held
+    curve tests/baselines/euler-SPH-2D.dat (rho averaged over phi at t =
+    0.1)."""
+    import numpy
+    npts = 32
+    L = numpy.pi * 2.0
+    spike = numpy.pi / 16
+    opts = {"coordsys": 2, "periodic": numpy.array([False, False, True]),
+            "x1": [L / 10, numpy.pi / 2 - .01, -spike], "xn": [L, numpy.pi / 2 + .01, spike],
+            "nn": [npts * 4, 1, npts]}
+    ss = pr.pyrandaSim("2dSpherical", opts, silent=True)
+    ss.addPackage(pr.pyrandaBC(ss))
+    ss.addPackage(pr.pyrandaTimestep(ss))
+    ss.EOM(SPH_EOM_2D)
+    ss.setIC(SPH_IC)
+    steps = _sph_run(ss, 0.1, 1.0 / ss.mesh.nn[0] * 0.05)
+    x = ss.mesh.coordsNative[0].data[:, 0, npts // 2].cpu().numpy()
+    rho = ss.var("rho").data[:, 0, :].double().cpu().numpy().mean(axis=1)
+    base = numpy.loadtxt(os.path.join(GOLDEN, "euler-SPH-2D.dat"))
+    assert numpy.allclose(x, base[0], rtol=0, atol=1e-14)
+    err = numpy.abs(rho - base[1]).max() / numpy.abs(base[1]).max()
+    _say("euler-SPH-2D", steps, "steps, max rel diff", err)
     # the reference harness accepts 1e-4 (tests/testObj.py:52-58)
     assert err <= 1e-9
