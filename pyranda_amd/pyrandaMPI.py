@@ -185,6 +185,17 @@ class pyrandaMPI:
 
     xbar, ybar, zbar = xsum, ysum, zsum
 
+    # reductions over two directions (pyrandaMPI.py:350-357): one value per
+    # index of the remaining direction
+    def xysum(self, data):
+        return torch.sum(data, dim=(0, 1), dtype=torch.float64).cpu().numpy()
+
+    def xzsum(self, data):
+        return torch.sum(data, dim=(0, 2), dtype=torch.float64).cpu().numpy()
+
+    def yzsum(self, data):
+        return torch.sum(data, dim=(1, 2), dtype=torch.float64).cpu().numpy()
+
     def subsum3xz(self, data):
         return torch.sum(data, dim=(0, 2), dtype=torch.float64).cpu().numpy()
 
