@@ -9,9 +9,10 @@ libpyranda_amd.so through the C-ABI in include/pyranda_amd.h.
 from .pyranda import pyrandaSim, pyrandaRestart
 from .pyrandaTimestep import pyrandaTimestep
 from .pyrandaBC import pyrandaBC
+from .pyrandaIBM import pyrandaIBM
 from .pyrandaPackage import pyrandaPackage
 from .pyrandaMesh import defaultMeshOptions
 from .group import pyrandaGroupSim
 
-__all__ = ["pyrandaSim", "pyrandaRestart", "pyrandaTimestep", "pyrandaBC", "pyrandaPackage", "defaultMeshOptions",
+__all__ = ["pyrandaSim", "pyrandaRestart", "pyrandaTimestep", "pyrandaBC", "pyrandaIBM", "pyrandaPackage", "defaultMeshOptions",
            "pyrandaGroupSim"]
