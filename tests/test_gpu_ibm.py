@@ -115,3 +115,86 @@ def test_cylinder_curve(pr, npts):
     # and per-operation rounding differences enter at every stage; measured
     # against the reference 7.4e-10 (32) and 1.1e-9 (64).
     assert err <= 1e-8
+
+
+MM_EOM = """
+ddt(:rho:)  =  -ddx(:rho:*:u: - :tau:) - ddy(:rho:*:v: - :tau:)
+ddt(:rhoA:)  =  -ddx(:rhoA:*:uA: - :tauA:) -ddy(:rhoA:*:vA: - :tauA:)
+ddt(:phi:)  =  - :gx: * :u1: - :gy: * :v1:
+:rho:       =  fbar( :rho:  )
+:rhoA:      =  fbar( :rhoA:  )
+:dr:        = ddx(:rho:)
+:drA:       = ddx(:rhoA:)
+:tau:       = 5.0e-7*gbar(abs(lap(lap(:rho:)))) * :dx4: * :u: * :dr: * :dx:
+:tauA:      = 5.0e-7*gbar(abs(lap(lap(:rhoA:)))) * :dx4: * :uA: * :drA: * :dx:
+[:gx:,:gy:,:gz:] = grad( :phi: )
+:mgp:  =  sqrt( :gx:**2 + :gy:**2 + :gz:**2 )
+:rho:         = ibmS( :rho: , :phi:, [:gx:,:gy:,:gz:] )
+:rhoA:         = ibmS( :rhoA: , -:phi:, [-:gx:,-:gy:,-:gz:] )
+:uT:        =  where( :phi: > 0.0, :u:, :uA:)
+:rhoB:        =  where( :phi: > 0.0, :rho:, :rhoA:)
+:rhoT:        =  where( abs(:phi:/:dx:*2.0) > 1.0, :rhoB:, .5*(:phi:/:dx:*2.0 + 1.0)*(:rho:-:rhoA:) + (:rhoA:)   )
+"""
+
+
+def _golden(name):
+    import json
+    with open(os.path.join(GOLDEN, "reference_scalars.json")) as f:
+        return json.load(f)[name]["value"]
+
+
+@pytest.mark.parametrize("diffusive", [False, True])
+@pytest.mark.parametrize("npts", [50, 100, 200])
+def test_interface_advection(pr, npts, diffusive):
+    """examples/interface.py in its 1-D test form: two materials on either
+    side of a level set advected at unit speed, each extended across the
+    interface by ibmS (the second with the reversed level set), hyper-
+    diffusion from lap(lap(.)); the minimum of the blended density over the
+    right half at t = 0.2 L against MM-adv1d-<n>[-d]
+    (tests/cases/testMM_simple.py:3-8)."""
+    L = np.pi * 2.0
+    lp = L * (npts - 1.0) / npts
+    opts = {"coordsys": 0, "periodic": np.array([False, False, True]), "dim": 3,
+            "x1": [0.0, 0.0, 0.0], "xn": [lp, lp, lp], "nn": [npts, 1, 1]}
+    ss = pr.pyrandaSim("advection", opts, silent=True)
+    ss.addPackage(pr.pyrandaIBM(ss))
+    ss.EOM(MM_EOM)
+    x = ss.mesh.coords[0].data
+    amp = 1.0e6 if diffusive else 1.0
+    V = ss.variables
+    V["rho"].data += ss.gfilter(torch.where(x < np.pi * .25, amp, 1.0))
+    V["rhoA"].data += 1.0e6 * (1.0 + .1 * torch.exp(-(x - np.pi * 1.5) ** 2 / .01))
+    V["phi"].data = 3.14159 - x
+    V["uA"].data += 1.0
+    V["u"].data += 1.0
+    V["u1"].data += 1.0
+    V["u1"].data = 1.0
+    V["dx"].data += (x[1, 0, 0] - x[0, 0, 0])
+    V["dx4"].data += (x[1, 0, 0] - x[0, 0, 0]) ** 4
+    ss.updateVars()
+    time = 0.0
+    dt_max = 1.0 / ss.mesh.nn[0] * L * .25
+    tt = L / 1.0 * 1.0 * .2
+    dt = dt_max
+    steps = 0
+    while tt > time:
+        time = ss.rk4(time, dt)
+        dt = min(dt_max, (tt - time))
+        V["rho"].data[0, :, :] = amp
+        steps += 1
+    v2 = ss.PyMPI.zbar(V["rhoT"].data)
+    got = np.min(v2[npts // 2:, 0])
+    name = "MM-adv1d-%d%s" % (npts, "-d" if diffusive else "")
+    want = _golden(name)
+    err = abs(got - want) / abs(want)
+    print(name, steps, "steps:", repr(got), "reference", want, "rel diff", err, flush=True)
+    if diffusive and npts < 200:
+        # the minimum is a small difference of 1e6-scale densities: measured
+        # 1.3e-6 (50) and 2.6e-5 (100) relative, 4e-11 and 2e-11 of the
+        # 1e6 field scale; held to the reference harness's own bar (1e-4,
+        # tests/testObj.py:52-58), the cause of the excess over rounding is
+        # not identified
+        assert err <= 1e-4 and abs(got - want) <= 1e-10 * amp
+    else:
+        # the reference holds 12 significant digits
+        assert err <= 1e-9
