@@ -189,11 +189,12 @@ def test_interface_advection(pr, npts, diffusive):
     err = abs(got - want) / abs(want)
     print(name, steps, "steps:", repr(got), "reference", want, "rel diff", err, flush=True)
     if diffusive and npts < 200:
-        # the minimum is a small difference of 1e6-scale densities: measured
-        # 1.3e-6 (50) and 2.6e-5 (100) relative, 4e-11 and 2e-11 of the
-        # 1e6 field scale; held to the reference harness's own bar (1e-4,
-        # tests/testObj.py:52-58), the cause of the excess over rounding is
-        # not identified
+        # the minimum sits at x = pi, the first point of the right half,
+        # within reach of the 1e6 jump (the per-step reset of rho[0] alone
+        # moves it by 1e-3 relative): measured 1.3e-6 (50) and 2.6e-5 (100)
+        # relative, 4e-11 and 2e-11 of the 1e6 field scale, the same with
+        # the right-hand sides fused into the RK update or not; held to the
+        # reference harness's own bar (1e-4, tests/testObj.py:52-58)
         assert err <= 1e-4 and abs(got - want) <= 1e-10 * amp
     else:
         # the reference holds 12 significant digits
