@@ -35,12 +35,14 @@ class SphericalMetrics:
         t1 = torch.tensor(ctx.coords1d[1], dtype=torch.float64, device=device)
         self.radius = r1[None, None, :].expand(nz, ny, nx).contiguous().permute(2, 1, 0)
         self.theta = t1[None, :, None].expand(nz, ny, nx).contiguous().permute(2, 1, 0)
-        if float(r1.abs().min()) == 0.0:
+        dx, dy, dz = ctx.dpatch
+        # the grid is x1 - dx/2 + (i - 1/2) dx, so a point meant at 0 lands
+        # within rounding of it: compare against the spacing
+        if float(r1.abs().min()) <= 1e-9 * abs(dx):
             raise ValueError("spherical mesh: a point at the origin (radius 0)")
         sin_t = torch.sin(self.theta)
-        if float(sin_t.abs().min()) == 0.0:
+        if float(torch.sin(t1).abs().min()) <= 1e-9 * abs(dy):
             raise ValueError("spherical mesh: a point on the polar axis (sin(theta) = 0)")
-        dx, dy, dz = ctx.dpatch
         self.dx = dx
         self.d1 = dx
         self.d2 = self.radius * dy

@@ -222,3 +222,17 @@ def test_euler_spherical_2d_curve(pr):
     _say("euler-SPH-2D", steps, "steps, max rel diff", err)
     # the reference harness accepts 1e-4 (tests/testObj.py:52-58)
     assert err <= 1e-9
+
+
+@pytest.mark.parametrize("where", ["origin", "pole"])
+def test_spherical_mesh_rejects_singular_points(pr, where):
+    """A mesh point at r = 0 or on the polar axis (sin(theta) = 0) would
+    divide by zero in the metrics; set-up raises instead."""
+    if where == "origin":
+        x1, xn = [0.0, 0.3, 0.0], [1.0, 1.2, 1.0]
+    else:
+        x1, xn = [0.5, 0.0, 0.0], [1.5, 1.2, 1.0]
+    opts = {"coordsys": 2, "periodic": np.array([False, False, True]),
+            "x1": x1, "xn": xn, "nn": [16, 12, 8]}
+    with pytest.raises(ValueError):
+        pr.pyrandaSim("sph_bad", opts, silent=True)
