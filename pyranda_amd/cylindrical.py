@@ -31,10 +31,12 @@ class CylindricalMetrics:
         nx, ny, nz = nn
         r1 = torch.tensor(ctx.coords1d[0], dtype=torch.float64, device=device)
         self.radius = r1[None, None, :].expand(nz, ny, nx).contiguous().permute(2, 1, 0)
-        if float(r1.abs().min()) == 0.0:
+        dx, dy, dz = ctx.dpatch
+        # a point meant at r = 0 lands within rounding of it (the grid is
+        # x1 - dx/2 + (i - 1/2) dx): compare against the spacing
+        if float(r1.abs().min()) <= 1e-9 * abs(dx):
             raise ValueError("cylindrical mesh: a point on the axis (radius 0)")
         self.iR = 1.0 / self.radius
-        dx, dy, dz = ctx.dpatch
         self.d1 = dx
         self.d2 = self.radius * dy
         self.d3 = dz

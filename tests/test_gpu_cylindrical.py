@@ -71,3 +71,12 @@ def test_cylindrical_operators_match_oracle(pr, oracle_mod, nn, x1):
     close(der.laplacian(fd[4]), C.laplacian(f[4]), "lap")
     close(der.ring(fd[5]), C.ring(f[5]), "ring")
     close(fil.filter(fd[6]), C.fbar(f[6]), "fbar")
+
+
+def test_cylindrical_mesh_rejects_axis_point(pr):
+    """A mesh point on the axis (r = 0, placed within rounding of it by the
+    grid formula) would divide by zero in the metrics; set-up raises."""
+    opts = {"coordsys": 1, "periodic": np.array([False, False, True]), "x1": [0.0, 0.0, 0.0],
+            "xn": [1.0, 0.5, 1.0], "nn": [16, 8, 8]}
+    with pytest.raises(ValueError):
+        pr.pyrandaSim("cyl_bad", opts, silent=True)
